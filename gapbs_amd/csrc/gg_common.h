@@ -1,0 +1,253 @@
+// Shared pieces of the gfx950 kernel library: error handling, the device
+// graph handle with its cached workspaces, and wave64 primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gapgpu.h"
+
+namespace gg {
+
+// ---------------------------------------------------------------------------
+// Errors: every C-ABI entry point runs its body inside Guard(), which turns
+// these exceptions into a negative status and a thread-local message.
+struct Failure : std::runtime_error {
+  int status;
+  Failure(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+void SetError(const std::string& msg);
+
+#define GG_HIP(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::gg::Failure(e_ == hipErrorOutOfMemory ? GG_ENOMEM : GG_EHIP,                 \
+                          std::string(#expr) + ": " + hipGetErrorString(e_) + " at " +     \
+                              __FILE__ + ":" + std::to_string(__LINE__));                  \
+  } while (0)
+
+#define GG_REQUIRE(cond, status, msg)                  \
+  do {                                                 \
+    if (!(cond)) throw ::gg::Failure((status), (msg)); \
+  } while (0)
+
+template <class F>
+int Guard(F&& f) {
+  try {
+    f();
+    return GG_OK;
+  } catch (const Failure& e) {
+    SetError(e.what());
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    SetError("host allocation failed");
+    return GG_ENOMEM;
+  } catch (const std::exception& e) {
+    SetError(e.what());
+    return GG_EHIP;
+  }
+}
+
+// Device allocation with the optional GAPGPU_MEM_CAP_BYTES cap (lets tests
+// drive the out-of-memory path from the host).
+void* DeviceAlloc(size_t bytes);
+void DeviceFree(void* p, size_t bytes);
+
+// ---------------------------------------------------------------------------
+// Device graph handle.
+struct Workspace {
+  struct Slot {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Slot> slots;
+  // Returns a buffer of at least `bytes`, reallocating if the cached one is
+  // smaller.  Contents are undefined.
+  void* Get(const std::string& name, size_t bytes);
+  void Release();
+  size_t Bytes() const;
+};
+
+}  // namespace gg
+
+struct gg_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  int64_t n = 0;
+  int64_t out_entries = 0, in_entries = 0;
+  bool directed = false, weighted = false;
+  int64_t* out_off = nullptr;
+  int32_t* out_dst = nullptr;  // int32 ids, or {v, w} pairs when weighted
+  int64_t* in_off = nullptr;   // == out_off for undirected graphs
+  int32_t* in_dst = nullptr;
+  size_t graph_bytes = 0;
+  gg::Workspace ws;
+  void* pinned = nullptr;  // small pinned host block for counter read-back
+  size_t pinned_bytes = 0;
+  int cu_count = 256;
+
+  // Per-kernel profiling (gg_profile_*): launches bracketed by events.
+  struct KernelTotals {
+    int64_t launches = 0;
+    double ms = 0;
+    double bytes = 0;
+  };
+  struct PendingLaunch {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  bool profile = false;
+  std::map<std::string, KernelTotals> kernel_totals;
+  std::vector<PendingLaunch> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  template <class T>
+  T* Scratch(const char* name, size_t count) {
+    return static_cast<T*>(ws.Get(name, count * sizeof(T)));
+  }
+  template <class T>
+  T* Pinned(size_t count);
+};
+
+namespace gg {
+
+// Host wall clock for gg_stats::total_ms.
+struct WallTimer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+inline void StatsReset(gg_stats* s) {
+  if (s) std::memset(s, 0, sizeof(*s));
+}
+
+// Records the start event; FinishTiming records stop, waits for the stream,
+// fills device_ms and settles pending profiled launches.
+void BeginTiming(gg_graph* g);
+void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt);
+
+// Runs `launch` (which enqueues one kernel on g->stream).  When profiling is
+// on, brackets it with events and returns the index of its record, so the
+// caller can credit algorithmic bytes known only after the kernel ran
+// (CreditBytes); returns -1 otherwise.
+int ProfileBegin(gg_graph* g, const char* name, double bytes);
+void ProfileEnd(gg_graph* g, int idx);
+void CreditBytes(gg_graph* g, int idx, double bytes);
+// Folds finished launches into the per-kernel totals (stream must be idle).
+void SettleProfile(gg_graph* g);
+
+template <class F>
+int Launch(gg_graph* g, const char* name, double bytes, F&& launch) {
+  const int idx = ProfileBegin(g, name, bytes);
+  launch();
+  ProfileEnd(g, idx);
+  return idx;
+}
+
+inline unsigned GridFor(int64_t items, int block, unsigned cap = 1u << 20) {
+  int64_t b = (items + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > static_cast<int64_t>(cap)) b = cap;
+  return static_cast<unsigned>(b);
+}
+
+// ---------------------------------------------------------------------------
+// wave64 device primitives.
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int LaneId() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t LanesBelow(uint64_t mask) {
+  return mask & ((uint64_t(1) << LaneId()) - 1);
+}
+
+// Wave-aggregated append: every active lane calls it together; lanes with
+// `take` get consecutive slots in `out` after one atomic per wave.
+template <class T>
+__device__ __forceinline__ void WaveAppend(bool take, T value, T* out, int32_t* tail) {
+  const uint64_t m = __ballot(take);
+  if (m == 0) return;
+  const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+  int base = 0;
+  if (LaneId() == leader) base = atomicAdd(tail, __popcll(m));
+  base = __shfl(base, leader);
+  if (take) out[base + __popcll(LanesBelow(m))] = value;
+}
+
+template <class T>
+__device__ __forceinline__ T WaveSum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Inclusive scan within a wave.
+__device__ __forceinline__ int WaveInclusiveScan(int v) {
+  const int lane = LaneId();
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one int per thread; returns the prefix and
+// writes the block total.  `sh` needs BS/64 ints of shared memory.
+template <int BS>
+__device__ __forceinline__ int BlockExclusiveScan(int v, int* total, int* sh) {
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  const int inc = WaveInclusiveScan(v);
+  if (lane == kWave - 1) sh[wid] = inc;
+  __syncthreads();
+  int wave_base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < BS / kWave; w++) {
+    const int s = sh[w];
+    if (w < wid) wave_base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_base + inc - v;
+}
+
+// Block-wide sum, result valid in every thread.  `sh` needs BS/64 slots.
+template <int BS, class T>
+__device__ __forceinline__ T BlockSum(T v, T* sh) {
+  v = WaveSum(v);
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  T t = 0;
+#pragma unroll
+  for (int w = 0; w < BS / kWave; w++) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+}  // namespace gg
+
+template <class T>
+T* gg_graph::Pinned(size_t count) {
+  const size_t bytes = count * sizeof(T);
+  if (bytes > pinned_bytes) {
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_bytes = 0;
+    GG_HIP(hipHostMalloc(&pinned, bytes < 4096 ? 4096 : bytes, hipHostMallocDefault));
+    pinned_bytes = bytes < 4096 ? 4096 : bytes;
+  }
+  return static_cast<T*>(pinned);
+}

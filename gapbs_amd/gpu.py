@@ -1,0 +1,186 @@
+"""Python face of the gfx950 kernel library (libgapgpu.so, include/gapgpu.h).
+
+The functions named like the reference's kernels take the same arguments and
+return the same result arrays as the reference functions they replace:
+
+    DOBFS(g, source, alpha=15, beta=18)      src/bfs.cc:122-179   -> parent[n]
+
+A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
+call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
+present the calls raise -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .graph import Graph
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_INIT_DEVICE = None
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("device_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("iterations", C.c_int64),
+        ("td_steps", C.c_int64),
+        ("bu_steps", C.c_int64),
+        ("reached", C.c_int64),
+        ("reached_edges", C.c_int64),
+        ("edges_examined", C.c_int64),
+        ("vertices_examined", C.c_int64),
+        ("residual", C.c_double),
+        ("launches", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTED = [
+    "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
+    "gg_bfs", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+]
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libgapgpu.so")
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.gg_init.argtypes = [C.c_int]
+        L.gg_graph_upload.argtypes = [C.c_int64, _i64p, _i32p, C.c_int, C.c_int, _i64p, _i32p,
+                                      C.POINTER(vp)]
+        L.gg_graph_free.argtypes = [vp]
+        L.gg_graph_device_bytes.argtypes = [vp]
+        L.gg_graph_device_bytes.restype = C.c_int64
+        L.gg_bfs.argtypes = [vp, C.c_int32, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(Stats)]
+        L.gg_profile_enable.argtypes = [vp, C.c_int]
+        L.gg_profile_reset.argtypes = [vp]
+        L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gg_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise GpuError(rc, lib().gg_last_error().decode())
+    return rc
+
+
+def init(device: int = 0) -> None:
+    global _INIT_DEVICE
+    if _INIT_DEVICE != device:
+        _check(lib().gg_init(device))
+        _INIT_DEVICE = device
+
+
+def _p(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class DeviceGraph:
+    """A graph copied into HBM (int64 offsets, int32 ids, AoS weights)."""
+
+    def __init__(self, g: Graph, device: int = 0):
+        init(device)
+        self.host = g
+        self.num_nodes = g.num_nodes
+        self.weighted = g.weighted
+        self.directed = g.directed
+        h = C.c_void_p()
+        off = g.out_offsets
+        dst = g.out_dests_raw.reshape(-1)
+        ioff = idst = None
+        if g.directed:
+            ioff = _p(g.in_offsets, C.c_int64)
+            idst = _p(g.in_dests_raw.reshape(-1), C.c_int32)
+        _check(lib().gg_graph_upload(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32),
+                                     int(g.weighted), int(g.directed), ioff, idst, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return lib().gg_graph_device_bytes(self._h)
+
+    def profile(self, on: bool = True) -> None:
+        """Time every kernel launch with HIP events (see gg_profile_enable)."""
+        _check(lib().gg_profile_enable(self._h, int(on)))
+
+    def profile_reset(self) -> None:
+        _check(lib().gg_profile_reset(self._h))
+
+    def profile_report(self) -> dict:
+        """{kernel: {launches, ms, alg_bytes}} accumulated since the last reset."""
+        out = {}
+        name = C.create_string_buffer(128)
+        idx = 0
+        while True:
+            n, ms, b = C.c_int64(), C.c_double(), C.c_double()
+            rc = lib().gg_profile_get(self._h, idx, name, 128, C.byref(n), C.byref(ms), C.byref(b))
+            if rc < 0:
+                break
+            out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "alg_bytes": b.value}
+            idx += 1
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "_h", None) is not None and _LIB is not None:
+            _LIB.gg_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+@dataclass
+class BFSResult:
+    parent: np.ndarray
+    depth: np.ndarray | None
+    stats: dict
+
+
+def bfs(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18,
+        want_depth: bool = True, want_parent: bool = True) -> BFSResult:
+    """gg_bfs; with want_parent/want_depth False the result stays on the device."""
+    n = dg.num_nodes
+    parent = np.empty(n, dtype=np.int32) if want_parent else None
+    depth = np.empty(n, dtype=np.int32) if want_depth else None
+    st = Stats()
+    _check(lib().gg_bfs(dg.handle, source, alpha, beta,
+                        _p(parent, C.c_int32) if want_parent else None,
+                        _p(depth, C.c_int32) if want_depth else None, C.byref(st)))
+    return BFSResult(parent, depth, st.as_dict())
+
+
+def DOBFS(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18) -> np.ndarray:
+    """Parent array of a direction-optimizing BFS (src/bfs.cc:122-179)."""
+    return bfs(dg, source, alpha, beta, want_depth=False).parent

@@ -1,0 +1,98 @@
+/* gapgpu -- C-ABI of the MI355X (gfx950) kernel library (libgapgpu.so).
+ *
+ * The reference has no FFI, plugin registry or operator table.  Its kernel
+ * boundary is the functor the harness calls once per trial,
+ *   Result kernel(const Graph&)            (src/benchmark.h:98-106)
+ * bound in each driver's main() to one kernel function.  Each entry point
+ * below replaces one of those kernel functions; a driver wraps it in the
+ * same lambda, so BenchmarkKernel, the analysis printers and the verifiers
+ * stay as they are.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative GG_E* status on failure; never exit or
+ *    abort.  gg_last_error() (thread-local) describes the last failure.
+ *  - Graph arrays are the .sg/.wsg layout: int64 offsets[n+1] and int32
+ *    destinations (weighted: {int32 v, int32 w} pairs).  gg_graph_upload
+ *    copies them into HBM once; the handle owns that device memory.
+ *  - Output buffers are caller-owned host memory sized as documented.
+ *    Calls are synchronous: they return when the results are in host memory.
+ *  - Each handle has its own HIP stream; use one handle from one host thread
+ *    at a time (the reference's harness is single-threaded).
+ *  - gg_stats (nullable) receives device time measured with HIP events on
+ *    that stream, and per-kernel counters.
+ */
+#ifndef GAPGPU_H_
+#define GAPGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GG_OK = 0,
+  GG_EINVAL = -22,   /* bad argument (root out of range, wrong graph kind ...) */
+  GG_ENODEV = -19,   /* no usable GPU / gg_init not called */
+  GG_ENOMEM = -12,   /* device allocation failed or exceeds GAPGPU_MEM_CAP_BYTES */
+  GG_EHIP = -5,      /* any other HIP runtime error */
+  GG_ENOTSUP = -95   /* unsupported graph shape for this kernel */
+};
+
+typedef struct gg_graph gg_graph;
+
+typedef struct gg_stats {
+  double device_ms;       /* HIP-event time of the kernel sequence on the handle's stream */
+  double total_ms;        /* host wall time of the call, result copies included */
+  int64_t iterations;     /* levels (BFS/BC), buckets (SSSP), iterations (PR), passes (CC) */
+  int64_t td_steps;       /* BFS: top-down levels */
+  int64_t bu_steps;       /* BFS: bottom-up levels */
+  int64_t reached;        /* vertices reached (BFS/SSSP/BC, summed over roots) */
+  int64_t reached_edges;  /* sum of out-degrees of reached vertices (summed over roots) */
+  int64_t edges_examined; /* adjacency entries read by the kernels, when counted */
+  int64_t vertices_examined; /* BFS: frontier vertices expanded + unvisited vertices scanned */
+  double residual;        /* PR: final L1 change (FP64) */
+  int64_t launches;       /* kernel launches issued */
+} gg_stats;
+
+/* Selects and initialises a HIP device for the calling process. */
+int gg_init(int device);
+
+/* Copies a CSR graph into HBM.  weighted: dests are {v, w} pairs (2 int32 per
+ * entry).  directed: the inverse CSR (in_offsets / in_dests) is required;
+ * for undirected graphs pass NULL and the out-arrays serve both directions
+ * (src/graph.h:138-143). */
+int gg_graph_upload(int64_t num_nodes, const int64_t* out_offsets, const int32_t* out_dests,
+                    int weighted, int directed, const int64_t* in_offsets, const int32_t* in_dests,
+                    gg_graph** out);
+int gg_graph_free(gg_graph* g);
+/* Device bytes held by the handle (graph + cached kernel workspaces). */
+int64_t gg_graph_device_bytes(const gg_graph* g);
+
+/* Direction-optimizing BFS from `source`.
+ * Replaces DOBFS(const Graph&, NodeID source, bool logging, int alpha=15,
+ * int beta=18) (src/bfs.cc:122-179) with the same alpha/beta switch.
+ * parent[n]: parent[source] = source, -1 if unreached.
+ * depth[n]: BFS level, -1 if unreached.
+ * Either output may be NULL: the result then stays in device memory only
+ * (used to time the kernels without the result copy). */
+int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* parent, int32_t* depth,
+           gg_stats* stats);
+
+/* Per-kernel measurement (no reference counterpart; used by bench.py).
+ * While enabled, every kernel launch on the handle is bracketed by HIP
+ * events on the handle's stream and credited with its algorithmic HBM bytes
+ * (the compulsory-traffic model of each kernel).  gg_profile_get returns the
+ * idx-th kernel's totals, or GG_EINVAL past the last one. */
+int gg_profile_enable(gg_graph* g, int on);
+int gg_profile_reset(gg_graph* g);
+int gg_profile_get(gg_graph* g, int idx, char* name, int name_len, int64_t* launches,
+                   double* total_ms, double* alg_bytes);
+
+const char* gg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAPGPU_H_ */

@@ -1,0 +1,194 @@
+// CPU oracle for the kernel hot path -- TEST INFRASTRUCTURE ONLY.
+// See oracle.h.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline load this library; the product never does.
+#include "oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline double Since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+inline bool Bit(const std::vector<uint64_t>& bm, int64_t i) { return (bm[i >> 6] >> (i & 63)) & 1; }
+
+}  // namespace
+
+extern "C" {
+
+// src/bfs.cc:204-215: plain BFS from src over out-neighbours.
+void orc_bfs_depth(int64_t n, const int64_t* off, const int32_t* dst, int32_t src, int32_t* depth) {
+  std::fill(depth, depth + n, -1);
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(n));
+  depth[src] = 0;
+  order.push_back(src);
+  for (size_t i = 0; i < order.size(); i++) {
+    const int32_t u = order[i];
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = dst[e];
+      if (depth[v] == -1) {
+        depth[v] = depth[u] + 1;
+        order.push_back(v);
+      }
+    }
+  }
+}
+
+// Restates DOBFS (src/bfs.cc:122-179) with its steps TDStep (67-89),
+// BUStep (46-64), QueueToBitmap/BitmapToQueue (92-112) and InitParent
+// (114-120).  Frontier appends go through per-thread buffers; bottom-up
+// works on whole 64-vertex bitmap words so `next` needs no atomics.
+double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
+                 const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
+                 int32_t* parent, orc_bfs_stats* st) {
+  const auto t0 = Clock::now();
+  orc_bfs_stats s;
+  std::memset(&s, 0, sizeof(s));
+  const int64_t nw = (n + 63) / 64;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t d = off[i + 1] - off[i];
+    parent[i] = d != 0 ? static_cast<int32_t>(-d) : -1;
+  }
+  parent[src] = src;
+  std::vector<int32_t> queue(static_cast<size_t>(n));
+  size_t qb = 0, qe = 1;
+  queue[0] = src;
+  std::vector<uint64_t> front(static_cast<size_t>(nw), 0), next(static_cast<size_t>(nw), 0);
+  int64_t edges_to_check = e_dir;
+  int64_t scout = off[src + 1] - off[src];
+  s.reached = 1;
+  s.reached_edges = scout;
+  while (qe > qb) {
+    if (scout > edges_to_check / alpha) {
+      std::fill(front.begin(), front.end(), 0);
+      for (size_t i = qb; i < qe; i++) front[queue[i] >> 6] |= uint64_t(1) << (queue[i] & 63);
+      int64_t awake = static_cast<int64_t>(qe - qb), old_awake;
+      do {
+        old_awake = awake;
+        int64_t aw = 0, ex = 0, vx = 0, re = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : aw, ex, vx, re)
+        for (int64_t w = 0; w < nw; w++) {
+          uint64_t word = 0;
+          const int64_t hi = std::min<int64_t>(64, n - w * 64);
+          for (int64_t l = 0; l < hi; l++) {
+            const int64_t u = w * 64 + l;
+            if (parent[u] >= 0) continue;
+            vx++;
+            for (int64_t e = ioff[u]; e < ioff[u + 1]; e++) {
+              ex++;
+              const int32_t v = idst[e];
+              if (Bit(front, v)) {
+                parent[u] = v;
+                word |= uint64_t(1) << l;
+                aw++;
+                re += off[u + 1] - off[u];
+                break;
+              }
+            }
+          }
+          next[w] = word;
+        }
+        awake = aw;
+        s.edges_examined += ex;
+        s.vertices_examined += vx;
+        s.reached += aw;
+        s.reached_edges += re;
+        s.bu_steps++;
+        front.swap(next);
+      } while (awake >= old_awake || awake > n / beta);
+      qb = qe = 0;
+      for (int64_t w = 0; w < nw; w++) {
+        uint64_t word = front[w];
+        while (word) {
+          const int l = __builtin_ctzll(word);
+          queue[qe++] = static_cast<int32_t>(w * 64 + l);
+          word &= word - 1;
+        }
+      }
+      scout = 1;
+    } else {
+      edges_to_check -= scout;
+      int64_t sc = 0, ex = 0, re = 0, found = 0;
+      size_t tail = qe;
+#pragma omp parallel reduction(+ : sc, ex, re, found)
+      {
+        std::vector<int32_t> local;
+#pragma omp for schedule(dynamic, 64) nowait
+        for (size_t i = qb; i < qe; i++) {
+          const int32_t u = queue[i];
+          ex += off[u + 1] - off[u];
+          for (int64_t e = off[u]; e < off[u + 1]; e++) {
+            const int32_t v = dst[e];
+            int32_t cur = __atomic_load_n(&parent[v], __ATOMIC_RELAXED);
+            if (cur < 0 && __atomic_compare_exchange_n(&parent[v], &cur, u, false, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED)) {
+              local.push_back(v);
+              sc += -static_cast<int64_t>(cur);
+              re += off[v + 1] - off[v];
+              found++;
+            }
+          }
+        }
+        size_t at;
+#pragma omp atomic capture
+        {
+          at = tail;
+          tail += local.size();
+        }
+        std::copy(local.begin(), local.end(), queue.begin() + at);
+      }
+      s.vertices_examined += static_cast<int64_t>(qe - qb);
+      s.edges_examined += ex;
+      s.reached += found;
+      s.reached_edges += re;
+      s.td_steps++;
+      scout = sc;
+      qb = qe;
+      qe = tail;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++)
+    if (parent[i] < -1) parent[i] = -1;
+  if (st) *st = s;
+  return Since(t0);
+}
+
+int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth) {
+  std::fill(depth, depth + n, -2);
+  if (parent[src] != src) return -1;
+  depth[src] = 0;
+  std::vector<int32_t> chain;
+  for (int64_t v = 0; v < n; v++) {
+    if (depth[v] != -2) continue;
+    if (parent[v] < 0) {
+      depth[v] = -1;
+      continue;
+    }
+    chain.clear();
+    int64_t x = v;
+    while (depth[x] == -2) {
+      if (parent[x] < 0 || parent[x] >= n || static_cast<int64_t>(chain.size()) > n) return -1;
+      chain.push_back(static_cast<int32_t>(x));
+      depth[x] = -3;  // on the current chain
+      x = parent[x];
+    }
+    if (depth[x] < 0) return -1;  // cycle or a chain into an unreached vertex
+    int32_t d = depth[x];
+    for (auto it = chain.rbegin(); it != chain.rend(); ++it) depth[*it] = ++d;
+  }
+  return 0;
+}
+
+}  // extern "C"

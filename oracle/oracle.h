@@ -1,0 +1,46 @@
+/* CPU oracle for the kernel hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A restatement of the reference's kernels over plain CSR arrays
+ * (int64 offsets[n+1], int32 destinations, {int32 v, int32 w} pairs when
+ * weighted), used by tests/ as the parity checker and by bench.py as the CPU
+ * baseline ("port").  Nothing in the product links or calls this library.
+ *
+ * Each function cites the reference code it restates.  How each one is
+ * pinned (golden values, the reference built from source, closed forms,
+ * brute force) is listed in the tests that exercise it.
+ */
+#ifndef ORACLE_H_
+#define ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_bfs_stats {
+  int64_t td_steps, bu_steps;
+  int64_t edges_examined;  /* adjacency entries read under the reference schedule */
+  int64_t vertices_examined; /* TD frontier sizes + BU unvisited vertices scanned */
+  int64_t reached, reached_edges;
+} orc_bfs_stats;
+
+/* Serial level-synchronous BFS depths (textbook; src/bfs.cc:204-215). */
+void orc_bfs_depth(int64_t n, const int64_t* off, const int32_t* dst, int32_t src, int32_t* depth);
+
+/* OpenMP direction-optimizing BFS, restating DOBFS (src/bfs.cc:46-179).
+ * Writes parent[n]; returns elapsed seconds.  ioff/idst: inverse CSR (the
+ * out-CSR again for undirected graphs).  e_dir = num_edges_directed. */
+double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
+                 const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
+                 int32_t* parent, orc_bfs_stats* st);
+
+/* Depths implied by a parent tree (-1 where unreached); returns -1 if the
+ * parent pointers do not form a tree rooted at src. */
+int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORACLE_H_ */

@@ -1,0 +1,85 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg,
+never by the product package.  Each function names the reference code it
+restates; see oracle.h.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class BFSStats(C.Structure):
+    _fields_ = [("td_steps", C.c_int64), ("bu_steps", C.c_int64),
+                ("edges_examined", C.c_int64), ("vertices_examined", C.c_int64),
+                ("reached", C.c_int64), ("reached_edges", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run `make -C oracle`")
+        L = C.CDLL(path)
+        L.orc_bfs_depth.argtypes = [C.c_int64, _i64p, _i32p, C.c_int32, _i32p]
+        L.orc_bfs_depth.restype = None
+        L.orc_dobfs.argtypes = [C.c_int64, _i64p, _i32p, _i64p, _i32p, C.c_int64, C.c_int32,
+                                C.c_int, C.c_int, _i32p, C.POINTER(BFSStats)]
+        L.orc_dobfs.restype = C.c_double
+        L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _csr(g):
+    off = g.out_offsets
+    dst = g.out_neighbors if not g.weighted else np.ascontiguousarray(g.out_neighbors)
+    return off, np.ascontiguousarray(dst)
+
+
+def bfs_depth(g, src: int) -> np.ndarray:
+    """Serial BFS depths (src/bfs.cc:204-215)."""
+    off, dst = _csr(g)
+    depth = np.empty(g.num_nodes, dtype=np.int32)
+    lib().orc_bfs_depth(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), src,
+                        _p(depth, C.c_int32))
+    return depth
+
+
+def dobfs(g, src: int, alpha: int = 15, beta: int = 18):
+    """OpenMP direction-optimizing BFS (src/bfs.cc:122-179): (parent, seconds, stats)."""
+    off, dst = _csr(g)
+    ioff, idst = g.in_offsets, np.ascontiguousarray(g.in_neighbors)
+    parent = np.empty(g.num_nodes, dtype=np.int32)
+    st = BFSStats()
+    secs = lib().orc_dobfs(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32),
+                           _p(ioff, C.c_int64), _p(idst, C.c_int32), g.num_edges_directed,
+                           src, alpha, beta, _p(parent, C.c_int32), C.byref(st))
+    return parent, secs, st.as_dict()
+
+
+def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:
+    """Depths implied by a parent tree; raises if it is not a tree rooted at src."""
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    depth = np.empty(len(parent), dtype=np.int32)
+    rc = lib().orc_depth_from_parent(len(parent), _p(parent, C.c_int32), src, _p(depth, C.c_int32))
+    if rc != 0:
+        raise ValueError("parent array is not a tree rooted at the source")
+    return depth

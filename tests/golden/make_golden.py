@@ -1,0 +1,69 @@
+"""Writes tests/golden/golden.json from the reference (built from its own
+sources by `make -C oracle ref`) and the reference's own test fixtures.
+
+Run in a container that has /root/reference:  python tests/golden/make_golden.py
+The JSON holds data only: expected stats lines, hashes of the reference's
+serialized graphs, SourcePicker root sequences and per-root result digests
+of the reference kernels, so the GPU box (which has no reference) can pin the
+oracle and the GPU path against them.
+"""
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from gapbs_amd.graph import Graph  # noqa: E402
+from oracle import oracle, reference as ref  # noqa: E402
+
+REF_TEST = "/root/reference/test/reference"
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}}
+    # The reference's own expected stats lines (test/test.mk:44-49).
+    for name in ("g10", "u10"):
+        with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
+            out["stats_lines"][name] = f.read().strip()
+    # Bytes of the reference converter's serialized graphs.
+    with tempfile.TemporaryDirectory() as td:
+        for flags, tag, suffix in ((["-g10"], "g10", ".sg"), (["-u10"], "u10", ".sg"),
+                                   (["-g16"], "g16", ".sg"), (["-g10", "-w"], "g10w", ".wsg")):
+            path = os.path.join(td, tag + suffix)
+            subprocess.run([os.path.join(REF_BIN, "converter"), *flags, "-b", path], check=True,
+                           stdout=subprocess.DEVNULL)
+            with open(path, "rb") as f:
+                out["sg_sha256"][tag] = hashlib.sha256(f.read()).hexdigest()
+    # SourcePicker sequences and reference DOBFS depth digests.
+    for tag, g in (("g10", Graph.kronecker(10)), ("g16", Graph.kronecker(16)),
+                   ("u10", Graph.uniform(10))):
+        roots = ref.pick_sources(g, 16)
+        out["roots"][tag] = [int(r) for r in roots]
+        rg = ref.RefGraph("bfs", g)
+        parents, _ = ref.bfs(rg, roots)
+        digests = []
+        for r, p in zip(roots, parents):
+            d = oracle.depth_from_parent(p, int(r))
+            digests.append({"root": int(r), "depth_sha256": sha(d),
+                            "reached": int((d >= 0).sum()), "max_depth": int(d.max())})
+        out["bfs"][tag] = digests
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote golden.json")
+
+
+if __name__ == "__main__":
+    main()
