@@ -14,75 +14,144 @@
 //     parents are the reference's;
 //   * the final fix-up parent < -1 -> -1 (bfs.cc:174-177).
 // What is GPU-specific:
-//   * the top-down step is one kernel that expands each 256-vertex tile of
-//     the frontier three ways: vertices with >= 256 edges by the whole block,
-//     >= 64 by one wavefront, the rest by a block-wide prefix sum over their
-//     edges staged in LDS; discovered vertices go to the next queue with one
-//     atomic per wavefront (ballot + popcount);
-//   * the bottom-up step gives each wavefront 64 consecutive vertices, so
-//     the frontier bitmap word for them is one ballot, written without
-//     atomics;
-//   * the controller runs on the host and reads back one small counter
-//     block per level.
+//   * top-down expands 256-vertex tiles of the frontier with gg_expand.h
+//     (hubs cut into chunks spread over the grid; when the frontier is small
+//     every vertex with >= 256 edges is chunked); discovered vertices are
+//     staged in an LDS queue and flushed with one atomic per block;
+//   * bottom-up gives each wavefront 64 consecutive vertices, so the next
+//     frontier word is one ballot, written without atomics; a visited bitmap
+//     (with vertices that have no in-neighbours pre-set) lets whole words be
+//     skipped; neighbours are probed 2, then 8 at a time;
+//   * per-step sums are striped over 64 slots and posted to the host by a
+//     one-block kernel through a mapped mailbox; the host runs the
+//     reference's controller on them.
 #include "gg_common.h"
+#include "gg_expand.h"
 
 namespace gg {
 namespace {
 
 constexpr int kBS = 256;
+constexpr int kQueueCap = 2048;  // LDS-staged queue entries per block
+
+// Sums are striped over kStripes slots (slot = block index mod kStripes)
+// so that thousands of blocks do not serialise on one address.
+constexpr int kStripes = 64;
+enum { kScout, kAwake, kReach, kExamined, kVexam, kNumSums };
 
 struct BfsCounters {
-  int32_t qtail;
-  int32_t pad;
-  unsigned long long scout;     // TD: sum of -parent of discovered vertices
-  unsigned long long awake;     // BU: vertices discovered
-  unsigned long long reach;     // sum of out-degrees of discovered vertices
-  unsigned long long examined;  // adjacency entries read
-  unsigned long long vexam;     // BU: unvisited vertices scanned
+  int32_t qtail;      // top-down output queue tail
+  int32_t nchunks;    // top-down hub chunks emitted this step
+  int32_t src_no_in;  // set by BfsInit: the source has no in-neighbours
+  int32_t bmtail;     // bitmap-to-queue output tail
+  unsigned long long sums[kStripes][kNumSums];
 };
 
-__global__ void BfsResetCounters(BfsCounters* c) {
+__device__ __forceinline__ unsigned long long* Stripe(BfsCounters* c) {
+  return c->sums[blockIdx.x & (kStripes - 1)];
+}
+
+__global__ __launch_bounds__(kBS) void BfsResetCounters(BfsCounters* c) {
+  unsigned long long* p = &c->sums[0][0];
+  for (int i = threadIdx.x; i < kStripes * kNumSums; i += blockDim.x) p[i] = 0;
   if (threadIdx.x == 0) {
     c->qtail = 0;
-    c->scout = 0;
-    c->awake = 0;
-    c->reach = 0;
-    c->examined = 0;
-    c->vexam = 0;
+    c->nchunks = 0;
+    c->src_no_in = 0;
+    c->bmtail = 0;
   }
 }
 
-__global__ __launch_bounds__(kBS) void BfsInit(const int64_t* __restrict__ off, int32_t* __restrict__ parent,
-                                               int32_t* __restrict__ depth, int64_t n, int32_t src,
-                                               int32_t* __restrict__ queue, BfsCounters* c) {
+// Values BfsPublish posts to the host mailbox.
+enum { kMailQtail = kNumSums, kMailSrcNoIn, kMailValues };
+
+// Sums the striped counters, posts the totals to the host mailbox and
+// clears the per-step counters.  One block of kBS threads.
+__global__ __launch_bounds__(kBS) void BfsPublish(BfsCounters* c, unsigned long long* mb,
+                                                  unsigned long long seq) {
+  __shared__ unsigned long long tot[kNumSums];
+  if (threadIdx.x < kNumSums) tot[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < kStripes * kNumSums; i += kBS) {
+    unsigned long long* p = &c->sums[i / kNumSums][i % kNumSums];
+    const unsigned long long v = *p;
+    if (v) atomicAdd(&tot[i % kNumSums], v);
+    *p = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long vals[kMailValues];
+    for (int k = 0; k < kNumSums; k++) vals[k] = tot[k];
+    vals[kMailQtail] = static_cast<unsigned long long>(c->qtail);
+    vals[kMailSrcNoIn] = static_cast<unsigned long long>(c->src_no_in);
+    c->qtail = 0;
+    c->nchunks = 0;
+    c->bmtail = 0;
+    MailPost(mb, vals, kMailValues, seq);
+  }
+}
+
+// Bit l of word w: vertex 64w + l has no in-neighbours (or lies past n).
+__global__ __launch_bounds__(kBS) void BfsNoInNeighbours(const int64_t* __restrict__ in_off, int64_t n,
+                                                         uint64_t* __restrict__ no_in,
+                                                         unsigned long long* count) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  const int lane = LaneId();
+  const int64_t nwords = (n + 63) >> 6;
+  const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
+  unsigned long long cnt = 0;
+  for (int64_t w = wave; w < nwords; w += nwaves) {
+    const int64_t u = w * 64 + lane;
+    const uint64_t m = __ballot(u >= n || in_off[u + 1] == in_off[u]);
+    if (lane == 0) {
+      no_in[w] = m;
+      const uint64_t valid = (w == nwords - 1 && (n & 63)) ? ((1ull << (n & 63)) - 1) : ~0ull;
+      cnt += __popcll(m & valid);
+    }
+  }
+  cnt = BlockSum<kBS>(cnt, s_red);
+  if (threadIdx.x == 0 && cnt) atomicAdd(count, cnt);
+}
+
+__global__ __launch_bounds__(kBS) void BfsInit(const int64_t* __restrict__ off,
+                                               const int64_t* __restrict__ in_off,
+                                               int32_t* __restrict__ parent, int32_t* __restrict__ depth,
+                                               int64_t n, int32_t src, int32_t* __restrict__ queue,
+                                               BfsCounters* c) {
   for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS) {
     const int64_t deg = off[i + 1] - off[i];
     int32_t p = deg != 0 ? static_cast<int32_t>(-deg) : -1;
     if (i == src) {
       p = src;
       queue[0] = src;
-      c->scout = static_cast<unsigned long long>(deg);
-      c->reach = static_cast<unsigned long long>(deg);
+      c->sums[0][kScout] = static_cast<unsigned long long>(deg);
+      c->sums[0][kReach] = static_cast<unsigned long long>(deg);
+      c->src_no_in = in_off[i + 1] == in_off[i];
     }
     parent[i] = p;
     if (depth) depth[i] = (i == src) ? 0 : -1;
   }
 }
 
-// One frontier edge u -> v (valid lanes only act; all lanes must call).
+// One frontier edge u -> dst[e] (valid lanes only act; all lanes of a
+// wavefront call together).  Winners of the parent CAS join the next queue.
 struct TdVisit {
+  const int32_t* dst;
   int32_t* parent;
   int32_t* depth;
   const int64_t* off;
+  LdsQueue<kQueueCap>* lq;
   int32_t* qout;
   int32_t* qtail;
   int32_t level;
   unsigned long long scout = 0, reach = 0;
 
-  __device__ __forceinline__ void operator()(int32_t u, int32_t v, bool valid) {
+  __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
     bool won = false;
-    int32_t cur = 0;
+    int32_t cur = 0, v = 0;
     if (valid) {
+      v = dst[e];
       cur = parent[v];
       if (cur < 0) won = atomicCAS(&parent[v], cur, u) == cur;
     }
@@ -93,29 +162,37 @@ struct TdVisit {
       reach += cur < -1 ? static_cast<unsigned long long>(-static_cast<int64_t>(cur))
                         : static_cast<unsigned long long>(off[v + 1] - off[v]);
     }
-    WaveAppend(won, v, qout, qtail);
+    LdsQueuePush(*lq, won, v, qout, qtail);
   }
 };
+
+__device__ __forceinline__ void FlushTdCounters(const TdVisit& visit, unsigned long long examined,
+                                                BfsCounters* c, unsigned long long* s_red) {
+  const unsigned long long sc = BlockSum<kBS>(visit.scout, s_red);
+  const unsigned long long re = BlockSum<kBS>(visit.reach, s_red);
+  const unsigned long long ex = BlockSum<kBS>(examined, s_red);
+  if (threadIdx.x == 0) {
+    unsigned long long* st = Stripe(c);
+    if (sc) atomicAdd(&st[kScout], sc);
+    if (re) atomicAdd(&st[kReach], re);
+    if (ex) atomicAdd(&st[kExamined], ex);
+  }
+}
 
 __global__ __launch_bounds__(kBS) void BfsTopDown(const int64_t* __restrict__ off,
                                                   const int32_t* __restrict__ dst,
                                                   int32_t* __restrict__ parent, int32_t* __restrict__ depth,
                                                   const int32_t* __restrict__ qin, int32_t qn,
-                                                  int32_t* __restrict__ qout, BfsCounters* c, int32_t level) {
-  __shared__ int s_claim;
-  __shared__ int32_t s_u;
-  __shared__ int64_t s_beg, s_end;
-  __shared__ int s_scan[kBS / kWave];
-  __shared__ int32_t s_src[kBS];
-  __shared__ int64_t s_eidx[kBS];
+                                                  int32_t* __restrict__ qout, BfsCounters* c, int32_t level,
+                                                  EdgeChunk* chunks, int64_t chunk_min) {
+  __shared__ ExpandShared sh;
+  __shared__ LdsQueue<kQueueCap> lq;
   __shared__ unsigned long long s_red[kBS / kWave];
-
-  const int tid = threadIdx.x, lane = LaneId();
-  TdVisit visit{parent, depth, off, qout, &c->qtail, level};
+  LdsQueueInit(lq);
+  TdVisit visit{dst, parent, depth, off, &lq, qout, &c->qtail, level};
   unsigned long long examined = 0;
-
   for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < qn; tile += int64_t(gridDim.x) * kBS) {
-    const int64_t i = tile + tid;
+    const int64_t i = tile + threadIdx.x;
     int32_t u = 0;
     int64_t beg = 0, end = 0;
     if (i < qn) {
@@ -124,71 +201,26 @@ __global__ __launch_bounds__(kBS) void BfsTopDown(const int64_t* __restrict__ of
       end = off[u + 1];
       examined += static_cast<unsigned long long>(end - beg);
     }
-    // Vertices with >= kBS edges: the whole block, one vertex at a time.
-    while (true) {
-      if (tid == 0) s_claim = -1;
-      __syncthreads();
-      if (end - beg >= kBS) s_claim = tid;
-      __syncthreads();
-      const int owner = s_claim;
-      if (owner < 0) break;
-      if (tid == owner) {
-        s_u = u;
-        s_beg = beg;
-        s_end = end;
-        beg = end;
-      }
-      __syncthreads();
-      const int32_t bu = s_u;
-      const int64_t bb = s_beg, be = s_end;
-      for (int64_t base = bb; base < be; base += kBS) {
-        const int64_t e = base + tid;
-        const bool ok = e < be;
-        visit(bu, ok ? dst[e] : 0, ok);
-      }
-      __syncthreads();
-    }
-    // Vertices with >= 64 edges: one wavefront each.
-    while (true) {
-      const uint64_t m = __ballot(end - beg >= kWave);
-      if (m == 0) break;
-      const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
-      const int32_t wu = __shfl(u, leader);
-      const int64_t wb = __shfl(beg, leader), we = __shfl(end, leader);
-      if (lane == leader) beg = end;
-      for (int64_t base = wb; base < we; base += kWave) {
-        const int64_t e = base + lane;
-        const bool ok = e < we;
-        visit(wu, ok ? dst[e] : 0, ok);
-      }
-    }
-    // The rest (< 64 edges each): block-wide prefix sum over their edges.
-    const int d = static_cast<int>(end - beg);
-    int total = 0;
-    const int rank = BlockExclusiveScan<kBS>(d, &total, s_scan);
-    for (int done = 0; done < total; done += kBS) {
-      const int lo = rank > done ? rank : done;
-      const int hi = (rank + d) < (done + kBS) ? (rank + d) : (done + kBS);
-      for (int j = lo; j < hi; j++) {
-        s_src[j - done] = u;
-        s_eidx[j - done] = beg + (j - rank);
-      }
-      __syncthreads();
-      const bool ok = done + tid < total;
-      const int32_t su = ok ? s_src[tid] : 0;
-      const int32_t v = ok ? dst[s_eidx[tid]] : 0;
-      visit(su, v, ok);
-      __syncthreads();
-    }
+    ExpandTile(sh, u, 0, beg, end, visit, chunks, &c->nchunks, chunk_min);
   }
-  const unsigned long long sc = BlockSum<kBS>(visit.scout, s_red);
-  const unsigned long long re = BlockSum<kBS>(visit.reach, s_red);
-  const unsigned long long ex = BlockSum<kBS>(examined, s_red);
-  if (tid == 0) {
-    if (sc) atomicAdd(&c->scout, sc);
-    if (re) atomicAdd(&c->reach, re);
-    if (ex) atomicAdd(&c->examined, ex);
-  }
+  LdsQueueFlush(lq, qout, &c->qtail);
+  FlushTdCounters(visit, examined, c, s_red);
+}
+
+// Edges of the vertices BfsTopDown cut into chunks.
+__global__ __launch_bounds__(kBS) void BfsTopDownChunks(const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ dst,
+                                                        int32_t* __restrict__ parent,
+                                                        int32_t* __restrict__ depth,
+                                                        int32_t* __restrict__ qout, BfsCounters* c,
+                                                        int32_t level, const EdgeChunk* chunks) {
+  __shared__ LdsQueue<kQueueCap> lq;
+  __shared__ unsigned long long s_red[kBS / kWave];
+  LdsQueueInit(lq);
+  TdVisit visit{dst, parent, depth, off, &lq, qout, &c->qtail, level};
+  ExpandChunks(chunks, c->nchunks, visit);
+  LdsQueueFlush(lq, qout, &c->qtail);
+  FlushTdCounters(visit, 0, c, s_red);
 }
 
 __global__ __launch_bounds__(kBS) void FillU64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
@@ -203,14 +235,35 @@ __global__ __launch_bounds__(kBS) void BfsQueueToBitmap(const int32_t* __restric
   }
 }
 
-// Each wavefront owns bitmap words; lane l handles vertex 64*word + l.
+// visited[w] bit l <=> parent[64w + l] >= 0, or vertex 64w + l has no
+// in-neighbours (bottom-up can never discover it), or lies past n.  Rebuilt
+// at each switch to bottom-up; bottom-up steps keep it current.
+__global__ __launch_bounds__(kBS) void BfsVisitedFromParent(const int32_t* __restrict__ parent, int64_t n,
+                                                            const uint64_t* __restrict__ no_in,
+                                                            uint64_t* __restrict__ visited) {
+  const int lane = LaneId();
+  const int64_t nwords = (n + 63) >> 6;
+  const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
+  for (int64_t w = wave; w < nwords; w += nwaves) {
+    const int64_t u = w * 64 + lane;
+    const uint64_t m = __ballot(u >= n || parent[u] >= 0);
+    if (lane == 0) visited[w] = m | no_in[w];
+  }
+}
+
+// Bottom-up step.  Each wavefront owns bitmap words; lane l handles vertex
+// 64*word + l and scans its in-neighbours in sorted order, taking the first
+// one in `front`.
+constexpr int kBuUnroll = 8;
+
 __global__ __launch_bounds__(kBS) void BfsBottomUp(const int64_t* __restrict__ in_off,
                                                    const int32_t* __restrict__ in_dst,
                                                    const int64_t* __restrict__ out_off,
                                                    int32_t* __restrict__ parent, int32_t* __restrict__ depth,
                                                    const uint64_t* __restrict__ front,
-                                                   uint64_t* __restrict__ next, int64_t n, BfsCounters* c,
-                                                   int32_t level) {
+                                                   uint64_t* __restrict__ next, uint64_t* __restrict__ visited,
+                                                   int64_t n, BfsCounters* c, int32_t level) {
   __shared__ unsigned long long s_red[kBS / kWave];
   const int lane = LaneId();
   const int64_t nwords = (n + 63) >> 6;
@@ -218,29 +271,62 @@ __global__ __launch_bounds__(kBS) void BfsBottomUp(const int64_t* __restrict__ i
   const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
   unsigned long long awake = 0, reach = 0, examined = 0, vexam = 0;
   for (int64_t w = wave; w < nwords; w += nwaves) {
+    const uint64_t vis = visited[w];
+    if (vis == ~0ull) {
+      if (lane == 0) next[w] = 0;
+      continue;
+    }
     const int64_t u = w * 64 + lane;
     bool found = false;
-    if (u < n && parent[u] < 0) {
+    if (!((vis >> lane) & 1)) {
       vexam++;
       const int64_t b = in_off[u], e = in_off[u + 1];
-      int64_t k = b;
-      for (; k < e; k++) {
-        const int32_t v = in_dst[k];
-        if ((front[v >> 6] >> (v & 63)) & 1) {
-          parent[u] = v;
-          found = true;
-          break;
+      int64_t hit = -1;
+      int32_t hv = 0;
+      // Most vertices that will be found are found at their first or second
+      // in-neighbour: probe two, then eight at a time (independent loads).
+      {
+        const int32_t v0 = b < e ? in_dst[b] : -1;
+        const int32_t v1 = b + 1 < e ? in_dst[b + 1] : -1;
+        const uint64_t w0 = v0 >= 0 ? front[v0 >> 6] : 0;
+        const uint64_t w1 = v1 >= 0 ? front[v1 >> 6] : 0;
+        if (v1 >= 0 && ((w1 >> (v1 & 63)) & 1)) {
+          hit = b + 1;
+          hv = v1;
+        }
+        if (v0 >= 0 && ((w0 >> (v0 & 63)) & 1)) {
+          hit = b;
+          hv = v0;
         }
       }
-      examined += static_cast<unsigned long long>((found ? k + 1 : e) - b);
-      if (found) {
+      for (int64_t k = b + 2; k < e && hit < 0; k += kBuUnroll) {
+        int32_t v[kBuUnroll];
+        uint64_t word[kBuUnroll];
+#pragma unroll
+        for (int j = 0; j < kBuUnroll; j++) v[j] = (k + j < e) ? in_dst[k + j] : -1;
+#pragma unroll
+        for (int j = 0; j < kBuUnroll; j++) word[j] = v[j] >= 0 ? front[v[j] >> 6] : 0;
+#pragma unroll
+        for (int j = kBuUnroll - 1; j >= 0; j--)
+          if (v[j] >= 0 && ((word[j] >> (v[j] & 63)) & 1)) {
+            hit = k + j;
+            hv = v[j];
+          }
+      }
+      if (hit >= 0) {
+        found = true;
+        parent[u] = hv;
         if (depth) depth[u] = level;
         reach += static_cast<unsigned long long>(out_off[u + 1] - out_off[u]);
+        examined += static_cast<unsigned long long>(hit + 1 - b);
+      } else {
+        examined += static_cast<unsigned long long>(e - b);
       }
     }
     const uint64_t m = __ballot(found);
     if (lane == 0) {
       next[w] = m;
+      visited[w] = vis | m;
       awake += __popcll(m);
     }
   }
@@ -249,22 +335,42 @@ __global__ __launch_bounds__(kBS) void BfsBottomUp(const int64_t* __restrict__ i
   const unsigned long long ex = BlockSum<kBS>(examined, s_red);
   const unsigned long long vx = BlockSum<kBS>(vexam, s_red);
   if (threadIdx.x == 0) {
-    if (aw) atomicAdd(&c->awake, aw);
-    if (re) atomicAdd(&c->reach, re);
-    if (ex) atomicAdd(&c->examined, ex);
-    if (vx) atomicAdd(&c->vexam, vx);
+    unsigned long long* st = Stripe(c);
+    if (aw) atomicAdd(&st[kAwake], aw);
+    if (re) atomicAdd(&st[kReach], re);
+    if (ex) atomicAdd(&st[kExamined], ex);
+    if (vx) atomicAdd(&st[kVexam], vx);
   }
 }
 
-__global__ __launch_bounds__(kBS) void BfsBitmapToQueue(const uint64_t* __restrict__ bm, int64_t n,
-                                                        int32_t* __restrict__ q, BfsCounters* c) {
-  const int lane = LaneId();
-  const int64_t nwords = (n + 63) >> 6;
-  const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
-  const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
-  for (int64_t w = wave; w < nwords; w += nwaves) {
-    const bool set = (bm[w] >> lane) & 1;
-    WaveAppend(set, static_cast<int32_t>(w * 64 + lane), q, &c->qtail);
+// Compacts the set bits of a bitmap into a queue.  Each block owns a
+// contiguous range of words: it counts its bits, reserves its output range
+// with one atomic, then writes the vertices.
+__global__ __launch_bounds__(kBS) void BfsBitmapToQueue(const uint64_t* __restrict__ bm, int64_t nwords,
+                                                        int32_t* __restrict__ q, int32_t* tail) {
+  __shared__ int s_scan[kBS / kWave];
+  __shared__ int s_red[kBS / kWave];
+  __shared__ int s_base;
+  const int64_t per = (nwords + gridDim.x - 1) / gridDim.x;
+  const int64_t w0 = int64_t(blockIdx.x) * per;
+  const int64_t w1 = w0 + per < nwords ? w0 + per : nwords;
+  int cnt = 0;
+  for (int64_t w = w0 + threadIdx.x; w < w1; w += kBS) cnt += __popcll(bm[w]);
+  cnt = BlockSum<kBS>(cnt, s_red);
+  if (cnt == 0) return;  // uniform: every thread holds the block total
+  if (threadIdx.x == 0) s_base = atomicAdd(tail, cnt);
+  __syncthreads();
+  int run = s_base;
+  for (int64_t wb = w0; wb < w1; wb += kBS) {
+    const int64_t w = wb + threadIdx.x;
+    uint64_t word = w < w1 ? bm[w] : 0;
+    int total = 0;
+    int at = run + BlockExclusiveScan<kBS>(__popcll(word), &total, s_scan);
+    while (word) {
+      q[at++] = static_cast<int32_t>(w * 64 + __builtin_ctzll(word));
+      word &= word - 1;
+    }
+    run += total;
   }
 }
 
@@ -298,97 +404,126 @@ extern "C" int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t*
     int32_t* qout = g->Scratch<int32_t>("bfs.qb", n);
     uint64_t* front = g->Scratch<uint64_t>("bfs.front", nwords);
     uint64_t* next = g->Scratch<uint64_t>("bfs.next", nwords);
+    uint64_t* visited = g->Scratch<uint64_t>("bfs.visited", nwords);
     BfsCounters* dc = g->Scratch<BfsCounters>("bfs.ctr", 1);
-    BfsCounters* hc = g->Pinned<BfsCounters>(1);
+    const bool use_chunks = g->max_degree >= kExpandBS;
+    EdgeChunk* chunks =
+        use_chunks ? g->Scratch<EdgeChunk>("bfs.chunks", MaxChunks(n, g->out_entries)) : nullptr;
 
-    const unsigned grid_n = GridFor(n, kBS, g->cu_count * 32);
-    const unsigned grid_w = GridFor(nwords, kBS / kWave, g->cu_count * 32);
+    const dim3 blk(kBS);
+    const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
+    const dim3 grid_w(GridFor(nwords, kBS / kWave, g->cu_count * 32));
+    const dim3 grid_few(GridFor(nwords, kBS / kWave, g->cu_count));
     const double N = static_cast<double>(n), W = static_cast<double>(nwords);
     int64_t launches = 0;
-    auto read_counters = [&]() {
-      GG_HIP(hipMemcpyAsync(hc, dc, sizeof(BfsCounters), hipMemcpyDeviceToHost, st));
-      GG_HIP(hipStreamSynchronize(st));
-    };
-    auto reset = [&]() {
-      BfsResetCounters<<<1, 64, 0, st>>>(dc);
+    unsigned long long mail[kMailValues];
+    auto sum = [&](int k) { return static_cast<int64_t>(mail[k]); };
+    // Posts this step's counter totals to the host (and clears them).
+    auto publish = [&]() {
+      const unsigned long long seq = MailNext(g);
+      Run(g, "bfs_publish", 0, BfsPublish, dim3(1), blk, dc, g->mail_dev, seq);
       launches++;
+      MailWait(g, seq, mail, kMailValues);
     };
 
+    // Vertices without in-neighbours, once per graph: bottom-up treats them
+    // as visited (it could never discover them).
+    uint64_t* no_in = g->Scratch<uint64_t>("graph.no_in", nwords);
+    if (g->no_in_count < 0) {
+      unsigned long long* cnt = g->Scratch<unsigned long long>("graph.no_in_count", 1);
+      GG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
+      BfsNoInNeighbours<<<grid_few, blk, 0, st>>>(g->in_off, n, no_in, cnt);
+      unsigned long long h = 0;
+      GG_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      g->no_in_count = static_cast<int64_t>(h);
+    }
+
     BeginTiming(g);
-    reset();
+    Run(g, "bfs_reset", 0, BfsResetCounters, dim3(1), blk, dc);
     // Algorithmic bytes per launch: the compulsory traffic of each kernel
     // (offsets 8 B, ids 4 B, per-vertex state 4 B, bitmaps N/8).
-    Launch(g, "bfs_init", 8 * (N + 1) + 4 * N + (depth ? 4 * N : 0), [&] {
-      BfsInit<<<grid_n, kBS, 0, st>>>(g->out_off, parent, depth, n, source, qin, dc);
-    });
-    launches++;
-    read_counters();
-    int64_t scout = static_cast<int64_t>(hc->scout);
-    int64_t reach = static_cast<int64_t>(hc->reach);
+    Run(g, "bfs_init", 8 * (N + 1) + 4 * N + (depth ? 4 * N : 0), BfsInit, grid_n, blk,
+        g->out_off, g->in_off, parent, depth, n, source, qin, dc);
+    launches += 2;
+    publish();
+    int64_t scout = sum(kScout);
+    int64_t reach = sum(kReach);
+    // Unvisited vertices without in-neighbours that the reference's
+    // bottom-up step examines (with zero edges) and this one skips.
+    const int64_t skipped_no_in = g->no_in_count - (mail[kMailSrcNoIn] ? 1 : 0);
     int64_t examined = 0, vexamined = 0, reached = 1;
     const int64_t e_dir = g->directed ? g->out_entries : 2 * (g->out_entries / 2);
     int64_t edges_to_check = e_dir;
     int64_t qsize = 1;
     int32_t level = 1;
     int64_t td = 0, bu = 0;
+    // Upper bound on the summed out-degree of the current frontier.
+    int64_t frontier_deg = scout;
     while (qsize > 0) {
       if (scout > edges_to_check / alpha) {
-        Launch(g, "bfs_bitmap_clear", 8 * W,
-               [&] { FillU64<<<grid_w, kBS, 0, st>>>(front, nwords, 0); });
-        Launch(g, "bfs_queue_to_bitmap", 4.0 * qsize, [&] {
-          BfsQueueToBitmap<<<GridFor(qsize, kBS, g->cu_count * 32), kBS, 0, st>>>(
-              qin, static_cast<int32_t>(qsize), reinterpret_cast<unsigned long long*>(front));
-        });
-        launches += 2;
+        Run(g, "bfs_bitmap_clear", 8 * W, FillU64, grid_w, blk, front, nwords, uint64_t(0));
+        Run(g, "bfs_queue_to_bitmap", 4.0 * qsize, BfsQueueToBitmap,
+            dim3(GridFor(qsize, kBS, g->cu_count * 32)), blk, qin, static_cast<int32_t>(qsize),
+            reinterpret_cast<unsigned long long*>(front));
+        Run(g, "bfs_visited_from_parent", 4 * N + 16 * W, BfsVisitedFromParent, grid_w, blk, parent, n,
+            no_in, visited);
+        launches += 3;
         int64_t awake = qsize, old_awake;
         do {
           old_awake = awake;
-          reset();
-          const int idx = Launch(g, "bfs_bottom_up", 2 * W * 8, [&] {
-            BfsBottomUp<<<grid_w, kBS, 0, st>>>(g->in_off, g->in_dst, g->out_off, parent, depth,
-                                                front, next, n, dc, level);
-          });
+          const int idx = Run(g, "bfs_bottom_up", 3 * W * 8, BfsBottomUp, grid_w, blk, g->in_off,
+                              g->in_dst, g->out_off, parent, depth, front, next, visited, n, dc, level);
           launches++;
-          read_counters();
-          awake = static_cast<int64_t>(hc->awake);
+          publish();
+          awake = sum(kAwake);
+          frontier_deg = sum(kReach);
           reached += awake;
-          reach += static_cast<int64_t>(hc->reach);
-          examined += static_cast<int64_t>(hc->examined);
-          vexamined += static_cast<int64_t>(hc->vexam);
-          CreditBytes(g, idx, 4.0 * hc->examined + 16.0 * hc->vexam);
+          reach += frontier_deg;
+          examined += sum(kExamined);
+          vexamined += sum(kVexam) + skipped_no_in;
+          CreditBytes(g, idx, 4.0 * sum(kExamined) + 16.0 * sum(kVexam));
           level++;
           bu++;
           std::swap(front, next);
         } while (awake >= old_awake || awake > n / beta);
-        reset();
-        Launch(g, "bfs_bitmap_to_queue", 8 * W + 4.0 * awake,
-               [&] { BfsBitmapToQueue<<<grid_w, kBS, 0, st>>>(front, n, qin, dc); });
+        Run(g, "bfs_bitmap_to_queue", 16 * W + 4.0 * awake, BfsBitmapToQueue, grid_few, blk, front,
+            nwords, qin, &dc->bmtail);
         launches++;
         qsize = awake;  // the queue holds exactly the last step's discoveries
         scout = 1;
       } else {
         edges_to_check -= scout;
-        reset();
-        const int idx = Launch(g, "bfs_top_down", 16.0 * qsize, [&] {
-          BfsTopDown<<<GridFor(qsize, kBS, g->cu_count * 16), kBS, 0, st>>>(
-              g->out_off, g->out_dst, parent, depth, qin, static_cast<int32_t>(qsize), qout, dc,
-              level);
-        });
+        const bool hubs = use_chunks && frontier_deg >= kExpandBS;
+        // A small frontier gets every vertex with >= 256 edges chunked across
+        // the grid; a large one only the real hubs.
+        const int64_t chunk_min = qsize <= 4096 ? kExpandBS : kChunkEdges;
+        const int idx = Run(g, "bfs_top_down", 16.0 * qsize, BfsTopDown,
+                            dim3(GridFor(qsize, kBS, g->cu_count * 8)), blk, g->out_off, g->out_dst,
+                            parent, depth, qin, static_cast<int32_t>(qsize), qout, dc, level,
+                            hubs ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
         launches++;
+        if (hubs) {
+          const dim3 grid_c(GridFor(frontier_deg / kExpandBS + kChunkSlices, 1, g->cu_count * 8));
+          Run(g, "bfs_top_down_chunks", 0, BfsTopDownChunks, grid_c, blk, g->out_off, g->out_dst,
+              parent, depth, qout, dc, level, static_cast<const EdgeChunk*>(chunks));
+          launches++;
+        }
         vexamined += qsize;
-        read_counters();
-        qsize = hc->qtail;
+        publish();
+        qsize = static_cast<int64_t>(mail[kMailQtail]);
         reached += qsize;
-        scout = static_cast<int64_t>(hc->scout);
-        reach += static_cast<int64_t>(hc->reach);
-        examined += static_cast<int64_t>(hc->examined);
-        CreditBytes(g, idx, 4.0 * hc->examined);
+        scout = sum(kScout);
+        frontier_deg = scout;
+        reach += sum(kReach);
+        examined += sum(kExamined);
+        CreditBytes(g, idx, 4.0 * sum(kExamined));
         level++;
         td++;
         std::swap(qin, qout);
       }
     }
-    Launch(g, "bfs_finish", 4 * N, [&] { BfsFinish<<<grid_n, kBS, 0, st>>>(parent, n); });
+    Run(g, "bfs_finish", 4 * N, BfsFinish, grid_n, blk, parent, n);
     launches++;
     GG_HIP(hipGetLastError());
     FinishTiming(g, stats, wt);

@@ -2,6 +2,7 @@
 // graph handle with its cached workspaces, and wave64 primitives.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -91,9 +92,16 @@ struct gg_graph {
   int64_t* in_off = nullptr;   // == out_off for undirected graphs
   int32_t* in_dst = nullptr;
   size_t graph_bytes = 0;
+  int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
+  int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
   gg::Workspace ws;
   void* pinned = nullptr;  // small pinned host block for counter read-back
   size_t pinned_bytes = 0;
+  // Mailbox: pinned, device-mapped words the GPU posts small per-step
+  // results into; word 0 is a sequence number the host polls (gg::MailWait).
+  volatile unsigned long long* mail_host = nullptr;
+  unsigned long long* mail_dev = nullptr;
+  unsigned long long mail_seq = 0;
   int cu_count = 256;
 
   // Per-kernel profiling (gg_profile_*): launches bracketed by events.
@@ -139,23 +147,43 @@ inline void StatsReset(gg_stats* s) {
 void BeginTiming(gg_graph* g);
 void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt);
 
-// Runs `launch` (which enqueues one kernel on g->stream).  When profiling is
-// on, brackets it with events and returns the index of its record, so the
-// caller can credit algorithmic bytes known only after the kernel ran
-// (CreditBytes); returns -1 otherwise.
-int ProfileBegin(gg_graph* g, const char* name, double bytes);
-void ProfileEnd(gg_graph* g, int idx);
+// Profiling: CreditBytes adds algorithmic bytes known only after a kernel
+// ran to its record.
 void CreditBytes(gg_graph* g, int idx, double bytes);
 // Folds finished launches into the per-kernel totals (stream must be idle).
 void SettleProfile(gg_graph* g);
 
-template <class F>
-int Launch(gg_graph* g, const char* name, double bytes, F&& launch) {
-  const int idx = ProfileBegin(g, name, bytes);
-  launch();
-  ProfileEnd(g, idx);
+// Takes two events from the pool (creating them on demand) and records a
+// pending profiled launch; returns its index.
+int ProfileSlot(gg_graph* g, const char* name, double bytes, hipEvent_t* a, hipEvent_t* b);
+
+// Launches kernel k on g->stream.  When profiling is on, the dispatch packet
+// itself records start/stop events (hipExtLaunchKernelGGL: no extra queue
+// packets), and the launch is credited to `name` with `bytes` algorithmic
+// bytes; returns the record index for CreditBytes, else -1.
+template <class... KArgs, class... Args>
+int Run(gg_graph* g, const char* name, double bytes, void (*k)(KArgs...), dim3 grid, dim3 block,
+        Args... args) {
+  if (!g->profile) {
+    hipLaunchKernelGGL(k, grid, block, 0, g->stream, args...);
+    return -1;
+  }
+  hipEvent_t a, b;
+  const int idx = ProfileSlot(g, name, bytes, &a, &b);
+  hipExtLaunchKernelGGL(k, grid, block, 0, g->stream, a, b, 0, args...);
   return idx;
 }
+
+// Mailbox protocol.  A one-block kernel calls MailPost with the next
+// sequence number after writing up to kMailWords values; the host waits for
+// it with MailWait, which spins briefly and otherwise falls back to a
+// stream synchronisation (so a lost post can never hang the host).
+constexpr int kMailWords = 32;
+void MailInit(gg_graph* g);
+// Returns the sequence number the next MailPost must carry.
+inline unsigned long long MailNext(gg_graph* g) { return ++g->mail_seq; }
+// Waits until the mailbox holds `seq`; copies values[0 .. k) out.
+void MailWait(gg_graph* g, unsigned long long seq, unsigned long long* values, int k);
 
 inline unsigned GridFor(int64_t items, int block, unsigned cap = 1u << 20) {
   int64_t b = (items + block - 1) / block;
@@ -221,6 +249,17 @@ __device__ __forceinline__ int BlockExclusiveScan(int v, int* total, int* sh) {
   __syncthreads();
   *total = tot;
   return wave_base + inc - v;
+}
+
+// Writes values to the host mailbox, then the sequence number, with system
+// scope so the host sees the values before the sequence number.  Call from
+// one thread.
+__device__ __forceinline__ void MailPost(unsigned long long* mb, const unsigned long long* values,
+                                         int k, unsigned long long seq) {
+  for (int i = 0; i < k; i++)
+    __hip_atomic_store(&mb[1 + i], values[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __threadfence_system();
+  __hip_atomic_store(&mb[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Block-wide sum, result valid in every thread.  `sh` needs BS/64 slots.

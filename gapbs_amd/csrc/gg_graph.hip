@@ -74,6 +74,33 @@ size_t Workspace::Bytes() const {
   return b;
 }
 
+void MailInit(gg_graph* g) {
+  if (g->mail_host) return;
+  void* p = nullptr;
+  GG_HIP(hipHostMalloc(&p, sizeof(unsigned long long) * (kMailWords + 1),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(p, 0, sizeof(unsigned long long) * (kMailWords + 1));
+  void* d = nullptr;
+  GG_HIP(hipHostGetDevicePointer(&d, p, 0));
+  g->mail_host = static_cast<volatile unsigned long long*>(p);
+  g->mail_dev = static_cast<unsigned long long*>(d);
+}
+
+void MailWait(gg_graph* g, unsigned long long seq, unsigned long long* values, int k) {
+  auto seen = [&] { return __atomic_load_n(&g->mail_host[0], __ATOMIC_ACQUIRE) >= seq; };
+  const auto t0 = std::chrono::steady_clock::now();
+  while (!seen()) {
+    __builtin_ia32_pause();
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      // Long step (or a failure): block on the stream instead of spinning.
+      GG_HIP(hipStreamSynchronize(g->stream));
+      GG_REQUIRE(seen(), GG_EHIP, "device did not post its step results");
+      break;
+    }
+  }
+  for (int i = 0; i < k; i++) values[i] = g->mail_host[1 + i];
+}
+
 void BeginTiming(gg_graph* g) { GG_HIP(hipEventRecord(g->ev_start, g->stream)); }
 
 void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt) {
@@ -88,8 +115,7 @@ void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt) {
   }
 }
 
-int ProfileBegin(gg_graph* g, const char* name, double bytes) {
-  if (!g->profile) return -1;
+int ProfileSlot(gg_graph* g, const char* name, double bytes, hipEvent_t* a, hipEvent_t* b) {
   hipEvent_t ev[2];
   for (hipEvent_t& e : ev) {
     if (g->event_pool.empty()) {
@@ -99,14 +125,10 @@ int ProfileBegin(gg_graph* g, const char* name, double bytes) {
       g->event_pool.pop_back();
     }
   }
-  GG_HIP(hipEventRecord(ev[0], g->stream));
   g->pending.push_back(gg_graph::PendingLaunch{name, ev[0], ev[1], bytes});
+  *a = ev[0];
+  *b = ev[1];
   return static_cast<int>(g->pending.size()) - 1;
-}
-
-void ProfileEnd(gg_graph* g, int idx) {
-  if (idx < 0) return;
-  GG_HIP(hipEventRecord(g->pending[idx].b, g->stream));
 }
 
 void CreditBytes(gg_graph* g, int idx, double bytes) {
@@ -127,6 +149,72 @@ void SettleProfile(gg_graph* g) {
   }
   g->pending.clear();
 }
+
+namespace {
+
+// Upload-time validation: offsets non-decreasing, every destination in
+// [0, n).  Kernels index per-vertex arrays with destinations unchecked, so a
+// malformed graph is rejected here instead of faulting later.
+__global__ __launch_bounds__(256) void CheckOffsets(const int64_t* __restrict__ off, int64_t n,
+                                                    unsigned long long* max_deg, int* bad) {
+  __shared__ unsigned long long s_max[4];
+  unsigned long long mx = 0;
+  bool neg = false;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t d = off[i + 1] - off[i];
+    neg |= d < 0;
+    if (d > 0 && static_cast<unsigned long long>(d) > mx) mx = static_cast<unsigned long long>(d);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = __shfl_xor(mx, o);
+    mx = t > mx ? t : mx;
+  }
+  if (LaneId() == 0) s_max[threadIdx.x / 64] = mx;
+  if (__ballot(neg) && LaneId() == 0) *bad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) mx = s_max[w] > mx ? s_max[w] : mx;
+    mx = s_max[0] > mx ? s_max[0] : mx;
+    if (mx) atomicMax(max_deg, mx);
+  }
+}
+
+__global__ __launch_bounds__(256) void CheckDests(const int32_t* __restrict__ dst, int64_t entries,
+                                                  int width, int64_t n, int* bad) {
+  bool out = false;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < entries;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t v = dst[i * width];
+    out |= v < 0 || v >= n;
+  }
+  if (__ballot(out) && LaneId() == 0) *bad = 1;
+}
+
+void ValidateSide(const int64_t* off, const int32_t* dst, int64_t entries, int width, int64_t n,
+                  int64_t* max_degree) {
+  struct Flags {
+    unsigned long long max_deg;
+    int bad;
+  };
+  Flags* d = static_cast<Flags*>(DeviceAlloc(sizeof(Flags)));
+  Flags h{0, 0};
+  try {
+    GG_HIP(hipMemcpy(d, &h, sizeof(Flags), hipMemcpyHostToDevice));
+    CheckOffsets<<<GridFor(n, 256, 1024), 256>>>(off, n, &d->max_deg, &d->bad);
+    if (entries) CheckDests<<<GridFor(entries, 256, 2048), 256>>>(dst, entries, width, n, &d->bad);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpy(&h, d, sizeof(Flags), hipMemcpyDeviceToHost));
+  } catch (...) {
+    DeviceFree(d, sizeof(Flags));
+    throw;
+  }
+  DeviceFree(d, sizeof(Flags));
+  GG_REQUIRE(!h.bad, GG_EINVAL, "graph has decreasing offsets or a destination outside [0, n)");
+  if (static_cast<int64_t>(h.max_deg) > *max_degree) *max_degree = static_cast<int64_t>(h.max_deg);
+}
+
+}  // namespace
 
 }  // namespace gg
 
@@ -201,6 +289,11 @@ int gg_graph_upload(int64_t num_nodes, const int64_t* out_offsets, const int32_t
         g->in_off = g->out_off;
         g->in_dst = g->out_dst;
       }
+      GG_REQUIRE(out_offsets[num_nodes] == e_out, GG_EINVAL, "offsets[n] != entries");
+      MailInit(g);
+      ValidateSide(g->out_off, g->out_dst, g->out_entries, width, num_nodes, &g->max_degree);
+      if (g->directed)
+        ValidateSide(g->in_off, g->in_dst, g->in_entries, width, num_nodes, &g->max_degree);
     } catch (...) {
       gg_graph_free(g);
       throw;
@@ -223,6 +316,7 @@ int gg_graph_free(gg_graph* g) {
       DeviceFree(g->in_dst, sizeof(int32_t) * width * g->in_entries);
     }
     if (g->pinned) (void)hipHostFree(g->pinned);
+    if (g->mail_host) (void)hipHostFree(const_cast<unsigned long long*>(g->mail_host));
     for (auto& p : g->pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);

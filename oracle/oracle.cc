@@ -51,7 +51,20 @@ void orc_bfs_depth(int64_t n, const int64_t* off, const int32_t* dst, int32_t sr
 double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
                  const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
                  int32_t* parent, orc_bfs_stats* st) {
+  return orc_dobfs_trace(n, off, dst, ioff, idst, e_dir, src, alpha, beta, parent, st, nullptr, 0,
+                         nullptr);
+}
+
+double orc_dobfs_trace(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
+                       const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
+                       int32_t* parent, orc_bfs_stats* st, orc_bfs_step* steps, int max_steps,
+                       int* nsteps) {
   const auto t0 = Clock::now();
+  int ns = 0;
+  auto record = [&](int bu, int64_t fr, int64_t vx, int64_t ex, int64_t disc) {
+    if (steps && ns < max_steps) steps[ns] = orc_bfs_step{bu, 0, fr, vx, ex, disc};
+    ns++;
+  };
   orc_bfs_stats s;
   std::memset(&s, 0, sizeof(s));
   const int64_t nw = (n + 63) / 64;
@@ -99,6 +112,7 @@ double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_
           }
           next[w] = word;
         }
+        record(1, old_awake, vx, ex, aw);
         awake = aw;
         s.edges_examined += ex;
         s.vertices_examined += vx;
@@ -148,6 +162,7 @@ double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_
         }
         std::copy(local.begin(), local.end(), queue.begin() + at);
       }
+      record(0, static_cast<int64_t>(qe - qb), static_cast<int64_t>(qe - qb), ex, found);
       s.vertices_examined += static_cast<int64_t>(qe - qb);
       s.edges_examined += ex;
       s.reached += found;
@@ -162,6 +177,7 @@ double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_
   for (int64_t i = 0; i < n; i++)
     if (parent[i] < -1) parent[i] = -1;
   if (st) *st = s;
+  if (nsteps) *nsteps = ns;
   return Since(t0);
 }
 

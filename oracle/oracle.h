@@ -25,6 +25,16 @@ typedef struct orc_bfs_stats {
   int64_t reached, reached_edges;
 } orc_bfs_stats;
 
+/* Per-step record of orc_dobfs (filled when steps != NULL, up to max_steps). */
+typedef struct orc_bfs_step {
+  int32_t bottom_up;
+  int32_t pad;
+  int64_t frontier;          /* TD: queue size; BU: vertices discovered in the previous step */
+  int64_t vertices_examined; /* TD: frontier size; BU: unvisited vertices scanned */
+  int64_t edges_examined;
+  int64_t discovered;
+} orc_bfs_step;
+
 /* Serial level-synchronous BFS depths (textbook; src/bfs.cc:204-215). */
 void orc_bfs_depth(int64_t n, const int64_t* off, const int32_t* dst, int32_t src, int32_t* depth);
 
@@ -34,6 +44,12 @@ void orc_bfs_depth(int64_t n, const int64_t* off, const int32_t* dst, int32_t sr
 double orc_dobfs(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
                  const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
                  int32_t* parent, orc_bfs_stats* st);
+/* Same, also recording each step into steps[0 .. max_steps) and returning
+ * the step count through *nsteps. */
+double orc_dobfs_trace(int64_t n, const int64_t* off, const int32_t* dst, const int64_t* ioff,
+                       const int32_t* idst, int64_t e_dir, int32_t src, int alpha, int beta,
+                       int32_t* parent, orc_bfs_stats* st, orc_bfs_step* steps, int max_steps,
+                       int* nsteps);
 
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */

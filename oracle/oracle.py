@@ -27,6 +27,12 @@ class BFSStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BFSStep(C.Structure):
+    _fields_ = [("bottom_up", C.c_int32), ("pad", C.c_int32), ("frontier", C.c_int64),
+                ("vertices_examined", C.c_int64), ("edges_examined", C.c_int64),
+                ("discovered", C.c_int64)]
+
+
 def lib() -> C.CDLL:
     global _LIB
     if _LIB is None:
@@ -39,6 +45,9 @@ def lib() -> C.CDLL:
         L.orc_dobfs.argtypes = [C.c_int64, _i64p, _i32p, _i64p, _i32p, C.c_int64, C.c_int32,
                                 C.c_int, C.c_int, _i32p, C.POINTER(BFSStats)]
         L.orc_dobfs.restype = C.c_double
+        L.orc_dobfs_trace.argtypes = L.orc_dobfs.argtypes + [C.POINTER(BFSStep), C.c_int,
+                                                             C.POINTER(C.c_int)]
+        L.orc_dobfs_trace.restype = C.c_double
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -73,6 +82,22 @@ def dobfs(g, src: int, alpha: int = 15, beta: int = 18):
                            _p(ioff, C.c_int64), _p(idst, C.c_int32), g.num_edges_directed,
                            src, alpha, beta, _p(parent, C.c_int32), C.byref(st))
     return parent, secs, st.as_dict()
+
+
+def dobfs_steps(g, src: int, alpha: int = 15, beta: int = 18) -> list[dict]:
+    """Per-step direction and work of the reference schedule."""
+    off, dst = _csr(g)
+    ioff, idst = g.in_offsets, np.ascontiguousarray(g.in_neighbors)
+    parent = np.empty(g.num_nodes, dtype=np.int32)
+    st = BFSStats()
+    steps = (BFSStep * 4096)()
+    ns = C.c_int()
+    lib().orc_dobfs_trace(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32),
+                          _p(ioff, C.c_int64), _p(idst, C.c_int32), g.num_edges_directed, src,
+                          alpha, beta, _p(parent, C.c_int32), C.byref(st), steps, 4096,
+                          C.byref(ns))
+    return [{k: getattr(steps[i], k) for k, _ in BFSStep._fields_ if k != "pad"}
+            for i in range(min(ns.value, 4096))]
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

@@ -95,6 +95,21 @@ def test_bfs_closed_forms():
     assert k.verify_bfs(11, res.parent)
 
 
+@pytest.mark.parametrize("alpha", [15, 1000000])
+def test_bfs_hub_chunks_star_of_stars(alpha):
+    # Two hubs of 9000 and 5000 leaves (> 4096-edge chunks) joined by an edge,
+    # so the top-down step from a hub spreads its edges over the grid.
+    print(f"[bfs] star of stars alpha={alpha}", flush=True)
+    a = np.zeros(9000, np.int32)
+    b = np.full(5000, 1, np.int32)
+    u = np.concatenate([a, b, [0]])
+    v = np.concatenate([np.arange(2, 9002), np.arange(9002, 14002), [1]])
+    g = Graph.from_edges(u, v)
+    dg = gpu.DeviceGraph(g)
+    for r in (0, 5, 9500):
+        check_root(g, dg, r, alpha)
+
+
 def test_bfs_disconnected_and_isolated_source():
     print("[bfs] disconnected", flush=True)
     g = Graph.from_edges([0, 1, 3, 5, 9], [1, 2, 4, 6, 9])  # 9-9 self-loop is squished away
