@@ -4,6 +4,7 @@ The functions named like the reference's kernels take the same arguments and
 return the same result arrays as the reference functions they replace:
 
     DOBFS(g, source, alpha=15, beta=18)      src/bfs.cc:122-179   -> parent[n]
+    DeltaStep(g, source, delta)              src/sssp.cc:87-153   -> dist[n]
 
 A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
 call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
@@ -55,7 +56,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_sssp", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -78,6 +79,7 @@ def lib() -> C.CDLL:
         L.gg_graph_device_bytes.argtypes = [vp]
         L.gg_graph_device_bytes.restype = C.c_int64
         L.gg_bfs.argtypes = [vp, C.c_int32, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(Stats)]
+        L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -184,3 +186,26 @@ def bfs(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18,
 def DOBFS(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18) -> np.ndarray:
     """Parent array of a direction-optimizing BFS (src/bfs.cc:122-179)."""
     return bfs(dg, source, alpha, beta, want_depth=False).parent
+
+
+K_DIST_INF = (2**31 - 1) // 2  # src/sssp.cc:64
+
+
+@dataclass
+class SSSPResult:
+    dist: np.ndarray | None
+    stats: dict
+
+
+def sssp(dg: DeviceGraph, source: int, delta: int, want_dist: bool = True) -> SSSPResult:
+    """gg_sssp; with want_dist False the distances stay on the device."""
+    dist = np.empty(dg.num_nodes, dtype=np.int32) if want_dist else None
+    st = Stats()
+    _check(lib().gg_sssp(dg.handle, source, delta, _p(dist, C.c_int32) if want_dist else None,
+                         C.byref(st)))
+    return SSSPResult(dist, st.as_dict())
+
+
+def DeltaStep(dg: DeviceGraph, source: int, delta: int) -> np.ndarray:
+    """Shortest distances by delta-stepping (src/sssp.cc:87-153)."""
+    return sssp(dg, source, delta).dist

@@ -79,6 +79,13 @@ int64_t gg_graph_device_bytes(const gg_graph* g);
 int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* parent, int32_t* depth,
            gg_stats* stats);
 
+/* Delta-stepping single-source shortest paths from `source` with bucket
+ * width `delta` on a weighted upload.
+ * Replaces DeltaStep(const WGraph&, NodeID source, WeightT delta, bool
+ * logging) (src/sssp.cc:87-153).  dist[n] (nullable: kept on the device):
+ * shortest distance, 1073741823 (INT32_MAX / 2) if unreached. */
+int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist, gg_stats* stats);
+
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP
  * events on the handle's stream and credited with its algorithmic HBM bytes

@@ -9,6 +9,9 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <functional>
+#include <limits>
+#include <queue>
 #include <vector>
 
 namespace {
@@ -178,6 +181,106 @@ double orc_dobfs_trace(int64_t n, const int64_t* off, const int32_t* dst, const 
     if (parent[i] < -1) parent[i] = -1;
   if (st) *st = s;
   if (nsteps) *nsteps = ns;
+  return Since(t0);
+}
+
+void orc_dijkstra(int64_t n, const int64_t* off, const int32_t* wdst, int32_t src, int32_t* dist) {
+  const int32_t inf = INT32_MAX / 2;
+  std::fill(dist, dist + n, inf);
+  typedef std::pair<int32_t, int32_t> Item;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+  dist[src] = 0;
+  pq.push(Item(0, src));
+  while (!pq.empty()) {
+    const Item top = pq.top();
+    pq.pop();
+    const int32_t d = top.first, u = top.second;
+    if (d != dist[u]) continue;
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = wdst[2 * e], w = wdst[2 * e + 1];
+      if (d + w < dist[v]) {
+        dist[v] = d + w;
+        pq.push(Item(d + w, v));
+      }
+    }
+  }
+}
+
+// Restates DeltaStep (src/sssp.cc:87-153) and RelaxEdges (68-85).
+double orc_delta_step(int64_t n, const int64_t* off, const int32_t* wdst, int64_t e_dir, int32_t src,
+                      int32_t delta, int32_t* dist) {
+  const auto t0 = Clock::now();
+  const int32_t inf = INT32_MAX / 2;
+  const size_t kMaxBin = std::numeric_limits<size_t>::max() / 2;
+  const size_t kFuse = 1000;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) dist[i] = inf;
+  dist[src] = 0;
+  std::vector<int32_t> frontier(static_cast<size_t>(e_dir > 0 ? e_dir : 1));
+  size_t shared_idx[2] = {0, kMaxBin};
+  size_t tails[2] = {1, 0};
+  frontier[0] = src;
+  auto relax = [&](int32_t u, std::vector<std::vector<int32_t>>& bins) {
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = wdst[2 * e], w = wdst[2 * e + 1];
+      int32_t old_d = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+      const int32_t new_d = __atomic_load_n(&dist[u], __ATOMIC_RELAXED) + w;
+      while (new_d < old_d) {
+        if (__atomic_compare_exchange_n(&dist[v], &old_d, new_d, false, __ATOMIC_RELAXED,
+                                        __ATOMIC_RELAXED)) {
+          const size_t b = static_cast<size_t>(new_d / delta);
+          if (b >= bins.size()) bins.resize(b + 1);
+          bins[b].push_back(v);
+          break;
+        }
+      }
+    }
+  };
+#pragma omp parallel
+  {
+    std::vector<std::vector<int32_t>> bins;
+    size_t iter = 0;
+    while (shared_idx[iter & 1] != kMaxBin) {
+      size_t& cur = shared_idx[iter & 1];
+      size_t& nxt = shared_idx[(iter + 1) & 1];
+      size_t& cur_tail = tails[iter & 1];
+      size_t& nxt_tail = tails[(iter + 1) & 1];
+#pragma omp for nowait schedule(dynamic, 64)
+      for (size_t i = 0; i < cur_tail; i++) {
+        const int32_t u = frontier[i];
+        if (dist[u] >= delta * static_cast<int32_t>(cur)) relax(u, bins);
+      }
+      while (cur < bins.size() && !bins[cur].empty() && bins[cur].size() < kFuse) {
+        std::vector<int32_t> copy;
+        copy.swap(bins[cur]);
+        for (int32_t u : copy) relax(u, bins);
+      }
+      for (size_t b = cur; b < bins.size(); b++)
+        if (!bins[b].empty()) {
+#pragma omp critical
+          nxt = std::min(nxt, b);
+          break;
+        }
+#pragma omp barrier
+#pragma omp single nowait
+      {
+        cur = kMaxBin;
+        cur_tail = 0;
+      }
+      if (nxt < bins.size()) {
+        size_t at;
+#pragma omp atomic capture
+        {
+          at = nxt_tail;
+          nxt_tail += bins[nxt].size();
+        }
+        std::copy(bins[nxt].begin(), bins[nxt].end(), frontier.begin() + at);
+        bins[nxt].clear();
+      }
+      iter++;
+#pragma omp barrier
+    }
+  }
   return Since(t0);
 }
 

@@ -51,6 +51,16 @@ double orc_dobfs_trace(int64_t n, const int64_t* off, const int32_t* dst, const 
                        int32_t* parent, orc_bfs_stats* st, orc_bfs_step* steps, int max_steps,
                        int* nsteps);
 
+/* Serial binary-heap Dijkstra (textbook; the reference verifier's oracle,
+ * src/sssp.cc:164-194).  wdst: {v, w} pairs.  Unreached: INT32_MAX / 2. */
+void orc_dijkstra(int64_t n, const int64_t* off, const int32_t* wdst, int32_t src, int32_t* dist);
+
+/* OpenMP delta-stepping restating DeltaStep (src/sssp.cc:68-153): shared
+ * frontier, thread-local bins, bucket fusion below 1000 entries.  Returns
+ * elapsed seconds. */
+double orc_delta_step(int64_t n, const int64_t* off, const int32_t* wdst, int64_t e_dir, int32_t src,
+                      int32_t delta, int32_t* dist);
+
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);

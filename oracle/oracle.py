@@ -48,6 +48,10 @@ def lib() -> C.CDLL:
         L.orc_dobfs_trace.argtypes = L.orc_dobfs.argtypes + [C.POINTER(BFSStep), C.c_int,
                                                              C.POINTER(C.c_int)]
         L.orc_dobfs_trace.restype = C.c_double
+        L.orc_dijkstra.argtypes = [C.c_int64, _i64p, _i32p, C.c_int32, _i32p]
+        L.orc_dijkstra.restype = None
+        L.orc_delta_step.argtypes = [C.c_int64, _i64p, _i32p, C.c_int64, C.c_int32, C.c_int32, _i32p]
+        L.orc_delta_step.restype = C.c_double
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -98,6 +102,32 @@ def dobfs_steps(g, src: int, alpha: int = 15, beta: int = 18) -> list[dict]:
                           C.byref(ns))
     return [{k: getattr(steps[i], k) for k, _ in BFSStep._fields_ if k != "pad"}
             for i in range(min(ns.value, 4096))]
+
+
+K_DIST_INF = (2**31 - 1) // 2
+
+
+def _wcsr(g):
+    if not g.weighted:
+        raise ValueError("needs a weighted graph")
+    return g.out_offsets, np.ascontiguousarray(g.out_dests_raw.reshape(-1))
+
+
+def dijkstra(g, src: int) -> np.ndarray:
+    """Serial Dijkstra distances (src/sssp.cc:164-194); unreached = INT32_MAX/2."""
+    off, wd = _wcsr(g)
+    dist = np.empty(g.num_nodes, dtype=np.int32)
+    lib().orc_dijkstra(g.num_nodes, _p(off, C.c_int64), _p(wd, C.c_int32), src, _p(dist, C.c_int32))
+    return dist
+
+
+def delta_step(g, src: int, delta: int):
+    """OpenMP delta-stepping (src/sssp.cc:87-153): (dist, seconds)."""
+    off, wd = _wcsr(g)
+    dist = np.empty(g.num_nodes, dtype=np.int32)
+    secs = lib().orc_delta_step(g.num_nodes, _p(off, C.c_int64), _p(wd, C.c_int32),
+                                g.num_edges_directed, src, delta, _p(dist, C.c_int32))
+    return dist, secs
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

@@ -116,3 +116,20 @@ def bfs_time(rg: RefGraph, roots) -> float:
     rg.lib.ref_bfs.argtypes = [C.c_void_p, _i32p, C.c_int, _i32p]
     roots = np.ascontiguousarray(roots, dtype=np.int32)
     return rg.lib.ref_bfs(rg.h, _p(roots, C.c_int32), len(roots), None)
+
+
+def sssp(rg: RefGraph, roots, delta: int) -> tuple[np.ndarray, float]:
+    """Reference DeltaStep from each root: (dist[k, n], summed seconds)."""
+    rg.lib.ref_sssp.restype = C.c_double
+    rg.lib.ref_sssp.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_int32, _i32p]
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    out = np.empty((len(roots), rg.n), dtype=np.int32)
+    secs = rg.lib.ref_sssp(rg.h, _p(roots, C.c_int32), len(roots), delta, _p(out, C.c_int32))
+    return out, secs
+
+
+def sssp_time(rg: RefGraph, roots, delta: int) -> float:
+    rg.lib.ref_sssp.restype = C.c_double
+    rg.lib.ref_sssp.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_int32, _i32p]
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    return rg.lib.ref_sssp(rg.h, _p(roots, C.c_int32), len(roots), delta, None)

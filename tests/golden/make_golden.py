@@ -33,7 +33,7 @@ def sha(a) -> str:
 
 
 def main():
-    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}}
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}}
     # The reference's own expected stats lines (test/test.mk:44-49).
     for name in ("g10", "u10"):
         with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
@@ -60,6 +60,16 @@ def main():
             digests.append({"root": int(r), "depth_sha256": sha(d),
                             "reached": int((d >= 0).sum()), "max_depth": int(d.max())})
         out["bfs"][tag] = digests
+    # Reference DeltaStep distance digests (delta 2, as benchmark/bench.mk:136-147).
+    for tag, g in (("g10w", Graph.kronecker(10, weighted=True)),
+                   ("g16w", Graph.kronecker(16, weighted=True)),
+                   ("u12w", Graph.uniform(12, weighted=True))):
+        roots = ref.pick_sources(g.strip_weights(), 16)
+        dists, _ = ref.sssp(ref.RefGraph("sssp", g), roots, 2)
+        out["sssp"][tag] = [{"root": int(r), "dist_sha256": sha(d),
+                             "reached": int((d < (2**31 - 1) // 2).sum()),
+                             "max_dist": int(d[d < (2**31 - 1) // 2].max())}
+                            for r, d in zip(roots, dists)]
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote golden.json")
