@@ -5,6 +5,7 @@ return the same result arrays as the reference functions they replace:
 
     DOBFS(g, source, alpha=15, beta=18)      src/bfs.cc:122-179   -> parent[n]
     DeltaStep(g, source, delta)              src/sssp.cc:87-153   -> dist[n]
+    PageRankPull(g, max_iters, epsilon)      src/pr_spmv.cc:35-61 -> scores[n]
 
 A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
 call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
@@ -56,7 +57,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_sssp", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -80,6 +81,8 @@ def lib() -> C.CDLL:
         L.gg_graph_device_bytes.restype = C.c_int64
         L.gg_bfs.argtypes = [vp, C.c_int32, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(Stats)]
         L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
+        L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
+                                  C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -209,3 +212,27 @@ def sssp(dg: DeviceGraph, source: int, delta: int, want_dist: bool = True) -> SS
 def DeltaStep(dg: DeviceGraph, source: int, delta: int) -> np.ndarray:
     """Shortest distances by delta-stepping (src/sssp.cc:87-153)."""
     return sssp(dg, source, delta).dist
+
+
+PR_FP32, PR_FP64 = 0, 1
+
+
+@dataclass
+class PRResult:
+    scores: np.ndarray | None
+    stats: dict
+
+
+def pagerank(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4, fp64: bool = False,
+             want_scores: bool = True) -> PRResult:
+    """gg_pagerank: FP32 compensated (default) or FP64 accumulation."""
+    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    st = Stats()
+    _check(lib().gg_pagerank(dg.handle, max_iters, epsilon, PR_FP64 if fp64 else PR_FP32,
+                             _p(out, C.c_float) if want_scores else None, C.byref(st)))
+    return PRResult(out, st.as_dict())
+
+
+def PageRankPull(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:
+    """Scores of Jacobi pull PageRank (src/pr_spmv.cc:35-61)."""
+    return pagerank(dg, max_iters, epsilon).scores

@@ -86,6 +86,17 @@ int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* parent, in
  * shortest distance, 1073741823 (INT32_MAX / 2) if unreached. */
 int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist, gg_stats* stats);
 
+/* Pull PageRank with damping 0.85.
+ * Replaces PageRankPull(const Graph&, int max_iters, double epsilon, bool
+ * logging) (src/pr_spmv.cc:35-61): Jacobi iterations until the L1 change
+ * (accumulated in double) drops below epsilon or max_iters is reached.
+ * mode GG_PR_FP32: float scores with compensated per-vertex sums;
+ * mode GG_PR_FP64: double scores and sums.  scores[n] (nullable) receives
+ * float scores either way.  stats->iterations and stats->residual report
+ * the iterations run and the last L1 change. */
+enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1 };
+int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* scores, gg_stats* stats);
+
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP
  * events on the handle's stream and credited with its algorithmic HBM bytes

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <functional>
@@ -282,6 +283,56 @@ double orc_delta_step(int64_t n, const int64_t* off, const int32_t* wdst, int64_
     }
   }
   return Since(t0);
+}
+
+}  // extern "C"
+
+namespace {
+template <class T>
+int PageRankPull(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
+                 int max_iters, double epsilon, T* scores, double* error_out) {
+  // Constants computed in float exactly as pr_spmv.cc:32, 38-39 does.
+  const float kDamp = 0.85f;
+  const float init_f = 1.0f / n;
+  const float base_f = (1.0f - kDamp) / n;
+  const T init = init_f, base = base_f, damp = kDamp;
+  std::vector<T> contrib(static_cast<size_t>(n));
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) scores[i] = init;
+  int iters = 0;
+  double error = 0;
+  for (int it = 0; it < max_iters; it++) {
+    error = 0;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) contrib[i] = scores[i] / static_cast<T>(out_off[i + 1] - out_off[i]);
+#pragma omp parallel for reduction(+ : error) schedule(dynamic, 16384)
+    for (int64_t u = 0; u < n; u++) {
+      T total = 0;
+      for (int64_t e = in_off[u]; e < in_off[u + 1]; e++) total += contrib[in_dst[e]];
+      const T old = scores[u];
+      // A product then a sum, as the reference's expression rounds.
+      volatile T prod = damp * total;
+      scores[u] = base + prod;
+      error += std::fabs(static_cast<double>(scores[u]) - static_cast<double>(old));
+    }
+    iters++;
+    if (error < epsilon) break;
+  }
+  *error_out = error;
+  return iters;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_pr_pull_f32(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
+                    int max_iters, double epsilon, float* scores, double* error) {
+  return PageRankPull<float>(n, in_off, in_dst, out_off, max_iters, epsilon, scores, error);
+}
+
+int orc_pr_pull_f64(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
+                    int max_iters, double epsilon, double* scores, double* error) {
+  return PageRankPull<double>(n, in_off, in_dst, out_off, max_iters, epsilon, scores, error);
 }
 
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth) {

@@ -61,6 +61,17 @@ void orc_dijkstra(int64_t n, const int64_t* off, const int32_t* wdst, int32_t sr
 double orc_delta_step(int64_t n, const int64_t* off, const int32_t* wdst, int64_t e_dir, int32_t src,
                       int32_t delta, int32_t* dist);
 
+/* Jacobi pull PageRank restating PageRankPull (src/pr_spmv.cc:35-61) in
+ * float (bit-identical to the reference: each vertex's sum is taken in
+ * in-neighbour order) -- orc_pr_pull_f32 -- and the same iteration in
+ * double with the reference's float constants widened -- orc_pr_pull_f64,
+ * the FP64 oracle.  in_off/in_dst: inverse CSR; out_off: out-degrees.
+ * Return the iterations run; *error receives the last L1 change. */
+int orc_pr_pull_f32(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
+                    int max_iters, double epsilon, float* scores, double* error);
+int orc_pr_pull_f64(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
+                    int max_iters, double epsilon, double* scores, double* error);
+
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);

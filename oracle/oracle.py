@@ -52,6 +52,10 @@ def lib() -> C.CDLL:
         L.orc_dijkstra.restype = None
         L.orc_delta_step.argtypes = [C.c_int64, _i64p, _i32p, C.c_int64, C.c_int32, C.c_int32, _i32p]
         L.orc_delta_step.restype = C.c_double
+        for nm, ct in (("orc_pr_pull_f32", C.c_float), ("orc_pr_pull_f64", C.c_double)):
+            getattr(L, nm).argtypes = [C.c_int64, _i64p, _i32p, _i64p, C.c_int, C.c_double,
+                                       C.POINTER(ct), C.POINTER(C.c_double)]
+            getattr(L, nm).restype = C.c_int
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -128,6 +132,19 @@ def delta_step(g, src: int, delta: int):
     secs = lib().orc_delta_step(g.num_nodes, _p(off, C.c_int64), _p(wd, C.c_int32),
                                 g.num_edges_directed, src, delta, _p(dist, C.c_int32))
     return dist, secs
+
+
+def pagerank(g, max_iters: int = 20, epsilon: float = 1e-4, fp64: bool = False):
+    """Jacobi pull PageRank (src/pr_spmv.cc:35-61): (scores, iterations, last L1 change).
+
+    fp64=False restates the reference in float; fp64=True is the FP64 oracle."""
+    ioff, idst = g.in_offsets, np.ascontiguousarray(g.in_neighbors)
+    scores = np.empty(g.num_nodes, dtype=np.float64 if fp64 else np.float32)
+    err = C.c_double()
+    fn = lib().orc_pr_pull_f64 if fp64 else lib().orc_pr_pull_f32
+    it = fn(g.num_nodes, _p(ioff, C.c_int64), _p(idst, C.c_int32), _p(g.out_offsets, C.c_int64),
+            max_iters, epsilon, _p(scores, C.c_double if fp64 else C.c_float), C.byref(err))
+    return scores, it, err.value
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

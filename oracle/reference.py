@@ -133,3 +133,13 @@ def sssp_time(rg: RefGraph, roots, delta: int) -> float:
     rg.lib.ref_sssp.argtypes = [C.c_void_p, _i32p, C.c_int, C.c_int32, _i32p]
     roots = np.ascontiguousarray(roots, dtype=np.int32)
     return rg.lib.ref_sssp(rg.h, _p(roots, C.c_int32), len(roots), delta, None)
+
+
+def pagerank(name: str, g, max_iters: int = 20, epsilon: float = 1e-4):
+    """Reference PageRankPullGS (name="pr") or PageRankPull (name="pr_spmv"): (scores, secs)."""
+    rg = RefGraph(name, g)
+    rg.lib.ref_pr.restype = C.c_double
+    rg.lib.ref_pr.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_float)]
+    out = np.empty(g.num_nodes, dtype=np.float32)
+    secs = rg.lib.ref_pr(rg.h, max_iters, epsilon, _p(out, C.c_float))
+    return out, secs

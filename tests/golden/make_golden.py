@@ -33,7 +33,7 @@ def sha(a) -> str:
 
 
 def main():
-    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}}
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}}
     # The reference's own expected stats lines (test/test.mk:44-49).
     for name in ("g10", "u10"):
         with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
@@ -70,6 +70,13 @@ def main():
                              "reached": int((d < (2**31 - 1) // 2).sum()),
                              "max_dist": int(d[d < (2**31 - 1) // 2].max())}
                             for r, d in zip(roots, dists)]
+    # Reference PageRankPull (pr_spmv) float scores, tolerance 1e-4, <= 20 iterations.
+    for tag, g in (("g10", Graph.kronecker(10)), ("g16", Graph.kronecker(16)),
+                   ("u12", Graph.uniform(12))):
+        scores, _ = ref.pagerank("pr_spmv", g, 20, 1e-4)
+        top = np.argsort(-scores, kind="stable")[:5]
+        out["pr"][tag] = {"scores_sha256": sha(scores), "sum": float(scores.astype(np.float64).sum()),
+                          "top5": [[int(i), float(scores[i])] for i in top]}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote golden.json")

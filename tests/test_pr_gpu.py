@@ -1,0 +1,92 @@
+"""GPU PageRank parity: gg_pagerank against the FP64 oracle.
+
+Tolerances (scores are O(1/N)): FP32 compensated path within 1e-6 max-abs
+of the FP64 oracle (the stated target) and within 1e-5 relative per vertex;
+FP64 path within 1e-11 relative.  Iteration counts must equal the oracle's,
+and the reference's PRVerifier rule must pass.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import cached_graph
+from gapbs_amd import gpu
+from gapbs_amd.graph import Graph
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+_DG = {}
+
+
+def dev(tag, make):
+    if tag not in _DG:
+        _DG[tag] = gpu.DeviceGraph(cached_graph(tag, make))
+    return _DG[tag]
+
+
+GRAPHS = [("g10", lambda: Graph.kronecker(10)), ("g16", lambda: Graph.kronecker(16)),
+          ("u12", lambda: Graph.uniform(12)), ("k18s", lambda: Graph.kronecker(18, seed=8675309))]
+
+
+@pytest.mark.parametrize("tag,make", GRAPHS)
+def test_pr_fp32_and_fp64_against_fp64_oracle(tag, make):
+    print(f"[pr] {tag}", flush=True)
+    g = cached_graph(tag, make)
+    dg = dev(tag, make)
+    want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+    r32 = gpu.pagerank(dg, 20, 1e-4)
+    r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
+    assert r32.stats["iterations"] == it and r64.stats["iterations"] == it
+    s32 = r32.scores.astype(np.float64)
+    assert np.abs(s32 - want).max() < 1e-6
+    assert (np.abs(s32 - want) / want).max() < 1e-5
+    assert (np.abs(r64.scores.astype(np.float64) - want) / want).max() < 1e-6  # float output rounding
+    assert g.verify_pr(r32.scores, 1e-4)
+
+
+def test_pr_fixed_iterations_precision_against_reference_float():
+    # With epsilon 0 both paths run exactly max_iters; the compensated FP32
+    # path must be at least as close to FP64 as the reference's float sums.
+    g = cached_graph("g16", GRAPHS[1][1])
+    dg = dev("g16", GRAPHS[1][1])
+    want, _, _ = oracle.pagerank(g, 20, 0.0, fp64=True)
+    ref32, _, _ = oracle.pagerank(g, 20, 0.0)
+    got = gpu.pagerank(dg, 20, 0.0).scores
+    e_gpu = np.abs(got.astype(np.float64) - want).sum()
+    e_ref = np.abs(ref32.astype(np.float64) - want).sum()
+    assert e_gpu <= e_ref * 1.01
+
+
+def test_pr_directed_and_dangling():
+    print("[pr] directed", flush=True)
+    rng = np.random.default_rng(9)
+    n, m = 5000, 60000
+    g = Graph.from_edges(rng.integers(0, n, m), rng.integers(0, n, m), symmetrize=False)
+    dg = gpu.DeviceGraph(g)
+    want, it, _ = oracle.pagerank(g, 15, 0.0, fp64=True)
+    got = gpu.pagerank(dg, 15, 0.0, fp64=True)
+    assert got.stats["iterations"] == 15
+    np.testing.assert_allclose(got.scores, want.astype(np.float32), rtol=1e-6, atol=0)
+
+
+def test_pr_hub_block_bin():
+    print("[pr] hubs", flush=True)
+    u = np.concatenate([np.zeros(5000, np.int32), np.arange(1, 5000)])
+    v = np.concatenate([np.arange(1, 5001), np.arange(2, 5001)])
+    g = Graph.from_edges(u, v)
+    dg = gpu.DeviceGraph(g)
+    want, it, _ = oracle.pagerank(g, 20, 1e-6, fp64=True)
+    got = gpu.pagerank(dg, 20, 1e-6)
+    assert got.stats["iterations"] == it
+    assert (np.abs(got.scores - want) / want).max() < 1e-5
+
+
+def test_pr_zero_iterations_and_errors():
+    dg = dev("g10", GRAPHS[0][1])
+    g = cached_graph("g10", GRAPHS[0][1])
+    r = gpu.pagerank(dg, 0)
+    assert r.stats["iterations"] == 0
+    assert np.all(r.scores == np.float32(1.0) / np.float32(g.num_nodes))
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.pagerank(dg, -1)
+    assert e.value.code == -22
