@@ -1,0 +1,226 @@
+// Afforest connected components on gfx950.
+//
+// Reference: Afforest, Link, Compress, SampleFrequentElement
+// (src/cc.cc:41-151).  Kept:
+//   * Link hooks the higher root under the lower one with a compare-and-swap
+//     and retries from the grandparents (cc.cc:41-55), so every component
+//     ends labelled with its smallest vertex id -- the result is exactly the
+//     reference's, not just the same partition;
+//   * neighbor_rounds sampled rounds (default 2): each vertex links to its
+//     r-th neighbour, then Compress (cc.cc:104-117);
+//   * the most frequent label among 1024 samples drawn with a default-seeded
+//     std::mt19937 and std::uniform_int_distribution over [0, N-1]
+//     (cc.cc:69-92) names the component whose vertices skip the final link;
+//   * the final link over neighbours from offset neighbor_rounds (plus all
+//     in-neighbours for directed graphs) and a last Compress (cc.cc:120-150).
+// GPU-specific: every read of comp[] inside Link and Compress is a relaxed
+// agent-scope atomic load.  Plain loads may be served from a stale copy in
+// another XCD's L2 during the kernel, and a retry loop that keeps reading a
+// stale root would never finish; coherent loads keep the reference's
+// progress guarantee.  The final link expands vertices over the grid with
+// gg_expand.h.
+#include <random>
+#include <unordered_map>
+
+#include "gg_common.h"
+#include "gg_expand.h"
+
+namespace gg {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kSamples = 1024;
+
+__device__ __forceinline__ int32_t Load(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void Link(int32_t u, int32_t v, int32_t* comp) {
+  int32_t p1 = Load(&comp[u]);
+  int32_t p2 = Load(&comp[v]);
+  while (p1 != p2) {
+    const int32_t high = p1 > p2 ? p1 : p2;
+    const int32_t low = p1 + (p2 - high);
+    const int32_t p_high = Load(&comp[high]);
+    if (p_high == low) break;
+    if (p_high == high && atomicCAS(&comp[high], high, low) == high) break;
+    p1 = Load(&comp[Load(&comp[high])]);
+    p2 = Load(&comp[low]);
+  }
+}
+
+__global__ __launch_bounds__(kBS) void CcInit(int32_t* __restrict__ comp, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS)
+    comp[i] = static_cast<int32_t>(i);
+}
+
+// Sampled round r: link u with its r-th out-neighbour, if it has one.
+__global__ __launch_bounds__(kBS) void CcLinkRound(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                   int32_t* comp, int64_t n, int r) {
+  for (int64_t u = blockIdx.x * int64_t(kBS) + threadIdx.x; u < n; u += int64_t(gridDim.x) * kBS) {
+    const int64_t b = off[u];
+    if (off[u + 1] - b > r) Link(static_cast<int32_t>(u), dst[b + r], comp);
+  }
+}
+
+// Pointer jumping until every vertex points at a root.
+__global__ __launch_bounds__(kBS) void CcCompress(int32_t* comp, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS) {
+    int32_t c = Load(&comp[i]);
+    int32_t cc = Load(&comp[c]);
+    while (c != cc) {
+      c = cc;
+      cc = Load(&comp[c]);
+    }
+    if (Load(&comp[i]) != c) comp[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void CcGather(const int32_t* __restrict__ comp, const int32_t* __restrict__ idx,
+                                                int32_t* __restrict__ out, int k) {
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < k; i += gridDim.x * kBS) out[i] = comp[idx[i]];
+}
+
+struct LinkVisit {
+  const int32_t* dst;
+  int32_t* comp;
+  __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
+    if (valid) Link(u, dst[e], comp);
+  }
+};
+
+// Final link: every vertex outside the skipped component links with its
+// neighbours from offset `start` on (start = 0 for the inverse CSR).
+__global__ __launch_bounds__(kBS) void CcFinalLink(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                   int32_t* comp, int64_t n, int32_t skip, int start,
+                                                   EdgeChunk* chunks, int32_t* nchunks,
+                                                   unsigned long long* examined) {
+  __shared__ ExpandShared sh;
+  __shared__ unsigned long long s_red[kBS / kWave];
+  LinkVisit visit{dst, comp};
+  unsigned long long ex = 0;
+  for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < n; tile += int64_t(gridDim.x) * kBS) {
+    const int64_t u = tile + threadIdx.x;
+    int64_t beg = 0, end = 0;
+    if (u < n && comp[u] != skip) {
+      end = off[u + 1];
+      beg = off[u] + start < end ? off[u] + start : end;
+      ex += static_cast<unsigned long long>(end - beg);
+    }
+    ExpandTile(sh, static_cast<int32_t>(u), 0, beg, end, visit, chunks, nchunks, kChunkEdges);
+  }
+  ex = BlockSum<kBS>(ex, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & 63], ex);
+}
+
+__global__ __launch_bounds__(kBS) void CcFinalLinkChunks(const int32_t* __restrict__ dst, int32_t* comp,
+                                                         const EdgeChunk* chunks, const int32_t* nchunks) {
+  LinkVisit visit{dst, comp};
+  ExpandChunks(chunks, *nchunks, visit);
+}
+
+// The reference's sample positions: default-seeded mt19937 and
+// uniform_int_distribution<int32_t>(0, n - 1), 1024 draws (cc.cc:75-80).
+std::vector<int32_t> SamplePositions(int64_t n) {
+  std::mt19937 gen;
+  std::uniform_int_distribution<int32_t> dist(0, static_cast<int32_t>(n - 1));
+  std::vector<int32_t> idx(kSamples);
+  for (int i = 0; i < kSamples; i++) idx[i] = dist(gen);
+  return idx;
+}
+
+}  // namespace
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "CC needs an unweighted graph upload");
+    GG_REQUIRE(neighbor_rounds >= 0, GG_EINVAL, "neighbor_rounds must be >= 0");
+    GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
+    GG_HIP(hipSetDevice(g->device));
+    const int64_t n = g->n;
+    hipStream_t st = g->stream;
+    const dim3 blk(kBS);
+    const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
+    const double N = static_cast<double>(n);
+
+    int32_t* comp = g->Scratch<int32_t>("cc.comp", n);
+    int32_t* sample_idx = g->Scratch<int32_t>("cc.sample_idx", kSamples);
+    int32_t* sample_val = g->Scratch<int32_t>("cc.sample_val", kSamples);
+    unsigned long long* examined = g->Scratch<unsigned long long>("cc.examined", 64);
+    int32_t* nchunks = g->Scratch<int32_t>("cc.nchunks", 2);
+    const bool use_chunks = g->max_degree >= kChunkEdges;
+    EdgeChunk* chunks =
+        use_chunks ? g->Scratch<EdgeChunk>("cc.chunks", MaxChunks(n, g->out_entries > g->in_entries
+                                                                         ? g->out_entries
+                                                                         : g->in_entries))
+                   : nullptr;
+    int32_t* h_samples = g->Pinned<int32_t>(kSamples);
+    if (g->cc_samples_n != n) {  // sample positions depend only on n
+      const std::vector<int32_t> idx = SamplePositions(n);
+      GG_HIP(hipMemcpy(sample_idx, idx.data(), sizeof(int32_t) * kSamples, hipMemcpyHostToDevice));
+      g->cc_samples_n = n;
+    }
+    int64_t launches = 0;
+
+    BeginTiming(g);
+    GG_HIP(hipMemsetAsync(examined, 0, 64 * sizeof(unsigned long long), st));
+    Run(g, "cc_init", 4 * N, CcInit, grid_n, blk, comp, n);
+    launches++;
+    for (int r = 0; r < neighbor_rounds; r++) {
+      // offsets 16 B (both ends), the r-th neighbour, comp read/write.
+      Run(g, "cc_link_round", 16 * N + 4 * N + 8 * N, CcLinkRound, grid_n, blk,
+          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), comp, n, r);
+      Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, comp, n);
+      launches += 2;
+    }
+    // Most frequent label among the samples (ties: smallest label; the
+    // choice only decides which component skips the final link).
+    CcGather<<<dim3(kSamples / kBS), blk, 0, st>>>(comp, sample_idx, sample_val, kSamples);
+    GG_HIP(hipMemcpyAsync(h_samples, sample_val, sizeof(int32_t) * kSamples, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    std::unordered_map<int32_t, int> counts(64);
+    for (int i = 0; i < kSamples; i++) counts[h_samples[i]]++;
+    int32_t skip = 0;
+    int best = -1;
+    for (const auto& kv : counts)
+      if (kv.second > best || (kv.second == best && kv.first < skip)) {
+        best = kv.second;
+        skip = kv.first;
+      }
+    auto final_link = [&](const int64_t* off, const int32_t* dst, int start) {
+      GG_HIP(hipMemsetAsync(nchunks, 0, sizeof(int32_t), st));
+      Run(g, "cc_final_link", 8 * N + 4.0 * (off == g->out_off ? g->out_entries : g->in_entries),
+          CcFinalLink, dim3(GridFor(n, kBS, g->cu_count * 8)), blk, off, dst, comp, n, skip, start,
+          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks, examined);
+      launches++;
+      if (use_chunks) {
+        Run(g, "cc_final_link_chunks", 0, CcFinalLinkChunks, dim3(g->cu_count * 4), blk, dst, comp,
+            static_cast<const EdgeChunk*>(chunks), static_cast<const int32_t*>(nchunks));
+        launches++;
+      }
+    };
+    final_link(g->out_off, g->out_dst, neighbor_rounds);
+    if (g->directed) final_link(g->in_off, g->in_dst, 0);
+    Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, comp, n);
+    launches++;
+    GG_HIP(hipGetLastError());
+    FinishTiming(g, stats, wt);
+    if (comp_out) GG_HIP(hipMemcpy(comp_out, comp, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (stats) {
+      unsigned long long h[64];
+      GG_HIP(hipMemcpy(h, examined, sizeof(h), hipMemcpyDeviceToHost));
+      unsigned long long ex = 0;
+      for (unsigned long long x : h) ex += x;
+      stats->total_ms = wt.ms();
+      stats->iterations = neighbor_rounds + 1;
+      stats->edges_examined = static_cast<int64_t>(ex);
+      stats->launches = launches;
+    }
+  });
+}

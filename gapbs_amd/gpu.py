@@ -6,6 +6,7 @@ return the same result arrays as the reference functions they replace:
     DOBFS(g, source, alpha=15, beta=18)      src/bfs.cc:122-179   -> parent[n]
     DeltaStep(g, source, delta)              src/sssp.cc:87-153   -> dist[n]
     PageRankPull(g, max_iters, epsilon)      src/pr_spmv.cc:35-61 -> scores[n]
+    Afforest(g, neighbor_rounds=2)           src/cc.cc:95-151     -> comp[n]
 
 A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
 call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
@@ -57,7 +58,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -83,6 +84,7 @@ def lib() -> C.CDLL:
         L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
+        L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -236,3 +238,23 @@ def pagerank(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4, fp64: 
 def PageRankPull(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:
     """Scores of Jacobi pull PageRank (src/pr_spmv.cc:35-61)."""
     return pagerank(dg, max_iters, epsilon).scores
+
+
+@dataclass
+class CCResult:
+    comp: np.ndarray | None
+    stats: dict
+
+
+def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True) -> CCResult:
+    """gg_cc: Afforest labels (smallest vertex id per component)."""
+    comp = np.empty(dg.num_nodes, dtype=np.int32) if want_comp else None
+    st = Stats()
+    _check(lib().gg_cc(dg.handle, neighbor_rounds, _p(comp, C.c_int32) if want_comp else None,
+                       C.byref(st)))
+    return CCResult(comp, st.as_dict())
+
+
+def Afforest(dg: DeviceGraph, neighbor_rounds: int = 2) -> np.ndarray:
+    """Component labels (src/cc.cc:95-151)."""
+    return cc(dg, neighbor_rounds).comp

@@ -97,6 +97,12 @@ int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist, gg_stats*
 enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1 };
 int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* scores, gg_stats* stats);
 
+/* Afforest connected components.
+ * Replaces Afforest(const Graph&, bool logging, int32_t neighbor_rounds=2)
+ * (src/cc.cc:95-151).  comp[n] (nullable): component label = the smallest
+ * vertex id in the component, exactly the reference's labels. */
+int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
+
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP
  * events on the handle's stream and credited with its algorithmic HBM bytes

@@ -13,6 +13,8 @@
 #include <functional>
 #include <limits>
 #include <queue>
+#include <random>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -333,6 +335,86 @@ int orc_pr_pull_f32(int64_t n, const int64_t* in_off, const int32_t* in_dst, con
 int orc_pr_pull_f64(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
                     int max_iters, double epsilon, double* scores, double* error) {
   return PageRankPull<double>(n, in_off, in_dst, out_off, max_iters, epsilon, scores, error);
+}
+
+namespace {
+// cc.cc:41-55
+void AfLink(int32_t u, int32_t v, int32_t* comp) {
+  int32_t p1 = __atomic_load_n(&comp[u], __ATOMIC_RELAXED);
+  int32_t p2 = __atomic_load_n(&comp[v], __ATOMIC_RELAXED);
+  while (p1 != p2) {
+    const int32_t high = p1 > p2 ? p1 : p2;
+    const int32_t low = p1 + (p2 - high);
+    int32_t p_high = __atomic_load_n(&comp[high], __ATOMIC_RELAXED);
+    if (p_high == low) break;
+    if (p_high == high &&
+        __atomic_compare_exchange_n(&comp[high], &p_high, low, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+      break;
+    p1 = __atomic_load_n(&comp[__atomic_load_n(&comp[high], __ATOMIC_RELAXED)], __ATOMIC_RELAXED);
+    p2 = __atomic_load_n(&comp[low], __ATOMIC_RELAXED);
+  }
+}
+
+// cc.cc:59-66
+void AfCompress(int64_t n, int32_t* comp) {
+#pragma omp parallel for schedule(dynamic, 16384)
+  for (int64_t i = 0; i < n; i++)
+    while (comp[i] != comp[comp[i]]) comp[i] = comp[comp[i]];
+}
+
+// cc.cc:69-92
+int32_t AfSample(int64_t n, const int32_t* comp) {
+  std::unordered_map<int32_t, int> counts(32);
+  std::mt19937 gen;
+  std::uniform_int_distribution<int32_t> dist(0, static_cast<int32_t>(n - 1));
+  for (int i = 0; i < 1024; i++) counts[comp[dist(gen)]]++;
+  auto best = std::max_element(counts.begin(), counts.end(),
+                               [](const std::pair<const int32_t, int>& a,
+                                  const std::pair<const int32_t, int>& b) { return a.second < b.second; });
+  return best->first;
+}
+}  // namespace
+
+double orc_afforest(int64_t n, const int64_t* off, const int32_t* dst, int directed,
+                    const int64_t* ioff, const int32_t* idst, int neighbor_rounds, int32_t* comp) {
+  const auto t0 = Clock::now();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) comp[i] = static_cast<int32_t>(i);
+  for (int r = 0; r < neighbor_rounds; r++) {
+#pragma omp parallel for schedule(dynamic, 16384)
+    for (int64_t u = 0; u < n; u++)
+      if (off[u + 1] - off[u] > r) AfLink(static_cast<int32_t>(u), dst[off[u] + r], comp);
+    AfCompress(n, comp);
+  }
+  const int32_t c = AfSample(n, comp);
+#pragma omp parallel for schedule(dynamic, 16384)
+  for (int64_t u = 0; u < n; u++) {
+    if (comp[u] == c) continue;
+    const int64_t b = std::min(off[u] + neighbor_rounds, off[u + 1]);
+    for (int64_t e = b; e < off[u + 1]; e++) AfLink(static_cast<int32_t>(u), dst[e], comp);
+    if (directed)
+      for (int64_t e = ioff[u]; e < ioff[u + 1]; e++) AfLink(static_cast<int32_t>(u), idst[e], comp);
+  }
+  AfCompress(n, comp);
+  return Since(t0);
+}
+
+void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_t* comp) {
+  std::vector<int32_t> parent(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) parent[i] = static_cast<int32_t>(i);
+  std::function<int32_t(int32_t)> find = [&](int32_t x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (int64_t u = 0; u < n; u++)
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t a = find(static_cast<int32_t>(u)), b = find(dst[e]);
+      if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+  for (int64_t i = 0; i < n; i++) comp[i] = find(static_cast<int32_t>(i));
 }
 
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth) {

@@ -72,6 +72,16 @@ int orc_pr_pull_f32(int64_t n, const int64_t* in_off, const int32_t* in_dst, con
 int orc_pr_pull_f64(int64_t n, const int64_t* in_off, const int32_t* in_dst, const int64_t* out_off,
                     int max_iters, double epsilon, double* scores, double* error);
 
+/* OpenMP Afforest restating Afforest/Link/Compress/SampleFrequentElement
+ * (src/cc.cc:41-151).  ioff/idst: inverse CSR (read for directed graphs).
+ * Returns elapsed seconds. */
+double orc_afforest(int64_t n, const int64_t* off, const int32_t* dst, int directed,
+                    const int64_t* ioff, const int32_t* idst, int neighbor_rounds, int32_t* comp);
+
+/* Textbook components (serial union-find over every edge, union by
+ * smaller id): label = smallest vertex id of the component. */
+void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_t* comp);
+
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);

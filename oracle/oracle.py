@@ -56,6 +56,10 @@ def lib() -> C.CDLL:
             getattr(L, nm).argtypes = [C.c_int64, _i64p, _i32p, _i64p, C.c_int, C.c_double,
                                        C.POINTER(ct), C.POINTER(C.c_double)]
             getattr(L, nm).restype = C.c_int
+        L.orc_afforest.argtypes = [C.c_int64, _i64p, _i32p, C.c_int, _i64p, _i32p, C.c_int, _i32p]
+        L.orc_afforest.restype = C.c_double
+        L.orc_cc_min_labels.argtypes = [C.c_int64, _i64p, _i32p, _i32p]
+        L.orc_cc_min_labels.restype = None
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -145,6 +149,25 @@ def pagerank(g, max_iters: int = 20, epsilon: float = 1e-4, fp64: bool = False):
     it = fn(g.num_nodes, _p(ioff, C.c_int64), _p(idst, C.c_int32), _p(g.out_offsets, C.c_int64),
             max_iters, epsilon, _p(scores, C.c_double if fp64 else C.c_float), C.byref(err))
     return scores, it, err.value
+
+
+def afforest(g, neighbor_rounds: int = 2):
+    """OpenMP Afforest (src/cc.cc:95-151): (comp, seconds)."""
+    off, dst = _csr(g)
+    comp = np.empty(g.num_nodes, dtype=np.int32)
+    ioff = _p(g.in_offsets, C.c_int64) if g.directed else None
+    idst = _p(np.ascontiguousarray(g.in_neighbors), C.c_int32) if g.directed else None
+    secs = lib().orc_afforest(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), int(g.directed),
+                              ioff, idst, neighbor_rounds, _p(comp, C.c_int32))
+    return comp, secs
+
+
+def cc_min_labels(g) -> np.ndarray:
+    """Components by serial union-find over the out-CSR; label = smallest id."""
+    off, dst = _csr(g)
+    comp = np.empty(g.num_nodes, dtype=np.int32)
+    lib().orc_cc_min_labels(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), _p(comp, C.c_int32))
+    return comp
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

@@ -143,3 +143,13 @@ def pagerank(name: str, g, max_iters: int = 20, epsilon: float = 1e-4):
     out = np.empty(g.num_nodes, dtype=np.float32)
     secs = rg.lib.ref_pr(rg.h, max_iters, epsilon, _p(out, C.c_float))
     return out, secs
+
+
+def afforest(g):
+    """Reference Afforest: (comp, secs)."""
+    rg = RefGraph("cc", g)
+    rg.lib.ref_cc.restype = C.c_double
+    rg.lib.ref_cc.argtypes = [C.c_void_p, _i32p]
+    out = np.empty(g.num_nodes, dtype=np.int32)
+    secs = rg.lib.ref_cc(rg.h, _p(out, C.c_int32))
+    return out, secs

@@ -33,7 +33,7 @@ def sha(a) -> str:
 
 
 def main():
-    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}}
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}, "cc": {}}
     # The reference's own expected stats lines (test/test.mk:44-49).
     for name in ("g10", "u10"):
         with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
@@ -77,6 +77,11 @@ def main():
         top = np.argsort(-scores, kind="stable")[:5]
         out["pr"][tag] = {"scores_sha256": sha(scores), "sum": float(scores.astype(np.float64).sum()),
                           "top5": [[int(i), float(scores[i])] for i in top]}
+    # Reference Afforest labels.
+    for tag, g in (("g10", Graph.kronecker(10)), ("g16", Graph.kronecker(16)),
+                   ("u14d2", Graph.uniform(14, degree=2))):
+        comp, _ = ref.afforest(g)
+        out["cc"][tag] = {"comp_sha256": sha(comp), "components": int(len(np.unique(comp)))}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote golden.json")
