@@ -7,6 +7,7 @@ return the same result arrays as the reference functions they replace:
     DeltaStep(g, source, delta)              src/sssp.cc:87-153   -> dist[n]
     PageRankPull(g, max_iters, epsilon)      src/pr_spmv.cc:35-61 -> scores[n]
     Afforest(g, neighbor_rounds=2)           src/cc.cc:95-151     -> comp[n]
+    Brandes(g, roots)                        src/bc.cc:95-150     -> scores[n]
 
 A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
 call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
@@ -58,7 +59,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -85,6 +86,7 @@ def lib() -> C.CDLL:
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
+        L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -258,3 +260,32 @@ def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True) -> CCR
 def Afforest(dg: DeviceGraph, neighbor_rounds: int = 2) -> np.ndarray:
     """Component labels (src/cc.cc:95-151)."""
     return cc(dg, neighbor_rounds).comp
+
+
+BC_FP32, BC_FP64, BC_FP32_KAHAN = 0, 1, 2
+
+
+@dataclass
+class BCResult:
+    scores: np.ndarray | None
+    stats: dict
+
+
+def bc(dg: DeviceGraph, roots, fp64: bool = False, kahan: bool = False,
+       want_scores: bool = True) -> BCResult:
+    """gg_bc: max-normalised Brandes scores from the given roots.
+
+    Default: the reference's float arithmetic; kahan=True: compensated score
+    sums; fp64=True: double dependencies and scores."""
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    mode = BC_FP64 if fp64 else (BC_FP32_KAHAN if kahan else BC_FP32)
+    st = Stats()
+    _check(lib().gg_bc(dg.handle, _p(roots, C.c_int32), len(roots), mode,
+                       _p(out, C.c_float) if want_scores else None, C.byref(st)))
+    return BCResult(out, st.as_dict())
+
+
+def Brandes(dg: DeviceGraph, roots) -> np.ndarray:
+    """Betweenness scores from `roots`, divided by the largest (src/bc.cc:95-150)."""
+    return bc(dg, roots).scores

@@ -103,6 +103,19 @@ int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* sco
  * vertex id in the component, exactly the reference's labels. */
 int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
 
+/* Brandes betweenness centrality from roots[0 .. k).
+ * Replaces Brandes(const Graph&, SourcePicker<Graph>&, NodeID num_iters,
+ * bool logging) (src/bc.cc:95-150): the caller passes the roots the
+ * SourcePicker would draw.  Path counts are FP64.  mode GG_BC_FP32: the
+ * reference's float arithmetic (dependencies accumulated in successor order,
+ * float scores), which its verifier requires to FLT_EPSILON;
+ * GG_BC_FP32_KAHAN: float dependencies and scores with Kahan-compensated
+ * per-vertex dependency sums and score sums; GG_BC_FP64: double
+ * dependencies and scores.
+ * scores[n] (nullable) receives float scores divided by the largest. */
+enum { GG_BC_FP32 = 0, GG_BC_FP64 = 1, GG_BC_FP32_KAHAN = 2 };
+int gg_bc(gg_graph* g, const int32_t* roots, int k, int mode, float* scores, gg_stats* stats);
+
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP
  * events on the handle's stream and credited with its algorithmic HBM bytes

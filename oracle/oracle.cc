@@ -417,6 +417,68 @@ void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_
   for (int64_t i = 0; i < n; i++) comp[i] = find(static_cast<int32_t>(i));
 }
 
+}  // extern "C"
+
+namespace {
+template <class T>
+void Brandes(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k, T* scores) {
+  std::fill(scores, scores + n, T(0));
+  std::vector<int32_t> depth(static_cast<size_t>(n));
+  std::vector<double> sigma(static_cast<size_t>(n));
+  std::vector<T> delta(static_cast<size_t>(n));
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(n));
+  for (int r = 0; r < k; r++) {
+    const int32_t src = roots[r];
+    std::fill(depth.begin(), depth.end(), -1);
+    std::fill(sigma.begin(), sigma.end(), 0.0);
+    std::fill(delta.begin(), delta.end(), T(0));
+    order.clear();
+    depth[src] = 0;
+    sigma[src] = 1;
+    order.push_back(src);
+    for (size_t i = 0; i < order.size(); i++) {
+      const int32_t u = order[i];
+      for (int64_t e = off[u]; e < off[u + 1]; e++) {
+        const int32_t v = dst[e];
+        if (depth[v] == -1) {
+          depth[v] = depth[u] + 1;
+          order.push_back(v);
+        }
+        if (depth[v] == depth[u] + 1) sigma[v] += sigma[u];
+      }
+    }
+    // Farthest first (reverse BFS order visits depths in decreasing order).
+    for (size_t i = order.size(); i-- > 0;) {
+      const int32_t u = order[i];
+      for (int64_t e = off[u]; e < off[u + 1]; e++) {
+        const int32_t v = dst[e];
+        if (depth[v] == depth[u] + 1) {
+          volatile double term = (sigma[u] / sigma[v]) * (1 + delta[v]);
+          delta[u] += term;
+        }
+      }
+      scores[u] += delta[u];
+    }
+  }
+  T biggest = 0;
+  for (int64_t i = 0; i < n; i++) biggest = std::max(biggest, scores[i]);
+  for (int64_t i = 0; i < n; i++) scores[i] = scores[i] / biggest;
+}
+}  // namespace
+
+extern "C" {
+
+void orc_brandes_f32(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
+                     float* scores) {
+  Brandes<float>(n, off, dst, roots, k, scores);
+}
+
+void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
+                     double* scores) {
+  Brandes<double>(n, off, dst, roots, k, scores);
+}
+
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth) {
   std::fill(depth, depth + n, -2);
   if (parent[src] != src) return -1;

@@ -82,6 +82,15 @@ double orc_afforest(int64_t n, const int64_t* off, const int32_t* dst, int direc
  * smaller id): label = smallest vertex id of the component. */
 void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_t* comp);
 
+/* Serial Brandes over roots[0 .. k), max-normalised, restating the
+ * reference verifier's algorithm (src/bc.cc:169-220): double path counts,
+ * successors from depths.  _f32: float dependencies and scores (the
+ * reference's types, bc.cc:46-48); _f64: double throughout (FP64 oracle). */
+void orc_brandes_f32(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
+                     float* scores);
+void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
+                     double* scores);
+
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);

@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
         L.orc_afforest.restype = C.c_double
         L.orc_cc_min_labels.argtypes = [C.c_int64, _i64p, _i32p, _i32p]
         L.orc_cc_min_labels.restype = None
+        for nm, ct in (("orc_brandes_f32", C.c_float), ("orc_brandes_f64", C.c_double)):
+            getattr(L, nm).argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int, C.POINTER(ct)]
+            getattr(L, nm).restype = None
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -168,6 +171,17 @@ def cc_min_labels(g) -> np.ndarray:
     comp = np.empty(g.num_nodes, dtype=np.int32)
     lib().orc_cc_min_labels(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), _p(comp, C.c_int32))
     return comp
+
+
+def brandes(g, roots, fp64: bool = False) -> np.ndarray:
+    """Serial Brandes, max-normalised (src/bc.cc:169-220); float or FP64."""
+    off, dst = _csr(g)
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    out = np.empty(g.num_nodes, dtype=np.float64 if fp64 else np.float32)
+    fn = lib().orc_brandes_f64 if fp64 else lib().orc_brandes_f32
+    fn(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), _p(roots, C.c_int32), len(roots),
+       _p(out, C.c_double if fp64 else C.c_float))
+    return out
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

@@ -153,3 +153,13 @@ def afforest(g):
     out = np.empty(g.num_nodes, dtype=np.int32)
     secs = rg.lib.ref_cc(rg.h, _p(out, C.c_int32))
     return out, secs
+
+
+def brandes(g, num_iters: int, given: int = -1):
+    """Reference Brandes with its own SourcePicker roots: (scores, secs)."""
+    rg = RefGraph("bc", g)
+    rg.lib.ref_bc.restype = C.c_double
+    rg.lib.ref_bc.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_float)]
+    out = np.empty(g.num_nodes, dtype=np.float32)
+    secs = rg.lib.ref_bc(rg.h, num_iters, given, _p(out, C.c_float))
+    return out, secs

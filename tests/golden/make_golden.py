@@ -33,7 +33,7 @@ def sha(a) -> str:
 
 
 def main():
-    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}, "cc": {}}
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}, "cc": {}, "bc": {}}
     # The reference's own expected stats lines (test/test.mk:44-49).
     for name in ("g10", "u10"):
         with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
@@ -82,6 +82,14 @@ def main():
                    ("u14d2", Graph.uniform(14, degree=2))):
         comp, _ = ref.afforest(g)
         out["cc"][tag] = {"comp_sha256": sha(comp), "components": int(len(np.unique(comp)))}
+    # Reference Brandes scores (its own SourcePicker roots), -i4 as in
+    # benchmark/bench.mk:158-159.
+    for tag, g, k in (("g10", Graph.kronecker(10), 4), ("g16", Graph.kronecker(16), 4),
+                      ("u12", Graph.uniform(12), 8)):
+        scores, _ = ref.brandes(g, k)
+        out["bc"][tag] = {"k": k, "roots": [int(r) for r in ref.pick_sources(g, k)],
+                          "scores_sha256": sha(scores),
+                          "argmax": int(np.argmax(scores))}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote golden.json")
