@@ -8,6 +8,7 @@ return the same result arrays as the reference functions they replace:
     PageRankPull(g, max_iters, epsilon)      src/pr_spmv.cc:35-61 -> scores[n]
     Afforest(g, neighbor_rounds=2)           src/cc.cc:95-151     -> comp[n]
     Brandes(g, roots)                        src/bc.cc:95-150     -> scores[n]
+    OrderedCount(g) / Hybrid(host_graph)     src/tc.cc:52-100     -> triangles
 
 A `DeviceGraph` is the HBM-resident copy of a host `graph.Graph`.  Every
 call goes through the C-ABI; if libgapgpu.so is missing or no GPU is
@@ -59,7 +60,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_tc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -87,6 +88,7 @@ def lib() -> C.CDLL:
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
+        L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -289,3 +291,23 @@ def bc(dg: DeviceGraph, roots, fp64: bool = False, kahan: bool = False,
 def Brandes(dg: DeviceGraph, roots) -> np.ndarray:
     """Betweenness scores from `roots`, divided by the largest (src/bc.cc:95-150)."""
     return bc(dg, roots).scores
+
+
+def tc(dg: DeviceGraph) -> tuple[int, dict]:
+    """gg_tc: ordered triangle count of the uploaded graph: (count, stats)."""
+    c = C.c_uint64()
+    st = Stats()
+    _check(lib().gg_tc(dg.handle, C.byref(c), C.byref(st)))
+    return int(c.value), st.as_dict()
+
+
+def OrderedCount(dg: DeviceGraph) -> int:
+    """Triangle count, each counted once at its largest vertex (src/tc.cc:52-71)."""
+    return tc(dg)[0]
+
+
+def Hybrid(g: Graph, device: int = 0) -> int:
+    """Relabel by degree when the reference's heuristic says so, then count
+    (src/tc.cc:75-100)."""
+    h = g.relabel_by_degree() if g.worth_relabelling() else g
+    return OrderedCount(DeviceGraph(h, device))

@@ -116,6 +116,13 @@ int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
 enum { GG_BC_FP32 = 0, GG_BC_FP64 = 1, GG_BC_FP32_KAHAN = 2 };
 int gg_bc(gg_graph* g, const int32_t* roots, int k, int mode, float* scores, gg_stats* stats);
 
+/* Ordered triangle count of the uploaded undirected graph.
+ * Replaces OrderedCount(const Graph&) (src/tc.cc:52-71); the Hybrid
+ * relabel-by-degree decision (tc.cc:75-100) is made by the caller, which
+ * uploads the relabelled graph when it applies.  stats->edges_examined is
+ * the number of merge steps. */
+int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats);
+
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP
  * events on the handle's stream and credited with its algorithmic HBM bytes

@@ -479,6 +479,49 @@ void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const in
   Brandes<double>(n, off, dst, roots, k, scores);
 }
 
+uint64_t orc_ordered_count(int64_t n, const int64_t* off, const int32_t* dst, double* seconds) {
+  const auto t0 = Clock::now();
+  uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 64)
+  for (int64_t u = 0; u < n; u++) {
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = dst[e];
+      if (v > u) break;
+      int64_t it = off[v];
+      for (int64_t f = off[u]; f < off[u + 1]; f++) {
+        const int32_t w = dst[f];
+        if (w > v) break;
+        while (it < off[v + 1] && dst[it] < w) it++;
+        if (it < off[v + 1] && dst[it] == w) total++;
+      }
+    }
+  }
+  if (seconds) *seconds = Since(t0);
+  return total;
+}
+
+uint64_t orc_tc_intersect(int64_t n, const int64_t* off, const int32_t* dst) {
+  uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 64)
+  for (int64_t u = 0; u < n; u++)
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = dst[e];
+      int64_t a = off[u], b = off[v];
+      while (a < off[u + 1] && b < off[v + 1]) {
+        if (dst[a] < dst[b]) {
+          a++;
+        } else if (dst[b] < dst[a]) {
+          b++;
+        } else {
+          total++;
+          a++;
+          b++;
+        }
+      }
+    }
+  return total / 6;
+}
+
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth) {
   std::fill(depth, depth + n, -2);
   if (parent[src] != src) return -1;

@@ -91,6 +91,14 @@ void orc_brandes_f32(int64_t n, const int64_t* off, const int32_t* dst, const in
 void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
                      double* scores);
 
+/* OpenMP OrderedCount restatement (src/tc.cc:52-71); returns the count,
+ * *seconds the elapsed time. */
+uint64_t orc_ordered_count(int64_t n, const int64_t* off, const int32_t* dst, double* seconds);
+/* Every triangle counted six times by sorted-list intersection over all
+ * directed edges, divided by 6 (the reference verifier's method,
+ * src/tc.cc:109-128). */
+uint64_t orc_tc_intersect(int64_t n, const int64_t* off, const int32_t* dst);
+
 /* Depths implied by a parent tree (-1 where unreached); returns -1 if the
  * parent pointers do not form a tree rooted at src. */
 int orc_depth_from_parent(int64_t n, const int32_t* parent, int32_t src, int32_t* depth);

@@ -63,6 +63,10 @@ def lib() -> C.CDLL:
         for nm, ct in (("orc_brandes_f32", C.c_float), ("orc_brandes_f64", C.c_double)):
             getattr(L, nm).argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int, C.POINTER(ct)]
             getattr(L, nm).restype = None
+        L.orc_ordered_count.argtypes = [C.c_int64, _i64p, _i32p, C.POINTER(C.c_double)]
+        L.orc_ordered_count.restype = C.c_uint64
+        L.orc_tc_intersect.argtypes = [C.c_int64, _i64p, _i32p]
+        L.orc_tc_intersect.restype = C.c_uint64
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -182,6 +186,20 @@ def brandes(g, roots, fp64: bool = False) -> np.ndarray:
     fn(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), _p(roots, C.c_int32), len(roots),
        _p(out, C.c_double if fp64 else C.c_float))
     return out
+
+
+def ordered_count(g):
+    """OpenMP OrderedCount (src/tc.cc:52-71): (count, seconds)."""
+    off, dst = _csr(g)
+    secs = C.c_double()
+    c = lib().orc_ordered_count(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), C.byref(secs))
+    return int(c), secs.value
+
+
+def tc_intersect(g) -> int:
+    """Triangles by full sorted-list intersection / 6 (src/tc.cc:109-128)."""
+    off, dst = _csr(g)
+    return int(lib().orc_tc_intersect(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32)))
 
 
 def depth_from_parent(parent: np.ndarray, src: int) -> np.ndarray:

@@ -163,3 +163,19 @@ def brandes(g, num_iters: int, given: int = -1):
     out = np.empty(g.num_nodes, dtype=np.float32)
     secs = rg.lib.ref_bc(rg.h, num_iters, given, _p(out, C.c_float))
     return out, secs
+
+
+def tc(g, hybrid: bool = True):
+    """Reference Hybrid (relabel when worth it) or OrderedCount: (count, secs)."""
+    rg = RefGraph("tc", g)
+    rg.lib.ref_tc.restype = C.c_double
+    rg.lib.ref_tc.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]
+    c = C.c_uint64()
+    secs = rg.lib.ref_tc(rg.h, 0 if hybrid else 1, C.byref(c))
+    return int(c.value), secs
+
+
+def worth_relabelling(g) -> bool:
+    rg = RefGraph("tc", g)
+    rg.lib.ref_tc_worth_relabelling.argtypes = [C.c_void_p]
+    return bool(rg.lib.ref_tc_worth_relabelling(rg.h))

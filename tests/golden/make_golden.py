@@ -33,7 +33,7 @@ def sha(a) -> str:
 
 
 def main():
-    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}, "cc": {}, "bc": {}}
+    out = {"stats_lines": {}, "sg_sha256": {}, "roots": {}, "bfs": {}, "sssp": {}, "pr": {}, "cc": {}, "bc": {}, "tc": {}}
     # The reference's own expected stats lines (test/test.mk:44-49).
     for name in ("g10", "u10"):
         with open(os.path.join(REF_TEST, f"graph-{name}.out")) as f:
@@ -90,6 +90,11 @@ def main():
         out["bc"][tag] = {"k": k, "roots": [int(r) for r in ref.pick_sources(g, k)],
                           "scores_sha256": sha(scores),
                           "argmax": int(np.argmax(scores))}
+    # Reference triangle counts (Hybrid) and its relabelling decision.
+    for tag, g in (("g10", Graph.kronecker(10)), ("g16", Graph.kronecker(16)),
+                   ("u12", Graph.uniform(12))):
+        count, _ = ref.tc(g)
+        out["tc"][tag] = {"triangles": count, "worth_relabelling": ref.worth_relabelling(g)}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote golden.json")
