@@ -13,7 +13,8 @@
 //   -c <side>   generate the side x side weighted 2D grid graph
 //   -x <dev>    HIP device ordinal
 //   -j <k>      roots per batched launch (bfs, sssp, bc; default 1 = per root)
-//   -p <mode>   score precision for pr / bc: fp32 (compensated, default) | fp64
+//   -p <mode>   score precision: pr fp32 (compensated, default) | fp64;
+//               bc fp32 (the reference's float arithmetic, default) | kahan | fp64
 //   -q          autotune direction/bucket/bin parameters for this graph
 #pragma once
 
@@ -67,7 +68,7 @@ class CLI {
     Help('v', "", "verify the output of each run", "false");
     Help('l', "", "log performance within each trial", "false");
     Help('j', "k", "roots per batched GPU launch", "1");
-    Help('p', "mode", "score precision: fp32 (compensated) or fp64", "fp32");
+    Help('p', "mode", "score precision: fp32, kahan (bc) or fp64", "fp32");
     Help('q', "", "autotune GPU parameters for this graph", "false");
     num_iters_ = default_iters;
     max_iters_ = default_iters;
@@ -279,6 +280,26 @@ class SourcePicker {
   std::mt19937_64 rng_;
   BoundedDraw<std::mt19937_64> draw_;
 };
+
+// Relabel by degree only for dense, skewed graphs: average degree >= 10 and
+// the sampled mean degree / 1.3 above the sampled median (1000 samples drawn
+// with SourcePicker), as WorthRelabelling does (src/tc.cc:75-91).
+inline bool WorthRelabelling(const Graph& g) {
+  const int64_t avg = g.num_nodes() ? g.num_edges() / g.num_nodes() : 0;
+  if (avg < 10) return false;
+  SourcePicker<Graph> sp(g);
+  const int64_t k = std::min<int64_t>(1000, g.num_nodes());
+  std::vector<int64_t> s(static_cast<size_t>(k));
+  int64_t total = 0;
+  for (int64_t i = 0; i < k; i++) {
+    s[i] = g.out_degree(sp.PickNext());
+    total += s[i];
+  }
+  std::sort(s.begin(), s.end());
+  const double mean = static_cast<double>(total) / k;
+  const double median = static_cast<double>(s[k / 2]);
+  return mean / 1.3 > median;
+}
 
 // The k largest (value, key) pairs, largest first.
 template <class KeyT, class ValT>

@@ -174,23 +174,7 @@ int gh_relabel_by_degree(const gh_graph* g, gh_graph** out) {
 }
 
 int gh_worth_relabelling(const gh_graph* g) {
-  return Guard([&] {
-    const Graph& gr = Unweighted(g);
-    const int64_t avg = gr.num_edges() / gr.num_nodes();
-    if (avg < 10) return 0;
-    SourcePicker<Graph> sp(gr);
-    const int64_t k = std::min<int64_t>(1000, gr.num_nodes());
-    std::vector<int64_t> s(static_cast<size_t>(k));
-    int64_t total = 0;
-    for (int64_t i = 0; i < k; i++) {
-      s[i] = gr.out_degree(sp.PickNext());
-      total += s[i];
-    }
-    std::sort(s.begin(), s.end());
-    const double mean = static_cast<double>(total) / k;
-    const double median = static_cast<double>(s[k / 2]);
-    return mean / 1.3 > median ? 1 : 0;
-  });
+  return Guard([&] { return WorthRelabelling(Unweighted(g)) ? 1 : 0; });
 }
 
 int gh_pick_sources(const gh_graph* g, int64_t given, int k, int32_t* out) {
