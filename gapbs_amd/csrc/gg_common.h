@@ -97,6 +97,8 @@ struct gg_graph {
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for
+  int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
+  int64_t ms_heavy_chunks = 0;           // and their in-edge chunks
   gg::Workspace ws;
   void* pinned = nullptr;  // small pinned host block for counter read-back
   size_t pinned_bytes = 0;

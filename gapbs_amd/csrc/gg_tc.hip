@@ -2,44 +2,42 @@
 //
 // Reference: OrderedCount (src/tc.cc:52-71).  Each triangle u > v > w is
 // counted once, at its largest vertex u: for every neighbour v < u of u,
-// the neighbours w of u below v that are also neighbours of v, found by a
-// merge of the two sorted lists with an early exit once w reaches v.  The
-// graph must be undirected with sorted, duplicate-free lists (the builder
-// guarantees it, tc.cc:31-35).  Whether to relabel by degree first
-// (Hybrid / WorthRelabelling, tc.cc:75-100) is the caller's choice; the
-// relabelled graph is counted the same way.
+// the neighbours w of u below v that are also neighbours of v.  The graph
+// must be undirected with sorted, duplicate-free lists (the builder
+// guarantees it, tc.cc:31-35).  Whether to relabel by degree first (Hybrid /
+// WorthRelabelling, tc.cc:75-100) is the caller's choice; the relabelled
+// graph is counted the same way.
 //
-// GPU design: the forward edges u -> v (v < u, a prefix of u's sorted list,
-// found by binary search) are spread over the grid with gg_expand.h; each
-// lane merges one (u, v) pair.  Counts are summed per block and added to 64
-// striped counters.
+// Writing N+(x) for x's neighbours below x (a prefix of its sorted list),
+// the reference's count is sum over u, v in N+(u) of |N+(u) & N+(v)|
+// (every w of N+(v) is below v, so the reference's "w <= v" bound is
+// implied).  GPU design:
+//   * TcForward: fdeg[x] = |N+(x)| by binary search, once per call;
+//   * TcItemCount / TcItemScan / TcItemFill: work items (u, v-chunk of 256
+//     entries of N+(u), set slice of kSetSlice entries of N+(u)) in a
+//     deterministic order (block prefix sums; no atomics);
+//   * TcCount: one block per item stages its slice of N+(u) in an LDS hash
+//     table, then streams the lists N+(v) of its 256 v's as one flat range
+//     (block prefix sum, owner by binary search) and probes every w.  Each
+//     probe is a coalesced 4-byte list read plus an LDS lookup, in place of
+//     the reference's dependent merge steps.
+// Counts are summed per block and added to 64 striped counters.
 #include "gg_common.h"
-#include "gg_expand.h"
 
 namespace gg {
 namespace {
 
 constexpr int kBS = 256;
 constexpr int kStripes = 64;
+constexpr int kSetSlice = 2048;     // entries of N+(u) per LDS table
+constexpr int kTable = 2 * kSetSlice;  // table slots (load factor <= 1/2)
+constexpr int kUnroll = 2;
 
-struct TcVisit {
-  const int64_t* off;
-  const int32_t* dst;
-  unsigned long long count = 0, work = 0;
-  __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
-    if (!valid) return;
-    const int32_t v = dst[e];
-    const int64_t ue = off[u + 1];
-    int64_t it = off[v];
-    const int64_t ve = off[v + 1];
-    for (int64_t iu = off[u]; iu < ue; iu++) {
-      const int32_t w = dst[iu];
-      if (w > v) break;
-      while (it < ve && dst[it] < w) it++;
-      work++;
-      if (it < ve && dst[it] == w) count++;
-    }
-  }
+struct TcItem {
+  int32_t u;
+  int32_t vb;  // first index into N+(u) of this item's v's
+  int32_t sb;  // first index into N+(u) of this item's set slice
+  int32_t pad;
 };
 
 // Index of the first entry of the sorted range [b, e) that is >= x.
@@ -54,47 +52,177 @@ __device__ __forceinline__ int64_t LowerBound(const int32_t* dst, int64_t b, int
   return b;
 }
 
-__global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
-                                               int64_t n, unsigned long long* counts, EdgeChunk* chunks,
-                                               int32_t* nchunks) {
-  __shared__ ExpandShared sh;
+__device__ __forceinline__ int64_t ItemsOf(int32_t d) {
+  if (d < 2) return 0;  // a triangle needs two entries in N+(u)
+  return int64_t((d + kBS - 1) / kBS) * ((d + kSetSlice - 1) / kSetSlice);
+}
+
+__device__ __forceinline__ uint32_t Hash(int32_t w, uint32_t mask) {
+  return (static_cast<uint32_t>(w) * 2654435761u >> 7) & mask;
+}
+
+__global__ __launch_bounds__(kBS) void TcForward(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                 int64_t n, int32_t* __restrict__ fdeg) {
+  for (int64_t x = blockIdx.x * int64_t(kBS) + threadIdx.x; x < n; x += int64_t(gridDim.x) * kBS)
+    fdeg[x] = static_cast<int32_t>(LowerBound(dst, off[x], off[x + 1], static_cast<int32_t>(x)) - off[x]);
+}
+
+// Vertices are split into gridDim.x contiguous spans; block b counts the
+// items of span b.
+__device__ __forceinline__ void Span(int64_t n, int64_t& lo, int64_t& hi) {
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  lo = per * blockIdx.x;
+  hi = lo + per < n ? lo + per : n;
+  if (lo > n) lo = n;
+}
+
+__global__ __launch_bounds__(kBS) void TcItemCount(const int32_t* __restrict__ fdeg, int64_t n,
+                                                   unsigned long long* __restrict__ span_items) {
   __shared__ unsigned long long s_red[kBS / kWave];
-  TcVisit visit{off, dst};
-  for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < n; tile += int64_t(gridDim.x) * kBS) {
-    const int64_t u = tile + threadIdx.x;
-    int64_t beg = 0, end = 0;
-    if (u < n) {
-      beg = off[u];
-      end = LowerBound(dst, beg, off[u + 1], static_cast<int32_t>(u));  // forward edges v < u
+  int64_t lo, hi;
+  Span(n, lo, hi);
+  unsigned long long c = 0;
+  for (int64_t u = lo + threadIdx.x; u < hi; u += kBS) c += static_cast<unsigned long long>(ItemsOf(fdeg[u]));
+  c = BlockSum<kBS>(c, s_red);
+  if (threadIdx.x == 0) span_items[blockIdx.x] = c;
+}
+
+// Exclusive prefix over the span totals (one block; spans <= a few thousand).
+__global__ __launch_bounds__(kBS) void TcItemScan(unsigned long long* span_items, int spans,
+                                                  unsigned long long* total) {
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int i = 0; i < spans; i++) {
+      const unsigned long long c = span_items[i];
+      span_items[i] = s;
+      s += c;
     }
-    ExpandTile(sh, static_cast<int32_t>(u), 0, beg, end, visit, chunks, nchunks, kChunkEdges);
-  }
-  const unsigned long long c = BlockSum<kBS>(visit.count, s_red);
-  const unsigned long long w = BlockSum<kBS>(visit.work, s_red);
-  if (threadIdx.x == 0) {
-    if (c) atomicAdd(&counts[2 * (blockIdx.x & (kStripes - 1))], c);
-    if (w) atomicAdd(&counts[2 * (blockIdx.x & (kStripes - 1)) + 1], w);
+    *total = s;
   }
 }
 
-__global__ __launch_bounds__(kBS) void TcCountChunks(const int64_t* __restrict__ off,
-                                                     const int32_t* __restrict__ dst,
-                                                     unsigned long long* counts, const EdgeChunk* chunks,
-                                                     const int32_t* nchunks) {
+__global__ __launch_bounds__(kBS) void TcItemFill(const int32_t* __restrict__ fdeg, int64_t n,
+                                                  const unsigned long long* __restrict__ span_base,
+                                                  TcItem* __restrict__ items) {
+  __shared__ unsigned long long s_scan[kBS];
+  int64_t lo, hi;
+  Span(n, lo, hi);
+  unsigned long long base = span_base[blockIdx.x];
+  for (int64_t t = lo; t < hi; t += kBS) {
+    const int64_t u = t + threadIdx.x;
+    const int32_t d = u < hi ? fdeg[u] : 0;
+    const unsigned long long ni = static_cast<unsigned long long>(u < hi ? ItemsOf(d) : 0);
+    // Block inclusive scan (Hillis-Steele in LDS; 64-bit counts).
+    s_scan[threadIdx.x] = ni;
+    __syncthreads();
+    for (int o = 1; o < kBS; o <<= 1) {
+      const unsigned long long add = threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0;
+      __syncthreads();
+      s_scan[threadIdx.x] += add;
+      __syncthreads();
+    }
+    unsigned long long pos = base + s_scan[threadIdx.x] - ni;
+    const unsigned long long tile_total = s_scan[kBS - 1];
+    __syncthreads();
+    if (ni) {
+      for (int32_t vb = 0; vb < d; vb += kBS)
+        for (int32_t sb = 0; sb < d; sb += kSetSlice) items[pos++] = TcItem{static_cast<int32_t>(u), vb, sb, 0};
+    }
+    base += tile_total;
+  }
+}
+
+__global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                               const int32_t* __restrict__ fdeg, const TcItem* __restrict__ items,
+                                               int64_t nitems, unsigned long long* counts) {
+  __shared__ int32_t table[kTable];
+  __shared__ int64_t s_vbeg[kBS];
+  __shared__ int s_pre[kBS + 1];
+  __shared__ int s_scan[kBS / kWave];
   __shared__ unsigned long long s_red[kBS / kWave];
-  TcVisit visit{off, dst};
-  ExpandChunks(chunks, *nchunks, visit);
-  const unsigned long long c = BlockSum<kBS>(visit.count, s_red);
-  const unsigned long long w = BlockSum<kBS>(visit.work, s_red);
-  if (threadIdx.x == 0) {
-    if (c) atomicAdd(&counts[2 * (blockIdx.x & (kStripes - 1))], c);
-    if (w) atomicAdd(&counts[2 * (blockIdx.x & (kStripes - 1)) + 1], w);
+  const int tid = threadIdx.x;
+  unsigned long long count = 0, probes = 0;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const TcItem item = items[it];
+    const int64_t ub = off[item.u];
+    const int32_t d = fdeg[item.u];
+    const int32_t se = item.sb + kSetSlice < d ? item.sb + kSetSlice : d;
+    const int32_t slice = se - item.sb;
+    uint32_t size = 64;
+    while (size < 2u * static_cast<uint32_t>(slice)) size <<= 1;
+    const uint32_t mask = size - 1;
+    for (uint32_t i = tid; i < size; i += kBS) table[i] = -1;
+    __syncthreads();
+    for (int32_t i = tid; i < slice; i += kBS) {
+      const int32_t w = dst[ub + item.sb + i];
+      uint32_t h = Hash(w, mask);
+      while (atomicCAS(&table[h], -1, w) != -1) h = (h + 1) & mask;
+    }
+    // The slice holds the sorted values [lo_w, hi_w]; probes outside skip
+    // the table.
+    const int32_t lo_w = dst[ub + item.sb], hi_w = dst[ub + se - 1];
+    // This thread's v and its list N+(v).
+    const int32_t j = item.vb + tid;
+    int len = 0;
+    int64_t vbeg = 0;
+    if (j < d) {
+      const int32_t v = dst[ub + j];
+      vbeg = off[v];
+      len = fdeg[v];
+    }
+    s_vbeg[tid] = vbeg;
+    int total = 0;
+    const int rank = BlockExclusiveScan<kBS>(len, &total, s_scan);
+    s_pre[tid] = rank;
+    if (tid == 0) s_pre[kBS] = total;
+    __syncthreads();
+    for (int j0 = 0; j0 < total; j0 += kBS * kUnroll) {
+      int32_t w[kUnroll];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        const int jj = j0 + k * kBS + tid;
+        w[k] = -1;
+        if (jj < total) {
+          int lo = 0, hi = kBS;  // s_pre[lo] <= jj < s_pre[hi]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= jj)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          w[k] = dst[s_vbeg[lo] + (jj - s_pre[lo])];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        if (w[k] < lo_w || w[k] > hi_w) continue;
+        uint32_t h = Hash(w[k], mask);
+        while (true) {
+          const int32_t t = table[h];
+          if (t == w[k]) {
+            count++;
+            break;
+          }
+          if (t < 0) break;
+          h = (h + 1) & mask;
+        }
+      }
+    }
+    if (tid == 0) probes += static_cast<unsigned long long>(total);
+    __syncthreads();
+  }
+  count = BlockSum<kBS>(count, s_red);
+  probes = BlockSum<kBS>(probes, s_red);
+  if (tid == 0) {
+    const int slot = 2 * (blockIdx.x & (kStripes - 1));
+    if (count) atomicAdd(&counts[slot], count);
+    if (probes) atomicAdd(&counts[slot + 1], probes);
   }
 }
 
-__global__ void TcClear(unsigned long long* counts, int32_t* nchunks) {
+__global__ void TcClear(unsigned long long* counts) {
   for (int i = threadIdx.x; i < 2 * kStripes; i += blockDim.x) counts[i] = 0;
-  if (threadIdx.x == 0) *nchunks = 0;
 }
 
 }  // namespace
@@ -113,43 +241,49 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
     GG_HIP(hipSetDevice(g->device));
     const int64_t n = g->n;
     hipStream_t st = g->stream;
-    unsigned long long* counts = g->Scratch<unsigned long long>("tc.counts", 2 * kStripes);
-    int32_t* nchunks = g->Scratch<int32_t>("tc.nchunks", 1);
-    const bool use_chunks = g->max_degree >= kChunkEdges;
-    EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("tc.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+    unsigned long long* counts = g->Scratch<unsigned long long>("tc.counts", 2 * kStripes + 1);
+    int32_t* fdeg = g->Scratch<int32_t>("tc.fdeg", n + 1);
+    const int spans = static_cast<int>(GridFor(n, kBS * 16, g->cu_count * 4));
+    unsigned long long* span_items = g->Scratch<unsigned long long>("tc.spans", spans);
     const dim3 blk(kBS);
+    const double N = static_cast<double>(n);
     int64_t launches = 0;
     BeginTiming(g);
-    Run(g, "tc_clear", 0, TcClear, dim3(1), blk, counts, nchunks);
-    // One pass over offsets and lists, plus the list reads of the merges
-    // (credited from the work counter below).
-    const int idx = Run(g, "tc_count", 8.0 * (n + 1) + 4.0 * g->out_entries, TcCount,
-                        dim3(GridFor(n, kBS, g->cu_count * 8)), blk, static_cast<const int64_t*>(g->out_off),
-                        static_cast<const int32_t*>(g->out_dst), n, counts,
-                        use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks);
+    Run(g, "tc_clear", 0, TcClear, dim3(1), blk, counts);
+    Run(g, "tc_forward", 16.0 * N + 4.0 * N, TcForward, dim3(GridFor(n, kBS, g->cu_count * 8)), blk,
+        static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), n, fdeg);
+    Run(g, "tc_items", 4.0 * N, TcItemCount, dim3(spans), blk, static_cast<const int32_t*>(fdeg), n, span_items);
+    Run(g, "tc_items", 0, TcItemScan, dim3(1), dim3(64), span_items, spans, counts + 2 * kStripes);
+    launches += 4;
+    unsigned long long* h = g->Pinned<unsigned long long>(2 * kStripes + 1);
+    GG_HIP(hipMemcpyAsync(h, counts + 2 * kStripes, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    const int64_t nitems = static_cast<int64_t>(h[0]);
+    TcItem* items = g->Scratch<TcItem>("tc.items", nitems + 1);
+    Run(g, "tc_items", 4.0 * N + 16.0 * nitems, TcItemFill, dim3(spans), blk, static_cast<const int32_t*>(fdeg), n,
+        static_cast<const unsigned long long*>(span_items), items);
+    // Per item: its set slice and 256 list headers; per probe one 4-byte
+    // list entry (credited below from the probe counter).
+    const int idx = Run(g, "tc_count", 16.0 * nitems, TcCount, dim3(GridFor(nitems, 1, g->cu_count * 16)), blk,
+                        static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+                        static_cast<const int32_t*>(fdeg), static_cast<const TcItem*>(items), nitems, counts);
     launches += 2;
-    if (use_chunks) {
-      Run(g, "tc_count_chunks", 0, TcCountChunks, dim3(g->cu_count * 4), blk,
-          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), counts,
-          static_cast<const EdgeChunk*>(chunks), static_cast<const int32_t*>(nchunks));
-      launches++;
-    }
     GG_HIP(hipGetLastError());
-    unsigned long long* h = g->Pinned<unsigned long long>(2 * kStripes);
     GG_HIP(hipMemcpyAsync(h, counts, sizeof(unsigned long long) * 2 * kStripes, hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
-    unsigned long long total = 0, work = 0;
+    unsigned long long total = 0, probes = 0;
     for (int i = 0; i < kStripes; i++) {
       total += h[2 * i];
-      work += h[2 * i + 1];
+      probes += h[2 * i + 1];
     }
-    CreditBytes(g, idx, 4.0 * static_cast<double>(work));  // merge reads
+    // Staged set entries: |N+(u)| per (u, v-chunk), at most the probes' order.
+    CreditBytes(g, idx, 4.0 * static_cast<double>(probes) + 12.0 * N);
     FinishTiming(g, stats, wt);
     *count = total;
     if (stats) {
       stats->total_ms = wt.ms();
       stats->iterations = 1;
-      stats->edges_examined = static_cast<int64_t>(work);
+      stats->edges_examined = static_cast<int64_t>(probes);
       stats->launches = launches;
     }
   });

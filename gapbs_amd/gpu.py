@@ -60,7 +60,8 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_tc", "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
+    "gg_bfs", "gg_bfs_batch", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_tc",
+    "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
 
@@ -83,6 +84,7 @@ def lib() -> C.CDLL:
         L.gg_graph_device_bytes.argtypes = [vp]
         L.gg_graph_device_bytes.restype = C.c_int64
         L.gg_bfs.argtypes = [vp, C.c_int32, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(Stats)]
+        L.gg_bfs_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
@@ -195,6 +197,25 @@ def bfs(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18,
 def DOBFS(dg: DeviceGraph, source: int, alpha: int = 15, beta: int = 18) -> np.ndarray:
     """Parent array of a direction-optimizing BFS (src/bfs.cc:122-179)."""
     return bfs(dg, source, alpha, beta, want_depth=False).parent
+
+
+@dataclass
+class BatchBFSResult:
+    depths: np.ndarray | None  # [k, n] int32, -1 where unreached
+    stats: dict
+
+
+def bfs_batch(dg: DeviceGraph, roots, alpha: int = 15, want_depths: bool = True) -> BatchBFSResult:
+    """gg_bfs_batch: up to 64 roots in one traversal; depths[r] equals the
+    single-root BFS depths from roots[r] (the reference's per-trial loop,
+    src/benchmark.h:106, over src/bfs.cc:122-179)."""
+    r = np.ascontiguousarray(roots, dtype=np.int32)
+    n = dg.num_nodes
+    depths = np.empty((len(r), n), dtype=np.int32) if want_depths else None
+    st = Stats()
+    _check(lib().gg_bfs_batch(dg.handle, _p(r, C.c_int32), len(r), alpha,
+                              _p(depths, C.c_int32) if want_depths else None, C.byref(st)))
+    return BatchBFSResult(depths, st.as_dict())
 
 
 K_DIST_INF = (2**31 - 1) // 2  # src/sssp.cc:64

@@ -79,6 +79,16 @@ int64_t gg_graph_device_bytes(const gg_graph* g);
 int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* parent, int32_t* depth,
            gg_stats* stats);
 
+/* Batched BFS: k (1..64) roots traversed together with 64-bit per-vertex
+ * masks.  Replaces k sequential DOBFS trials (src/benchmark.h:106,
+ * src/bfs.cc:256-259); each root's depths equal its single-root BFS depths.
+ * depths[k * n] (nullable: kept on the device), root-major: depths[r * n +
+ * v] is v's level from roots[r], -1 if unreached.  Repeated roots are
+ * allowed.  alpha: go bottom-up when the frontier's out-edges exceed the
+ * unexplored edges / alpha (as DOBFS).  stats->reached counts reached
+ * (root, vertex) pairs; stats->reached_edges sums their out-degrees. */
+int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha, int32_t* depths, gg_stats* stats);
+
 /* Delta-stepping single-source shortest paths from `source` with bucket
  * width `delta` on a weighted upload.
  * Replaces DeltaStep(const WGraph&, NodeID source, WeightT delta, bool

@@ -480,10 +480,18 @@ void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const in
 }
 
 uint64_t orc_ordered_count(int64_t n, const int64_t* off, const int32_t* dst, double* seconds) {
+  return orc_ordered_count_sample(n, off, dst, 1, 0, seconds);
+}
+
+// OrderedCount's outer loop restricted to u = first, first + stride, ...:
+// the counts over first = 0 .. stride-1 sum to the full count, and one of
+// them times a bounded sample of the whole run.
+uint64_t orc_ordered_count_sample(int64_t n, const int64_t* off, const int32_t* dst, int64_t stride,
+                                  int64_t first, double* seconds) {
   const auto t0 = Clock::now();
   uint64_t total = 0;
 #pragma omp parallel for reduction(+ : total) schedule(dynamic, 64)
-  for (int64_t u = 0; u < n; u++) {
+  for (int64_t u = first; u < n; u += stride) {
     for (int64_t e = off[u]; e < off[u + 1]; e++) {
       const int32_t v = dst[e];
       if (v > u) break;

@@ -94,6 +94,8 @@ void orc_brandes_f64(int64_t n, const int64_t* off, const int32_t* dst, const in
 /* OpenMP OrderedCount restatement (src/tc.cc:52-71); returns the count,
  * *seconds the elapsed time. */
 uint64_t orc_ordered_count(int64_t n, const int64_t* off, const int32_t* dst, double* seconds);
+uint64_t orc_ordered_count_sample(int64_t n, const int64_t* off, const int32_t* dst, int64_t stride,
+                                  int64_t first, double* seconds);
 /* Every triangle counted six times by sorted-list intersection over all
  * directed edges, divided by 6 (the reference verifier's method,
  * src/tc.cc:109-128). */

@@ -65,6 +65,9 @@ def lib() -> C.CDLL:
             getattr(L, nm).restype = None
         L.orc_ordered_count.argtypes = [C.c_int64, _i64p, _i32p, C.POINTER(C.c_double)]
         L.orc_ordered_count.restype = C.c_uint64
+        L.orc_ordered_count_sample.argtypes = [C.c_int64, _i64p, _i32p, C.c_int64, C.c_int64,
+                                               C.POINTER(C.c_double)]
+        L.orc_ordered_count_sample.restype = C.c_uint64
         L.orc_tc_intersect.argtypes = [C.c_int64, _i64p, _i32p]
         L.orc_tc_intersect.restype = C.c_uint64
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
@@ -193,6 +196,15 @@ def ordered_count(g):
     off, dst = _csr(g)
     secs = C.c_double()
     c = lib().orc_ordered_count(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), C.byref(secs))
+    return int(c), secs.value
+
+
+def ordered_count_sample(g, stride: int, first: int = 0):
+    """OrderedCount over u = first (mod stride) only: (partial count, seconds)."""
+    off, dst = _csr(g)
+    secs = C.c_double()
+    c = lib().orc_ordered_count_sample(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), stride, first,
+                                       C.byref(secs))
     return int(c), secs.value
 
 

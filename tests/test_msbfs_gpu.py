@@ -1,0 +1,183 @@
+"""GPU batched BFS parity: gg_bfs_batch (through the C-ABI) against the oracle.
+
+The reference runs one BFS per trial (src/benchmark.h:106 over
+src/bfs.cc:122-179); depths from a root are unique, so every root's row of
+the batched result must equal the serial BFS depths of that root alone
+(oracle.bfs_depth, restating src/bfs.cc:204-215) bit for bit.  Stats are
+checked against sums over the per-root serial runs.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import cached_graph
+from gapbs_amd import gpu
+from gapbs_amd.graph import Graph
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+_DG = {}
+
+
+def dev(tag, make):
+    if tag not in _DG:
+        _DG[tag] = gpu.DeviceGraph(cached_graph(tag, make))
+    return _DG[tag]
+
+
+def check_batch(g, dg, roots, alpha=15):
+    res = gpu.bfs_batch(dg, roots, alpha)
+    deg = np.diff(g.out_offsets)
+    reached = reached_edges = 0
+    for i, r in enumerate(roots):
+        want = oracle.bfs_depth(g, int(r))
+        assert np.array_equal(res.depths[i], want), f"root #{i} = {r}"
+        reached += int((want >= 0).sum())
+        reached_edges += int(deg[want >= 0].sum())
+    assert res.stats["reached"] == reached
+    assert res.stats["reached_edges"] == reached_edges
+    return res
+
+
+@pytest.mark.parametrize("tag,make", [("g10", lambda: Graph.kronecker(10)),
+                                      ("g16", lambda: Graph.kronecker(16)),
+                                      ("u10", lambda: Graph.uniform(10))])
+def test_batch_64_sourcepicker_roots(tag, make):
+    print(f"[msbfs] {tag} 64 roots", flush=True)
+    g = cached_graph(tag, make)
+    check_batch(g, dev(tag, make), g.pick_sources(64))
+
+
+def test_batch_seeded_kronecker_partial_batches():
+    print("[msbfs] k18 seed 8675309: 1, 7, 63, 64 roots", flush=True)
+    make = lambda: Graph.kronecker(18, seed=8675309)  # noqa: E731
+    g = cached_graph("k18s", make)
+    dg = dev("k18s", make)
+    roots = g.pick_sources(64)
+    for k in (1, 7, 63, 64):
+        res = check_batch(g, dg, roots[:k])
+        assert res.stats["td_steps"] + res.stats["bu_steps"] == res.stats["iterations"]
+
+
+@pytest.mark.parametrize("alpha", [1, 1000000])
+def test_batch_direction_extremes(alpha):
+    # Bottom-up when the frontier's out-edges exceed unfinished in-edges /
+    # alpha: alpha 1 keeps most levels top-down, alpha 1e6 turns bottom-up
+    # from level 2 on (level 1 is always top-down).
+    print(f"[msbfs] alpha={alpha}", flush=True)
+    make = lambda: Graph.kronecker(16)  # noqa: E731
+    g = cached_graph("g16", make)
+    res = check_batch(g, dev("g16", make), g.pick_sources(64), alpha)
+    assert res.stats["td_steps"] >= 1
+    if alpha == 1000000:
+        assert res.stats["bu_steps"] > res.stats["td_steps"]
+
+
+def test_batch_repeated_and_isolated_roots():
+    print("[msbfs] repeated / isolated roots", flush=True)
+    g = Graph.from_edges([0, 1, 3, 5, 9, 2], [1, 2, 4, 6, 9, 0])  # 9: self-loop only -> degree 0
+    dg = gpu.DeviceGraph(g)
+    roots = [0, 0, 9, 3, 2, 0, 7]
+    res = check_batch(g, dg, roots)
+    assert res.depths[2].tolist() == [-1] * 9 + [0]
+    assert np.array_equal(res.depths[0], res.depths[1])
+
+
+@pytest.mark.parametrize("alpha", [15, 1000000])
+def test_batch_roots_in_small_components(alpha):
+    # A Kronecker graph plus a separate 12-vertex path: the path roots' bits
+    # die out after a few levels while the others are still spreading, so
+    # bottom-up levels must stop looking for them (the live-root mask).
+    print(f"[msbfs] small components alpha={alpha}", flush=True)
+    base = cached_graph("g16", lambda: Graph.kronecker(16))
+    off, nb = base.out_offsets, base.out_neighbors
+    u = np.repeat(np.arange(base.num_nodes), np.diff(off))
+    n0 = base.num_nodes
+    pu = np.arange(n0, n0 + 11)
+    g = Graph.from_edges(np.concatenate([u, pu]), np.concatenate([nb, pu + 1]))
+    dg = gpu.DeviceGraph(g)
+    roots = np.concatenate([base.pick_sources(60), [n0, n0 + 5, n0 + 11, n0 + 3]])
+    check_batch(g, dg, roots, alpha)
+
+
+def test_batch_directed_graph_uses_in_edges_bottom_up():
+    # A directed uniform graph: bottom-up levels scan in-neighbours, so a
+    # wrong direction shows up as wrong depths.
+    print("[msbfs] directed", flush=True)
+    rng = np.random.default_rng(5)
+    n, m = 1 << 14, 1 << 18
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    g = Graph.from_edges(u, v, symmetrize=False)
+    dg = gpu.DeviceGraph(g)
+    roots = rng.choice(n, 64, replace=False)
+    for alpha in (1, 15):
+        check_batch(g, dg, roots, alpha)
+
+
+def test_batch_closed_forms():
+    print("[msbfs] closed forms", flush=True)
+    side = 96
+    r, c = np.divmod(np.arange(side * side), side)
+    ids = np.arange(side * side)
+    u = np.concatenate([ids[c < side - 1], ids[r < side - 1]])
+    w = np.concatenate([ids[c < side - 1] + 1, ids[r < side - 1] + side])
+    dg = gpu.DeviceGraph(Graph.from_edges(u, w))
+    rng = np.random.default_rng(3)
+    roots = rng.choice(side * side, 64, replace=False)
+    for alpha in (1, 15):
+        res = gpu.bfs_batch(dg, roots, alpha)
+        for i, s in enumerate(roots):
+            rs, cs = divmod(int(s), side)
+            assert np.array_equal(res.depths[i], np.abs(r - rs) + np.abs(c - cs)), f"root {s}"
+    n = 3000
+    path = gpu.DeviceGraph(Graph.from_edges(np.arange(n - 1), np.arange(1, n)))
+    res = gpu.bfs_batch(path, [0, n - 1, 1500])
+    assert res.depths[0].tolist() == list(range(n))
+    assert res.depths[1].tolist() == list(range(n - 1, -1, -1))
+    assert res.depths[2].tolist() == [abs(i - 1500) for i in range(n)]
+
+
+def test_batch_hub_chunks_star_of_stars():
+    print("[msbfs] star of stars", flush=True)
+    a = np.zeros(9000, np.int32)
+    b = np.full(5000, 1, np.int32)
+    u = np.concatenate([a, b, [0]])
+    v = np.concatenate([np.arange(2, 9002), np.arange(9002, 14002), [1]])
+    g = Graph.from_edges(u, v)
+    dg = gpu.DeviceGraph(g)
+    for alpha in (15, 1000000):
+        check_batch(g, dg, [0, 5, 9500, 1, 13000], alpha)
+
+
+def test_batch_matches_single_root_gpu_and_leaves_no_state():
+    # Two different batches in a row on one handle, then single-root BFS.
+    print("[msbfs] consecutive batches", flush=True)
+    make = lambda: Graph.kronecker(16)  # noqa: E731
+    g = cached_graph("g16", make)
+    dg = dev("g16", make)
+    roots = g.pick_sources(64)
+    a = gpu.bfs_batch(dg, roots[:40]).depths
+    b = gpu.bfs_batch(dg, roots[40:]).depths
+    for i in (0, 17, 39):
+        assert np.array_equal(a[i], gpu.bfs(dg, int(roots[i])).depth)
+    for i in (0, 23):
+        assert np.array_equal(b[i], gpu.bfs(dg, int(roots[40 + i])).depth)
+
+
+def test_batch_errors():
+    print("[msbfs] errors", flush=True)
+    g = Graph.kronecker(10)
+    dg = gpu.DeviceGraph(g)
+    for roots in ([], list(range(65)), [0, g.num_nodes]):
+        with pytest.raises(gpu.GpuError) as e:
+            gpu.bfs_batch(dg, roots)
+        assert e.value.code == -22
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.bfs_batch(dg, [0], alpha=0)
+    assert e.value.code == -22
+    wdg = gpu.DeviceGraph(Graph.kronecker(10, weighted=True))
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.bfs_batch(wdg, [0])
+    assert e.value.code == -95

@@ -52,3 +52,12 @@ def test_closed_forms():
     lat = Graph.from_edges(np.concatenate([ids[c < side - 1], ids[r < side - 1]]),
                            np.concatenate([ids[c < side - 1] + 1, ids[r < side - 1] + side]))
     assert oracle.ordered_count(lat)[0] == 0
+
+
+def test_sampled_counts_partition_the_full_count(golden):
+    # The bounded CPU sample times OrderedCount's outer loop over one residue
+    # class of u; the classes must partition the count exactly.
+    g = cached_graph("g16", GRAPHS["g16"]).relabel_by_degree()
+    want = golden["tc"]["g16"]["triangles"]
+    parts = [oracle.ordered_count_sample(g, 7, first)[0] for first in range(7)]
+    assert sum(parts) == want and all(p > 0 for p in parts)
