@@ -16,6 +16,7 @@ device:
   pr    PageRank, damping 0.85, tolerance 1e-4, at most 20 iterations (FP32)
   cc    Afforest
   bc    Brandes from the first --bc-roots roots (the reference's float arithmetic)
+  bc_batch  Brandes from all 64 roots in one batched traversal (same arithmetic)
   tc    ordered triangle count on the graph relabelled by degree
 
 The roofline figure is measured live: the library times each kernel launch
@@ -43,7 +44,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "sssp", "pr", "cc", "bc", "tc")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "cc", "bc", "tc")
 
 
 def log(msg):
@@ -254,6 +255,8 @@ def main():
             gpu.cc(dg, want_comp=False)
         elif k == "bc":
             return gpu.bc(dg, bc_roots, want_scores=False).stats
+        elif k == "bc_batch":
+            return gpu.bc_batch(dg, roots, want_scores=False).stats
         elif k == "tc":
             return gpu.tc(drg)[1]
         return None
@@ -282,6 +285,8 @@ def main():
             r["gteps"] = round(sum(m_per_root[:len(sssp_roots)]) / per / 1e9, 3)
         if k == "bc":
             r["gteps"] = round(sum(m_per_root[:len(bc_roots)]) / per / 1e9, 3)
+        if k == "bc_batch":
+            r["gteps"] = round(m_total / per / 1e9, 3)
         if k == "pr" and last:
             r["iterations"] = last["iterations"]
         if k == "tc" and last is not None:
@@ -291,6 +296,9 @@ def main():
     if "bfs_batch" in results and "bfs" in results:
         results["bfs_batch"]["speedup_vs_sequential"] = round(results["bfs"]["seconds"] /
                                                               results["bfs_batch"]["seconds"], 3)
+    if "bc_batch" in results and "bc" in results:  # per-root GTEPS: bc runs fewer roots
+        results["bc_batch"]["speedup_vs_sequential"] = round(results["bc_batch"]["gteps"] /
+                                                             results["bc"]["gteps"], 3)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baselines(kernels, g, wg, rg_tc, roots, m_per_root, len(sssp_roots), len(bc_roots),
@@ -315,7 +323,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{workload}: batched BFS of {len(roots)} SourcePicker roots per step "
                                f"(headline); per-kernel lines for 64 sequential BFS, sssp "
-                               f"({len(sssp_roots)} roots), pr, cc, bc ({len(bc_roots)} roots), tc",
+                               f"({len(sssp_roots)} roots), pr, cc, bc ({len(bc_roots)} roots), batched bc "
+                               f"({len(roots)} roots), tc",
                    "num_nodes": g.num_nodes, "num_edges_directed": g.num_edges_directed,
                    "roots": len(roots), "alpha": args.alpha, "sssp_delta": args.delta},
         "roofline": head["roofline"],

@@ -1,0 +1,449 @@
+// Batched Brandes betweenness centrality: up to 64 roots in one traversal.
+//
+// Reference: Brandes (src/bc.cc:95-150) runs one root at a time: a BFS
+// that counts shortest paths (PBFS, bc.cc:51-92, path counts in double),
+// then, level by level from the deepest to the root's own, the dependency
+// delta_u = sum over successors v (depth(v) = depth(u) + 1, in u's sorted
+// out-list order) of (sigma_u / sigma_v) * (1 + delta_v), accumulated in a
+// float, and scores[u] += delta_u (bc.cc:121-137).  After all roots every
+// score is divided by the largest (bc.cc:141-148).
+//
+// Batched design (vertex-major rows, lane = root):
+//   * the batched BFS traversal (gg_msbfs.h) keeps every level's frontier
+//     masks F_L (bit r of F_L(v): root r reached v at level L);
+//   * sigma[v][r] (double, 64 per vertex): per level, every vertex with new
+//     bits sums sigma_r over the in-neighbours u with bit r in F_{L-1}(u).
+//     A wavefront serves one vertex, lane r handles root r, and the 64
+//     sigma values of an in-neighbour are one 512-byte row read.  Path
+//     counts are integers, so the order of these sums does not change them
+//     (below 2^53); vertices with >= 512 in-edges are split into chunks
+//     that add into the row with atomics;
+//   * delta[u][r]: per level from the deepest, lane r folds the terms of
+//     root r over u's out-list in order, with the reference's arithmetic
+//     (float: the double term rounded into a float total; FP64 mode:
+//     double throughout), so each delta_r(u) equals the single-root value;
+//   * scores: per vertex, the deltas of the roots that reached it added in
+//     root order (the order of the reference's trial loop), then the max
+//     normalisation.  In FP32 mode the scores equal the reference's bit for
+//     bit.
+// Device memory: the frontier masks (8 bytes per vertex per level), sigma
+// (512 bytes per vertex) and delta (256 or 512 bytes per vertex).
+#include "gg_msbfs.h"
+
+namespace gg {
+namespace {
+
+constexpr int kMbBS = 256;
+constexpr int kMbStripes = 64;
+constexpr int kSigmaLoads = 8;   // sigma rows in flight per wavefront step
+constexpr int kDeltaLoads = 16;  // (sigma, delta) row pairs in flight per wavefront step
+
+__global__ void MbRootSigma(const int32_t* __restrict__ roots, int k, double* __restrict__ sigma) {
+  const int r = threadIdx.x;
+  if (r < k) sigma[int64_t(roots[r]) * 64 + r] = 1.0;
+}
+
+// Pops up to B set bits of `any` (lowest first) into j[]; the rest are -1.
+// Wave-uniform.
+template <int B>
+__device__ __forceinline__ void PopBits(uint64_t& any, int (&j)[B]) {
+#pragma unroll
+  for (int t = 0; t < B; t++) {
+    if (any) {
+      j[t] = __builtin_ctzll(any);
+      any &= any - 1;
+    } else {
+      j[t] = -1;
+    }
+  }
+}
+
+// Sigma of the vertices that gained bits at this level (cur = F_L, prev =
+// F_{L-1}).  Each wavefront scans 64 consecutive vertices and serves those
+// with new bits one at a time.  Heavy vertices (>= kHeavyEdges in-edges)
+// only get their entries zeroed here; MbSigmaHeavy adds their sums.
+__global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                                 const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
+                                                 double* __restrict__ sigma, int64_t n,
+                                                 unsigned long long* examined) {
+  __shared__ unsigned long long s_red[kMbBS / kWave];
+  const int lane = LaneId();
+  const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
+  unsigned long long ex = 0;
+  for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
+    const int64_t vl = base + lane;
+    uint64_t fl = 0;
+    int64_t bl = 0, el = 0;
+    if (vl < n) {
+      fl = cur[vl];
+      if (fl) {
+        bl = in_off[vl];
+        el = in_off[vl + 1];
+      }
+    }
+    uint64_t act = __ballot(fl != 0);
+    while (act) {
+      const int i = __builtin_ctzll(act);
+      act &= act - 1;
+      const int64_t v = base + i;
+      const uint64_t fv = __shfl(fl, i);
+      const int64_t b = __shfl(bl, i), e = __shfl(el, i);
+      const bool mine = (fv >> lane) & 1;
+      double* row = sigma + v * 64;
+      if (e - b >= kHeavyEdges) {
+        if (mine) row[lane] = 0.0;
+        continue;
+      }
+      double acc = 0.0;
+      for (int64_t cb = b; cb < e; cb += kWave) {
+        const int64_t k = cb + lane;
+        int32_t ul = 0;
+        uint64_t ml = 0;
+        if (k < e) {
+          ul = in_dst[k];
+          ml = prev[ul] & fv;
+        }
+        uint64_t any = __ballot(ml != 0);
+        while (any) {
+          int j[kSigmaLoads];
+          PopBits(any, j);
+          double x[kSigmaLoads];
+#pragma unroll
+          for (int t = 0; t < kSigmaLoads; t++) {
+            x[t] = 0.0;
+            if (j[t] >= 0) {
+              const uint64_t m = __shfl(ml, j[t]);
+              const int32_t u = __shfl(ul, j[t]);
+              if ((m >> lane) & 1) x[t] = sigma[int64_t(u) * 64 + lane];
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < kSigmaLoads; t++) acc += x[t];
+        }
+      }
+      ex += static_cast<unsigned long long>(e - b);
+      if (mine) row[lane] = acc;
+    }
+  }
+  ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+}
+
+// Sigma of heavy vertices: one wavefront per chunk of in-edges, partial
+// sums added into the row (integers: exact in any order).
+__global__ __launch_bounds__(kMbBS) void MbSigmaHeavy(const int32_t* __restrict__ in_dst,
+                                                      const uint64_t* __restrict__ cur,
+                                                      const uint64_t* __restrict__ prev, double* sigma,
+                                                      const EdgeChunk* __restrict__ chunks, int32_t nch,
+                                                      unsigned long long* examined) {
+  __shared__ unsigned long long s_red[kMbBS / kWave];
+  const int lane = LaneId();
+  const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
+  unsigned long long ex = 0;
+  for (int64_t it = wave; it < nch; it += nwaves) {
+    const EdgeChunk ch = chunks[it];
+    const uint64_t fv = cur[ch.u];
+    if (fv == 0) continue;
+    double acc = 0.0;
+    for (int64_t cb = ch.beg; cb < ch.end; cb += kWave) {
+      const int64_t k = cb + lane;
+      int32_t ul = 0;
+      uint64_t ml = 0;
+      if (k < ch.end) {
+        ul = in_dst[k];
+        ml = prev[ul] & fv;
+      }
+      uint64_t any = __ballot(ml != 0);
+      while (any) {
+        int j[kSigmaLoads];
+        PopBits(any, j);
+        double x[kSigmaLoads];
+#pragma unroll
+        for (int t = 0; t < kSigmaLoads; t++) {
+          x[t] = 0.0;
+          if (j[t] >= 0) {
+            const uint64_t m = __shfl(ml, j[t]);
+            const int32_t u = __shfl(ul, j[t]);
+            if ((m >> lane) & 1) x[t] = sigma[int64_t(u) * 64 + lane];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kSigmaLoads; t++) acc += x[t];
+      }
+    }
+    ex += static_cast<unsigned long long>(ch.end - ch.beg);
+    if (acc != 0.0) unsafeAtomicAdd(&sigma[int64_t(ch.u) * 64 + lane], acc);
+  }
+  ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+}
+
+// The reference's dependency term and accumulation (float totals take each
+// double term rounded in, bc.cc:128-131; `1 + deltas[v]` is a float sum).
+__device__ __forceinline__ double Term(double su, double sv, float dv) {
+  return __dmul_rn(__ddiv_rn(su, sv), static_cast<double>(__fadd_rn(1.0f, dv)));
+}
+__device__ __forceinline__ double Term(double su, double sv, double dv) {
+  return __dmul_rn(__ddiv_rn(su, sv), __dadd_rn(1.0, dv));
+}
+__device__ __forceinline__ float Fold(float acc, double t) {
+  return static_cast<float>(__dadd_rn(static_cast<double>(acc), t));
+}
+__device__ __forceinline__ double Fold(double acc, double t) { return __dadd_rn(acc, t); }
+
+// Folds the dependencies of vertex u for the roots in fu (its bits at this
+// level) over its out-list [b, e): lane r accumulates root r's terms in
+// list order and stores delta_r(u).  Called by a whole wavefront; returns
+// the entries read.  The next chunk's successors and masks are loaded
+// before the current chunk's rows are folded, and up to kDeltaLoads row
+// pairs are in flight per step.
+template <class T>
+__device__ __forceinline__ int64_t DeltaVertex(const int32_t* __restrict__ dst, const uint64_t* __restrict__ next,
+                                               const double* __restrict__ sigma, T* __restrict__ delta, int64_t u,
+                                               uint64_t fu, int64_t b, int64_t e) {
+  const int lane = LaneId();
+  const bool mine = (fu >> lane) & 1;
+  const double su = mine ? sigma[u * 64 + lane] : 1.0;
+  T acc = T(0);
+  int32_t vn = 0;
+  uint64_t mn = 0;
+  if (b + lane < e) {
+    vn = dst[b + lane];
+    mn = next[vn] & fu;
+  }
+  for (int64_t cb = b; cb < e; cb += kWave) {
+    const int32_t vl = vn;
+    const uint64_t ml = mn;
+    const int64_t kn = cb + kWave + lane;
+    vn = 0;
+    mn = 0;
+    if (kn < e) {
+      vn = dst[kn];
+      mn = next[vn] & fu;
+    }
+    uint64_t any = __ballot(ml != 0);
+    while (any) {
+      int j[kDeltaLoads];
+      PopBits(any, j);
+      double sv[kDeltaLoads];
+      T dv[kDeltaLoads];
+      bool on[kDeltaLoads];
+#pragma unroll
+      for (int t = 0; t < kDeltaLoads; t++) {
+        on[t] = false;
+        sv[t] = 1.0;
+        dv[t] = T(0);
+        if (j[t] >= 0) {
+          const uint64_t m = __shfl(ml, j[t]);
+          const int32_t v = __shfl(vl, j[t]);
+          on[t] = (m >> lane) & 1;
+          if (on[t]) {
+            sv[t] = sigma[int64_t(v) * 64 + lane];
+            dv[t] = delta[int64_t(v) * 64 + lane];
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kDeltaLoads; t++)
+        if (on[t]) acc = Fold(acc, Term(su, sv[t], dv[t]));
+    }
+  }
+  if (mine) delta[u * 64 + lane] = acc;
+  return e - b;
+}
+
+// Dependencies of the vertices at level L (cur = F_L, next = F_{L+1}, null
+// at the deepest level).  Each wavefront scans 64 consecutive vertices and
+// serves those with bits at L one at a time.  The fold of each (vertex,
+// root) is sequential by definition, so a hub's wavefront is the longest
+// one; spread over the scan, hubs overlap with the other vertices' work.
+template <class T>
+__global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                 const uint64_t* __restrict__ cur, const uint64_t* __restrict__ next,
+                                                 const double* __restrict__ sigma, T* __restrict__ delta, int64_t n,
+                                                 unsigned long long* examined) {
+  __shared__ unsigned long long s_red[kMbBS / kWave];
+  const int lane = LaneId();
+  const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
+  unsigned long long ex = 0;
+  for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
+    const int64_t ul = base + lane;
+    uint64_t fl = 0;
+    int64_t bl = 0, el = 0;
+    if (ul < n) {
+      fl = cur[ul];
+      if (fl) {
+        bl = off[ul];
+        el = off[ul + 1];
+        if (!next) el = bl;                   // deepest level: no successors
+      }
+    }
+    uint64_t act = __ballot(fl != 0);
+    while (act) {
+      const int i = __builtin_ctzll(act);
+      act &= act - 1;
+      ex += static_cast<unsigned long long>(
+          DeltaVertex<T>(dst, next, sigma, delta, base + i, __shfl(fl, i), __shfl(bl, i), __shfl(el, i)));
+    }
+  }
+  ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+}
+
+__device__ __forceinline__ float AddT(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double AddT(double a, double b) { return __dadd_rn(a, b); }
+
+// score[u] = the deltas of the roots that reached u, added in root order.
+template <class T>
+__global__ __launch_bounds__(kMbBS) void MbScores(const uint64_t* __restrict__ seen, const T* __restrict__ delta,
+                                                  int64_t n, T* __restrict__ score) {
+  for (int64_t u = blockIdx.x * int64_t(kMbBS) + threadIdx.x; u < n; u += int64_t(gridDim.x) * kMbBS) {
+    uint64_t s = seen[u];
+    T acc = T(0);
+    while (s) {
+      const int r = __builtin_ctzll(s);
+      s &= s - 1;
+      acc = AddT(acc, delta[u * 64 + r]);
+    }
+    score[u] = acc;
+  }
+}
+
+// Largest score (scores are >= 0, so their bit patterns order like the
+// values), then every score divided by it (bc.cc:141-148).
+template <class T>
+__global__ __launch_bounds__(kMbBS) void MbMax(const T* __restrict__ score, int64_t n, unsigned long long* max_bits) {
+  __shared__ double s_max[kMbBS / kWave];
+  double m = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(kMbBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kMbBS) {
+    const double v = static_cast<double>(score[i]);
+    m = v > m ? v : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(m, o);
+    m = t > m ? t : m;
+  }
+  if (LaneId() == 0) s_max[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kMbBS / kWave; w++) m = s_max[w] > m ? s_max[w] : m;
+    atomicMax(max_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kMbBS) void MbNormalize(const T* __restrict__ score, int64_t n,
+                                                     const unsigned long long* max_bits, float* __restrict__ out) {
+  const double mx = __longlong_as_double(static_cast<long long>(*max_bits));
+  const T biggest = static_cast<T>(mx);
+  for (int64_t i = blockIdx.x * int64_t(kMbBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kMbBS)
+    out[i] = static_cast<float>(score[i] / biggest);
+}
+
+__global__ void MbClear(unsigned long long* p, int count) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+
+template <class T>
+void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, gg_stats* stats,
+                     const WallTimer& wt) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  const dim3 blk(kMbBS);
+  const dim3 grid_n(GridFor(n, kMbBS, g->cu_count * 16));
+  const dim3 grid_w(GridFor(n, kMbBS, g->cu_count * 16));  // each wavefront takes 64 vertices at a time
+  const char* tag = sizeof(T) == 4 ? "mb.delta32" : "mb.delta64";
+  double* sigma = g->Scratch<double>("mb.sigma", n * 64);
+  T* delta = g->Scratch<T>(tag, n * 64);
+  T* score = g->Scratch<T>(sizeof(T) == 4 ? "mb.score32" : "mb.score64", n);
+  float* out = g->Scratch<float>("mb.out", n);
+  // [0, 64): sigma examined, [64, 128): delta examined, [128]: max bits.
+  unsigned long long* ctr = g->Scratch<unsigned long long>("mb.ctr", 2 * kMbStripes + 1);
+  const double N = static_cast<double>(n);
+
+  BeginTiming(g);
+  MsTraversal t = MsTraverse(g, droots, k, 15, nullptr, true);
+  const int levels = static_cast<int>(t.frontiers.size());
+  int64_t launches = t.launches;
+  Run(g, "msbc_clear", 0, MbClear, dim3(1), blk, ctr, 2 * kMbStripes + 1);
+  Run(g, "msbc_root_sigma", 0, MbRootSigma, dim3(1), dim3(64), droots, k, sigma);
+  launches += 2;
+  const int32_t nhchunks = static_cast<int32_t>(g->ms_heavy_chunks);
+  const EdgeChunk* hchunks = g->Scratch<EdgeChunk>("ms.hchunks", 1);
+  for (int L = 1; L < levels; L++) {
+    Run(g, "msbc_sigma", 16.0 * N, MbSigma, grid_w, blk, static_cast<const int64_t*>(g->in_off),
+        static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
+        static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, n, ctr);
+    Run(g, "msbc_sigma_heavy", 0, MbSigmaHeavy, dim3(GridFor(int64_t(nhchunks) * kWave, kMbBS, g->cu_count * 16)),
+        blk, static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
+        static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, hchunks, nhchunks, ctr);
+    launches += 2;
+  }
+  for (int L = levels - 1; L >= 0; L--) {
+    const uint64_t* next = L + 1 < levels ? t.frontiers[L + 1] : nullptr;
+    Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
+        static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
+        static_cast<const double*>(sigma), delta, n, ctr + kMbStripes);
+    launches++;
+  }
+  Run(g, "msbc_scores", 8.0 * N + sizeof(T) * N, MbScores<T>, grid_n, blk, static_cast<const uint64_t*>(t.seen),
+      static_cast<const T*>(delta), n, score);
+  Run(g, "msbc_max", sizeof(T) * N, MbMax<T>, grid_n, blk, static_cast<const T*>(score), n, ctr + 2 * kMbStripes);
+  Run(g, "msbc_normalize", (sizeof(T) + 4.0) * N, MbNormalize<T>, grid_n, blk, static_cast<const T*>(score), n,
+      static_cast<const unsigned long long*>(ctr + 2 * kMbStripes), out);
+  launches += 3;
+  GG_HIP(hipGetLastError());
+  FinishTiming(g, stats, wt);
+  if (scores) {
+    GG_HIP(hipMemcpyAsync(scores, out, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+  }
+  if (stats) {
+    unsigned long long h[2 * kMbStripes];
+    GG_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    unsigned long long* sums = g->Scratch<unsigned long long>("ms.reach", 2);
+    MsZero2<<<1, 64, 0, st>>>(sums);
+    MsReach<<<grid_n, dim3(kBS), 0, st>>>(t.seen, g->out_off, n, sums);
+    unsigned long long reach[2];
+    GG_HIP(hipMemcpyAsync(reach, sums, sizeof(reach), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    unsigned long long ex = 0;
+    for (int i = 0; i < 2 * kMbStripes; i++) ex += h[i];
+    stats->total_ms = wt.ms();
+    stats->iterations = levels;
+    stats->td_steps = t.td;
+    stats->bu_steps = t.bu;
+    stats->reached = static_cast<int64_t>(reach[0]);
+    stats->reached_edges = static_cast<int64_t>(reach[1]);
+    stats->edges_examined = t.examined + static_cast<int64_t>(ex);
+    stats->launches = launches;
+  }
+}
+
+}  // namespace
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_bc_batch(gg_graph* g, const int32_t* roots, int k, int mode, float* scores, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "BC needs an unweighted graph upload");
+    GG_REQUIRE(k >= 1 && k <= 64 && roots, GG_EINVAL, "1 to 64 roots");
+    GG_REQUIRE(mode == GG_BC_FP32 || mode == GG_BC_FP64, mode == GG_BC_FP32_KAHAN ? GG_ENOTSUP : GG_EINVAL,
+               "batched BC runs in GG_BC_FP32 or GG_BC_FP64");
+    for (int i = 0; i < k; i++) GG_REQUIRE(roots[i] >= 0 && roots[i] < g->n, GG_EINVAL, "root out of range");
+    GG_HIP(hipSetDevice(g->device));
+    int32_t* droots = g->Scratch<int32_t>("ms.roots", 64);
+    GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * k, hipMemcpyHostToDevice, g->stream));
+    if (mode == GG_BC_FP32)
+      RunBatchBrandes<float>(g, droots, k, scores, stats, wt);
+    else
+      RunBatchBrandes<double>(g, droots, k, scores, stats, wt);
+  });
+}

@@ -1,0 +1,728 @@
+// Batched breadth-first search: up to 64 roots in one traversal (kernels
+// and the host level loop, shared by the batched BFS and batched BC entry
+// points).
+//
+// Reference counterpart: none directly -- the reference runs one root per
+// trial (src/benchmark.h:106, src/bfs.cc:256-259).  Each root's depths must
+// equal what DOBFS gives for it alone (bfs.cc:122-179); depths are unique,
+// so this is checked bit for bit.
+//
+// Design (the multi-source BFS of Then et al., "The More the Merrier", VLDB
+// 2015, on wave64): per vertex a 64-bit `seen` mask (bit r: root r reached
+// it) and a 64-bit frontier mask (the bits that arrived at the last level).
+// A level is either
+//   top-down  -- the active vertices' edges are expanded (gg_expand.h) and
+//                each neighbour's accumulator gets frontier & ~seen with
+//                atomicOr; the first toucher queues the neighbour, and an
+//                update pass turns accumulators into new bits; or
+//   bottom-up -- every vertex not yet reached by all roots ORs its
+//                in-neighbours' frontier masks, stopping once every root has
+//                reached it (one thread per vertex below 64 in-edges, one
+//                wavefront per heavier vertex from a list built once per
+//                graph); no atomics, and the next frontier array is written
+//                densely, so no list is built.
+// Direction: bottom-up while the active vertices' out-edges exceed
+// unfinished / alpha, the form of the reference's test (bfs.cc:143) with
+// "unexplored edges" read as the in-edges of vertices that some root has
+// not reached yet (the reference's edges_to_check would go negative here:
+// a vertex is active at one level per distinct depth).  A top-down level
+// after a bottom-up one first compacts the dense frontier into a list.
+// Depths are kept vertex-major as uint16 (64 per vertex, 128 bytes),
+// 0xFFFF = unreached.
+#pragma once
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gg_common.h"
+#include "gg_expand.h"
+
+namespace gg {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kQueueCap = 2048;
+constexpr int kStripes = 64;
+constexpr uint16_t kNoDepth = 0xFFFF;
+constexpr int64_t kHeavyEdges = 512;  // bottom-up: light tier below, heavy tier from here
+constexpr int64_t kHeavyChunk = 2048;  // heavy in-edges per wavefront work item
+constexpr int kUnroll = 4;             // gathers in flight per lane
+
+struct MsCounters {
+  int32_t active_tail;   // top-down: next level's active list
+  int32_t touched_tail;  // top-down: vertices whose accumulator was touched
+  int32_t compact_tail;  // dense frontier -> list
+  int32_t nchunks;
+  int32_t heavy_tail;    // heavy-vertex list (built once per graph)
+  int32_t heavy_chunks;  // and its chunks
+  unsigned long long awake[kStripes];     // bottom-up: vertices with new bits
+  unsigned long long scout[kStripes];     // out-degree of vertices with new bits
+  unsigned long long examined[kStripes];  // adjacency entries read (top-down, light bottom-up)
+  unsigned long long heavy_ex[kStripes];  // adjacency entries read by the heavy tier
+  unsigned long long finished[kStripes];  // in-edges of vertices that became reached by all roots
+  unsigned long long live[kStripes];      // OR of the new bits
+};
+
+enum { kMailActive, kMailScout, kMailExamined, kMailHeavyEx, kMailFinished, kMailLive, kMailValues };
+
+__device__ __forceinline__ uint64_t WaveOr(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// Block-wide OR, result valid in every thread.  `sh` needs BS/64 slots.
+template <int BS>
+__device__ __forceinline__ unsigned long long BlockOr(unsigned long long v, unsigned long long* sh) {
+  v = WaveOr(v);
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+#pragma unroll
+  for (int w = 0; w < BS / kWave; w++) t |= sh[w];
+  __syncthreads();
+  return t;
+}
+
+__global__ void MsPublish(MsCounters* c, unsigned long long* mb, unsigned long long seq) {
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, s = 0, ex = 0, hx = 0, fin = 0, lv = 0;
+    for (int i = 0; i < kStripes; i++) {
+      a += c->awake[i];
+      s += c->scout[i];
+      ex += c->examined[i];
+      hx += c->heavy_ex[i];
+      fin += c->finished[i];
+      lv |= c->live[i];
+      c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->finished[i] = c->live[i] = 0;
+    }
+    unsigned long long vals[kMailValues] = {a + static_cast<unsigned long long>(c->active_tail), s, ex, hx,
+                                            fin, lv};
+    c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
+    MailPost(mb, vals, kMailValues, seq);
+  }
+}
+
+__global__ __launch_bounds__(kBS) void MsInit(uint64_t* __restrict__ seen, uint64_t* __restrict__ f0,
+                                              uint64_t* __restrict__ f1, uint64_t* __restrict__ acc,
+                                              uint16_t* __restrict__ depth, int64_t n, MsCounters* c) {
+  const int64_t total = depth ? n * 64 : n;
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < total; i += int64_t(gridDim.x) * kBS) {
+    if (depth) depth[i] = kNoDepth;
+    if (i < n) {
+      seen[i] = 0;
+      f0[i] = 0;
+      f1[i] = 0;
+      acc[i] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kStripes) {
+    const int t = threadIdx.x;
+    c->awake[t] = c->scout[t] = c->examined[t] = c->heavy_ex[t] = c->finished[t] = c->live[t] = 0;
+    if (t == 0) c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
+  }
+}
+
+// Vertices with at least kHeavyEdges in-edges (once per graph).
+__global__ __launch_bounds__(kBS) void MsHeavyList(const int64_t* __restrict__ in_off, int64_t n,
+                                                   int32_t* out, MsCounters* c) {
+  __shared__ LdsQueue<kQueueCap> lq;
+  LdsQueueInit(lq);
+  for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
+    const int64_t i = base + threadIdx.x;
+    const bool heavy = i < n && in_off[i + 1] - in_off[i] >= kHeavyEdges;
+    LdsQueuePush(lq, heavy, static_cast<int32_t>(i), out, &c->heavy_tail);
+  }
+  LdsQueueFlush(lq, out, &c->heavy_tail);
+}
+
+// Roots: seen and frontier bits, depth 0, the first active list (each
+// distinct root vertex once; a repeated root sets several bits on it).
+// One block of 64 threads.
+__global__ void MsRoots(const int32_t* __restrict__ roots, int k, uint64_t* seen, uint64_t* frontier,
+                        uint16_t* depth, int32_t* active, MsCounters* c) {
+  const int r = threadIdx.x;
+  if (r < k) {
+    const int32_t v = roots[r];
+    atomicOr(reinterpret_cast<unsigned long long*>(&seen[v]), 1ull << r);
+    atomicOr(reinterpret_cast<unsigned long long*>(&frontier[v]), 1ull << r);
+    if (depth) depth[int64_t(v) * 64 + r] = 0;
+    bool first = true;
+    for (int j = 0; j < r; j++)
+      if (roots[j] == v) first = false;
+    if (first) active[atomicAdd(&c->active_tail, 1)] = v;
+  }
+}
+
+// Records `level` as the depth of every new bit of vertex w.
+__device__ __forceinline__ void WriteDepths(uint16_t* depth, int32_t w, uint64_t bits, uint16_t level) {
+  if (!depth) return;
+  uint16_t* row = depth + int64_t(w) * 64;
+  while (bits) {
+    row[__builtin_ctzll(bits)] = level;
+    bits &= bits - 1;
+  }
+}
+
+// ---- top-down ---------------------------------------------------------------
+
+struct TdVisit {
+  const int32_t* dst;
+  const uint64_t* frontier;
+  const uint64_t* seen;
+  uint64_t* acc;
+  LdsQueue<kQueueCap>* lq;
+  int32_t* touched;
+  int32_t* touched_tail;
+  __device__ __forceinline__ void operator()(int32_t v, int32_t, int64_t e, bool valid) {
+    bool first = false;
+    int32_t w = 0;
+    if (valid) {
+      w = dst[e];
+      const uint64_t bits = frontier[v] & ~seen[w];
+      // acc[w] only gains bits during the level, so a plain (possibly
+      // stale) read that already holds them makes the atomic redundant.
+      if (bits && (bits & ~acc[w])) {
+        const uint64_t old = atomicOr(reinterpret_cast<unsigned long long*>(&acc[w]), bits);
+        first = old == 0;
+      }
+    }
+    LdsQueuePush(*lq, first, w, touched, touched_tail);
+  }
+};
+
+__global__ __launch_bounds__(kBS) void MsTopDown(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                 const uint64_t* __restrict__ frontier,
+                                                 const uint64_t* __restrict__ seen, uint64_t* acc,
+                                                 const int32_t* __restrict__ active, int32_t na,
+                                                 int32_t* touched, MsCounters* c, EdgeChunk* chunks,
+                                                 int64_t chunk_min) {
+  __shared__ ExpandShared sh;
+  __shared__ LdsQueue<kQueueCap> lq;
+  __shared__ unsigned long long s_red[kBS / kWave];
+  LdsQueueInit(lq);
+  TdVisit visit{dst, frontier, seen, acc, &lq, touched, &c->touched_tail};
+  unsigned long long ex = 0;
+  for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < na; tile += int64_t(gridDim.x) * kBS) {
+    const int64_t i = tile + threadIdx.x;
+    int32_t v = 0;
+    int64_t beg = 0, end = 0;
+    if (i < na) {
+      v = active[i];
+      beg = off[v];
+      end = off[v + 1];
+      ex += static_cast<unsigned long long>(end - beg);
+    }
+    ExpandTile(sh, v, 0, beg, end, visit, chunks, &c->nchunks, chunk_min);
+  }
+  LdsQueueFlush(lq, touched, &c->touched_tail);
+  ex = BlockSum<kBS>(ex, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&c->examined[blockIdx.x & (kStripes - 1)], ex);
+}
+
+__global__ __launch_bounds__(kBS) void MsTopDownChunks(const int32_t* __restrict__ dst,
+                                                       const uint64_t* __restrict__ frontier,
+                                                       const uint64_t* __restrict__ seen, uint64_t* acc,
+                                                       int32_t* touched, MsCounters* c, const EdgeChunk* chunks) {
+  __shared__ LdsQueue<kQueueCap> lq;
+  LdsQueueInit(lq);
+  TdVisit visit{dst, frontier, seen, acc, &lq, touched, &c->touched_tail};
+  ExpandChunks(chunks, c->nchunks, visit);
+  LdsQueueFlush(lq, touched, &c->touched_tail);
+}
+
+// Turns the accumulators of touched vertices into new bits and lists the
+// vertices that gained any.
+__global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touched, const MsCounters* cc,
+                                                uint64_t* __restrict__ acc, uint64_t* __restrict__ seen,
+                                                uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level,
+                                                const int64_t* __restrict__ off,
+                                                const int64_t* __restrict__ in_off, uint64_t full, int32_t* active,
+                                                MsCounters* c) {
+  __shared__ LdsQueue<kQueueCap> lq;
+  __shared__ unsigned long long s_red[kBS / kWave];
+  LdsQueueInit(lq);
+  const int32_t nt = cc->touched_tail;
+  unsigned long long scout = 0, fin = 0, lv = 0;
+  for (int64_t base = int64_t(blockIdx.x) * kBS; base < nt; base += int64_t(gridDim.x) * kBS) {
+    const int64_t i = base + threadIdx.x;
+    bool take = false;
+    int32_t w = 0;
+    if (i < nt) {
+      w = touched[i];
+      const uint64_t s = seen[w];
+      const uint64_t nb = acc[w] & ~s;
+      acc[w] = 0;
+      if (nb) {
+        seen[w] = s | nb;
+        fnext[w] = nb;
+        WriteDepths(depth, w, nb, level);
+        scout += static_cast<unsigned long long>(off[w + 1] - off[w]);
+        if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(in_off[w + 1] - in_off[w]);
+        lv |= nb;
+        take = true;
+      }
+    }
+    LdsQueuePush(lq, take, w, active, &c->active_tail);
+  }
+  LdsQueueFlush(lq, active, &c->active_tail);
+  scout = BlockSum<kBS>(scout, s_red);
+  fin = BlockSum<kBS>(fin, s_red);
+  lv = BlockOr<kBS>(lv, s_red);
+  if (threadIdx.x == 0) {
+    const int slot = blockIdx.x & (kStripes - 1);
+    if (scout) atomicAdd(&c->scout[slot], scout);
+    if (fin) atomicAdd(&c->finished[slot], fin);
+    if (lv) atomicOr(&c->live[slot], lv);
+  }
+}
+
+// ---- bottom-up --------------------------------------------------------------
+//
+// `live` holds the roots with bits in the current frontier: a vertex can
+// only gain bits of live roots, so it is skipped when it has all of them,
+// and the heavy tier stops scanning once it has found them.
+
+// Light tier: vertices below kHeavyEdges in-edges, 256 per block.  The
+// in-edges of the tile's vertices that still need live bits are laid out as
+// one flat range (block prefix sum); each thread gathers kUnroll
+// neighbours' frontier masks per step, locating their owner by binary
+// search over the prefix, and ORs them into the owner's LDS slot.  Writes
+// fnext for every vertex without a pending live bit and every light one.
+__global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                                  const int64_t* __restrict__ out_off,
+                                                  const uint64_t* __restrict__ frontier, uint64_t* __restrict__ seen,
+                                                  uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level, int64_t n, uint64_t live,
+                                                  uint64_t full, MsCounters* c) {
+  __shared__ int64_t s_beg[kBS];
+  __shared__ int s_pre[kBS + 1];
+  __shared__ unsigned long long s_got[kBS];
+  __shared__ int s_scan[kBS / kWave];
+  __shared__ unsigned long long s_red[kBS / kWave];
+  const int tid = threadIdx.x;
+  unsigned long long awake = 0, scout = 0, ex = 0, fin = 0, lv = 0;
+  for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
+    const int64_t i = base + tid;
+    uint64_t s = 0;
+    int64_t b = 0;
+    int d = 0;
+    bool light = false;
+    if (i < n) {
+      s = seen[i];
+      if ((live & ~s) == 0) {
+        fnext[i] = 0;
+      } else {
+        b = in_off[i];
+        const int64_t deg = in_off[i + 1] - b;
+        if (deg < kHeavyEdges) {
+          light = true;
+          d = static_cast<int>(deg);
+        }
+      }
+    }
+    s_beg[tid] = b;
+    s_got[tid] = 0;
+    int total = 0;
+    const int rank = BlockExclusiveScan<kBS>(d, &total, s_scan);
+    s_pre[tid] = rank;
+    if (tid == 0) s_pre[kBS] = total;
+    __syncthreads();
+    for (int j0 = 0; j0 < total; j0 += kBS * kUnroll) {
+      int32_t nbr[kUnroll];
+      int own[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int j = j0 + u * kBS + tid;
+        own[u] = -1;
+        nbr[u] = 0;
+        if (j < total) {
+          int lo = 0, hi = kBS;  // s_pre[lo] <= j < s_pre[hi]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= j)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          own[u] = lo;
+          nbr[u] = in_dst[s_beg[lo] + (j - s_pre[lo])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        if (own[u] >= 0) {
+          const uint64_t f = frontier[nbr[u]];
+          if (f) atomicOr(&s_got[own[u]], static_cast<unsigned long long>(f));
+        }
+      }
+    }
+    if (tid == 0) ex += static_cast<unsigned long long>(total);
+    __syncthreads();
+    if (light) {
+      const uint64_t nb = s_got[tid] & ~s;
+      fnext[i] = nb;
+      if (nb) {
+        seen[i] = s | nb;
+        WriteDepths(depth, static_cast<int32_t>(i), nb, level);
+        scout += static_cast<unsigned long long>(out_off[i + 1] - out_off[i]);
+        if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(d);
+        lv |= nb;
+        awake++;
+      }
+    }
+    __syncthreads();
+  }
+  awake = BlockSum<kBS>(awake, s_red);
+  scout = BlockSum<kBS>(scout, s_red);
+  ex = BlockSum<kBS>(ex, s_red);
+  fin = BlockSum<kBS>(fin, s_red);
+  lv = BlockOr<kBS>(lv, s_red);
+  if (threadIdx.x == 0) {
+    const int slot = blockIdx.x & (kStripes - 1);
+    if (awake) atomicAdd(&c->awake[slot], awake);
+    if (scout) atomicAdd(&c->scout[slot], scout);
+    if (ex) atomicAdd(&c->examined[slot], ex);
+    if (fin) atomicAdd(&c->finished[slot], fin);
+    if (lv) atomicOr(&c->live[slot], lv);
+  }
+}
+
+// Splits every heavy vertex's in-edges into chunks of kHeavyChunk (once per
+// graph).
+__global__ __launch_bounds__(kBS) void MsHeavyChunkBuild(const int64_t* __restrict__ in_off,
+                                                         const int32_t* __restrict__ heavy, int32_t nheavy,
+                                                         EdgeChunk* out, MsCounters* c) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < nheavy; i += int64_t(gridDim.x) * kBS) {
+    const int32_t v = heavy[i];
+    const int64_t b = in_off[v], e = in_off[v + 1];
+    const int nc = static_cast<int>((e - b + kHeavyChunk - 1) / kHeavyChunk);
+    const int base = atomicAdd(&c->heavy_chunks, nc);
+    for (int k = 0; k < nc; k++) {
+      const int64_t cb = b + k * kHeavyChunk;
+      out[base + k] = EdgeChunk{cb, cb + kHeavyChunk < e ? cb + kHeavyChunk : e, v, 0};
+    }
+  }
+}
+
+// Heavy tier: one wavefront per chunk of a vertex that still needs live
+// bits; kUnroll x 64 in-neighbours per step, OR-reduced across the
+// wavefront, stopping once this chunk and the others (acc, read without
+// atomics: it only gains bits) hold every needed bit.  Results are ORed into
+// acc[v] for MsHeavyFinish.
+__global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__ in_dst,
+                                                     const uint64_t* __restrict__ frontier,
+                                                     const uint64_t* __restrict__ seen, uint64_t* acc, uint64_t live,
+                                                     const EdgeChunk* __restrict__ chunks, int32_t nch,
+                                                     MsCounters* c) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  const int lane = LaneId();
+  const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
+  const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
+  unsigned long long ex = 0;
+  for (int64_t it = wave; it < nch; it += nwaves) {
+    const EdgeChunk ch = chunks[it];
+    const uint64_t need = live & ~seen[ch.u];
+    if (need == 0) continue;
+    uint64_t got = 0;
+    for (int64_t base = ch.beg; base < ch.end; base += kWave * kUnroll) {
+      int32_t nbr[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int64_t k = base + u * kWave + lane;
+        nbr[u] = k < ch.end ? in_dst[k] : -1;
+      }
+      uint64_t f = 0;
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++)
+        if (nbr[u] >= 0) f |= frontier[nbr[u]];
+      got |= WaveOr(f);
+      const int64_t stop = base + kWave * kUnroll;
+      ex += static_cast<unsigned long long>((stop < ch.end ? stop : ch.end) - base);
+      // The exit test must be uniform across the wavefront (WaveOr above
+      // needs every lane): lane 0 reads acc and broadcasts it.
+      uint64_t other = lane == 0 ? acc[ch.u] : 0;
+      other = __shfl(other, 0);
+      if (((got | other) & need) == need) break;
+    }
+    if (lane == 0 && (got & need)) atomicOr(reinterpret_cast<unsigned long long*>(&acc[ch.u]), got & need);
+  }
+  ex = BlockSum<kBS>(lane == 0 ? ex : 0ull, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&c->heavy_ex[blockIdx.x & (kStripes - 1)], ex);
+}
+
+// Heavy tier, second pass: acc -> new bits, one thread per heavy vertex.
+// Writes fnext for every heavy vertex.
+__global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__ heavy, int32_t nheavy,
+                                                     uint64_t* __restrict__ seen, uint64_t* __restrict__ acc,
+                                                     uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level,
+                                                     const int64_t* __restrict__ out_off,
+                                                     const int64_t* __restrict__ in_off, uint64_t full,
+                                                     MsCounters* c) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  unsigned long long awake = 0, scout = 0, fin = 0, lv = 0;
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < nheavy; i += int64_t(gridDim.x) * kBS) {
+    const int32_t v = heavy[i];
+    const uint64_t a = acc[v];
+    const uint64_t s = seen[v];
+    const uint64_t nb = a & ~s;
+    if (a) acc[v] = 0;
+    fnext[v] = nb;
+    if (nb) {
+      seen[v] = s | nb;
+      WriteDepths(depth, v, nb, level);
+      scout += static_cast<unsigned long long>(out_off[v + 1] - out_off[v]);
+      if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(in_off[v + 1] - in_off[v]);
+      lv |= nb;
+      awake++;
+    }
+  }
+  awake = BlockSum<kBS>(awake, s_red);
+  scout = BlockSum<kBS>(scout, s_red);
+  fin = BlockSum<kBS>(fin, s_red);
+  lv = BlockOr<kBS>(lv, s_red);
+  if (threadIdx.x == 0) {
+    const int slot = blockIdx.x & (kStripes - 1);
+    if (awake) atomicAdd(&c->awake[slot], awake);
+    if (scout) atomicAdd(&c->scout[slot], scout);
+    if (fin) atomicAdd(&c->finished[slot], fin);
+    if (lv) atomicOr(&c->live[slot], lv);
+  }
+}
+
+// Dense frontier -> list of its non-zero entries.
+__global__ __launch_bounds__(kBS) void MsCompact(const uint64_t* __restrict__ frontier, int64_t n, int32_t* out,
+                                                 MsCounters* c) {
+  __shared__ LdsQueue<kQueueCap> lq;
+  LdsQueueInit(lq);
+  for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
+    const int64_t i = base + threadIdx.x;
+    LdsQueuePush(lq, i < n && frontier[i] != 0, static_cast<int32_t>(i), out, &c->compact_tail);
+  }
+  LdsQueueFlush(lq, out, &c->compact_tail);
+}
+
+__global__ __launch_bounds__(kBS) void MsClearList(uint64_t* __restrict__ frontier,
+                                                   const int32_t* __restrict__ active, int32_t na) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < na; i += int64_t(gridDim.x) * kBS)
+    frontier[active[i]] = 0;
+}
+
+__global__ __launch_bounds__(kBS) void MsClearAll(uint64_t* __restrict__ frontier, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS)
+    frontier[i] = 0;
+}
+
+// out[0] = sum over vertices of popcount(seen) (reached (root, vertex)
+// pairs), out[1] = sum of out-degree x popcount(seen).
+__global__ __launch_bounds__(kBS) void MsReach(const uint64_t* __restrict__ seen, const int64_t* __restrict__ off,
+                                               int64_t n, unsigned long long* out) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  unsigned long long cnt = 0, deg = 0;
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS) {
+    const unsigned long long p = __popcll(seen[i]);
+    cnt += p;
+    deg += p * static_cast<unsigned long long>(off[i + 1] - off[i]);
+  }
+  cnt = BlockSum<kBS>(cnt, s_red);
+  deg = BlockSum<kBS>(deg, s_red);
+  if (threadIdx.x == 0) {
+    if (cnt) atomicAdd(&out[0], cnt);
+    if (deg) atomicAdd(&out[1], deg);
+  }
+}
+
+__global__ void MsZero2(unsigned long long* p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0;
+}
+
+// depth16[v][r] -> depth32[v] of root r (-1 where unreached).
+__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16, int64_t n, int r,
+                                                  int32_t* __restrict__ out) {
+  for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
+    const uint16_t d = d16[v * 64 + r];
+    out[v] = d == kNoDepth ? -1 : d;
+  }
+}
+
+struct MsTraversal {
+  uint64_t* seen = nullptr;          // final visit masks
+  std::vector<uint64_t*> frontiers;  // keep mode: frontiers[L] holds the bits that arrived at level L
+  int64_t td = 0, bu = 0, examined = 0, launches = 0;
+};
+
+// Runs the level loop on g->stream from the k (1..64) roots in droots
+// (device memory, already validated).  depth16 (n x 64, may be null)
+// receives the depths.  With keep, every level's frontier array is kept for
+// the caller (workspace arrays named ms.level.<L>, valid until the next
+// call).  The caller has set the device and started timing.
+inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, int k, int alpha, uint16_t* depth16, bool keep) {
+  MsTraversal res;
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  const dim3 blk(kBS);
+  MsCounters* dc = g->Scratch<MsCounters>("ms.ctr", 1);
+  int32_t* heavy = g->Scratch<int32_t>("ms.heavy", n + 1);
+  // Heavy vertices and their in-edge chunks depend only on the graph.
+  const int64_t max_chunks = g->in_entries / kHeavyChunk + g->in_entries / kHeavyEdges + 1;
+  EdgeChunk* hchunks = g->Scratch<EdgeChunk>("ms.hchunks", max_chunks);
+  if (g->ms_heavy < 0) {
+    GG_HIP(hipMemsetAsync(dc, 0, sizeof(MsCounters), st));
+    MsHeavyList<<<dim3(GridFor(n, kBS, g->cu_count * 4)), blk, 0, st>>>(g->in_off, n, heavy, dc);
+    GG_HIP(hipGetLastError());
+    int32_t nh = 0;
+    GG_HIP(hipMemcpyAsync(&nh, &dc->heavy_tail, sizeof(nh), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    MsHeavyChunkBuild<<<dim3(GridFor(nh, kBS, g->cu_count * 4)), blk, 0, st>>>(g->in_off, heavy, nh, hchunks,
+                                                                                dc);
+    GG_HIP(hipGetLastError());
+    int32_t nc = 0;
+    GG_HIP(hipMemcpyAsync(&nc, &dc->heavy_chunks, sizeof(nc), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    GG_REQUIRE(nc <= max_chunks, GG_EHIP, "heavy chunk count exceeds its bound");
+    g->ms_heavy = nh;
+    g->ms_heavy_chunks = nc;
+  }
+  const int32_t nheavy = static_cast<int32_t>(g->ms_heavy);
+  const int32_t nhchunks = static_cast<int32_t>(g->ms_heavy_chunks);
+  uint64_t* seen = g->Scratch<uint64_t>("ms.seen", n);
+  // keep: one frontier array per level, else two alternating ones.
+  auto level_array = [&](int level) {
+    if (!keep) return g->Scratch<uint64_t>((level & 1) ? "ms.fb" : "ms.fa", n);
+    return g->Scratch<uint64_t>(("ms.level." + std::to_string(level)).c_str(), n);
+  };
+  uint64_t* fa = level_array(0);
+  uint64_t* fb = level_array(1);
+  if (keep) res.frontiers.push_back(fa);
+  uint64_t* acc = g->Scratch<uint64_t>("ms.acc", n);
+  int32_t* act_a = g->Scratch<int32_t>("ms.act_a", n + 64);
+  int32_t* act_b = g->Scratch<int32_t>("ms.act_b", n + 64);
+  int32_t* touched = g->Scratch<int32_t>("ms.touched", n + 64);
+  const bool use_chunks = g->max_degree >= kExpandBS;
+  EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("ms.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+  const uint64_t full = k == 64 ? ~0ull : ((1ull << k) - 1);
+  const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
+  const double N = static_cast<double>(n);
+  int64_t& launches = res.launches;
+  unsigned long long mail[kMailValues];
+  auto publish = [&]() {
+    const unsigned long long seq = MailNext(g);
+    Run(g, "msbfs_publish", 0, MsPublish, dim3(1), dim3(64), dc, g->mail_dev, seq);
+    launches++;
+    MailWait(g, seq, mail, kMailValues);
+  };
+
+  Run(g, "msbfs_init", 32 * N + 128 * N, MsInit, dim3(GridFor(n * 64, kBS, g->cu_count * 16)), blk, seen, fa,
+      fb, acc, depth16, n, dc);
+  Run(g, "msbfs_roots", 0, MsRoots, dim3(1), dim3(64), static_cast<const int32_t*>(droots), k, seen, fa,
+      depth16, act_a, dc);
+  launches += 2;
+  publish();
+  int64_t na = static_cast<int64_t>(mail[kMailActive]);
+  int64_t scout = 0;  // level 1 is top-down whatever the roots' degrees
+  int64_t unfinished = g->in_entries;
+  uint64_t* fcur = fa;
+  uint64_t* fnext = fb;
+  int32_t* act = act_a;
+  int32_t* act_next = act_b;
+  // Invariants at the top of a level: fcur holds the last level's bits
+  // (listed in act[0, na) when `listed`); fnext is all zero unless the
+  // level is bottom-up (which overwrites every entry).
+  bool listed = true;
+  bool bottom_up = false;
+  uint64_t live = full;  // roots with bits in fcur
+  int64_t& td = res.td;
+  int64_t& bu = res.bu;
+  int64_t& examined = res.examined;
+  for (int level = 1; na > 0; level++) {
+    GG_REQUIRE(level < 65535, GG_ENOTSUP, "more than 65534 levels");
+    const uint16_t lv = static_cast<uint16_t>(level);
+    if (keep) {
+      // A fresh array per level; a top-down level writes only the entries
+      // that gain bits, so it starts from zeros.
+      fnext = level_array(level);
+      res.frontiers.push_back(fnext);
+      if (!bottom_up) {
+        Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fnext, n);
+        launches++;
+      }
+    }
+    if (bottom_up) {
+      // Per vertex: seen (8 B) and fnext (8 B); per in-edge gathered: 4 B
+      // and the neighbour's 8 B frontier mask.
+      const int idx = Run(g, "msbfs_bottom_up", 16 * N, MsBottomUp, grid_n, blk,
+                          static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
+                          static_cast<const int64_t*>(g->out_off), static_cast<const uint64_t*>(fcur), seen,
+                          fnext, depth16, lv, n, live, full, dc);
+      const int hidx = Run(g, "msbfs_bottom_up_heavy", 0, MsHeavyChunks,
+                           dim3(GridFor(int64_t(nhchunks) * kWave, kBS, g->cu_count * 16)), blk,
+                           static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(fcur),
+                           static_cast<const uint64_t*>(seen), acc, live,
+                           static_cast<const EdgeChunk*>(hchunks), nhchunks, dc);
+      Run(g, "msbfs_heavy_finish", 32.0 * nheavy, MsHeavyFinish, dim3(GridFor(nheavy, kBS, g->cu_count * 4)),
+          blk, static_cast<const int32_t*>(heavy), nheavy, seen, acc, fnext, depth16, lv,
+          static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), full, dc);
+      launches += 3;
+      bu++;
+      publish();
+      CreditBytes(g, idx, 12.0 * mail[kMailExamined]);
+      CreditBytes(g, hidx, 12.0 * mail[kMailHeavyEx]);
+    } else {
+      if (!listed) {
+        Run(g, "msbfs_compact", 8 * N, MsCompact, dim3(GridFor(n, kBS, g->cu_count * 4)), blk,
+            static_cast<const uint64_t*>(fcur), n, act, dc);
+        launches++;
+        listed = true;
+      }
+      const int64_t chunk_min = na <= 4096 ? kExpandBS : kChunkEdges;
+      const int idx = Run(g, "msbfs_top_down", 24.0 * na, MsTopDown, dim3(GridFor(na, kBS, g->cu_count * 8)),
+                          blk, static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+                          static_cast<const uint64_t*>(fcur), static_cast<const uint64_t*>(seen), acc,
+                          static_cast<const int32_t*>(act), static_cast<int32_t>(na), touched, dc,
+                          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
+      launches++;
+      if (use_chunks) {
+        Run(g, "msbfs_top_down_chunks", 0, MsTopDownChunks, dim3(g->cu_count * 4), blk,
+            static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(fcur),
+            static_cast<const uint64_t*>(seen), acc, touched, dc, static_cast<const EdgeChunk*>(chunks));
+        launches++;
+      }
+      Run(g, "msbfs_update", 0, MsUpdate, dim3(g->cu_count * 4), blk, static_cast<const int32_t*>(touched),
+          static_cast<const MsCounters*>(dc), acc, seen, fnext, depth16, lv,
+          static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), full, act_next, dc);
+      launches++;
+      td++;
+      publish();
+      // 4 B per edge read, the neighbour's 8 B seen mask and 8 B accumulator.
+      CreditBytes(g, idx, 20.0 * mail[kMailExamined]);
+    }
+    examined += static_cast<int64_t>(mail[kMailExamined] + mail[kMailHeavyEx]);
+    const int64_t na_next = static_cast<int64_t>(mail[kMailActive]);
+    scout = static_cast<int64_t>(mail[kMailScout]);
+    unfinished -= static_cast<int64_t>(mail[kMailFinished]);
+    live = mail[kMailLive];
+    const bool next_bu = scout > unfinished / alpha;
+    // A top-down level writes only the entries that gain bits, so the
+    // array it will write (fcur, just consumed) must be cleared first.
+    if (!keep && na_next > 0 && !next_bu) {
+      if (listed) {
+        Run(g, "msbfs_clear", 12.0 * na, MsClearList, dim3(GridFor(na, kBS, g->cu_count * 8)), blk, fcur,
+            static_cast<const int32_t*>(act), static_cast<int32_t>(na));
+      } else {
+        Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fcur, n);
+      }
+      launches++;
+    }
+    listed = !bottom_up;
+    bottom_up = next_bu;
+    na = na_next;
+    std::swap(fcur, fnext);
+    std::swap(act, act_next);
+  }
+  res.seen = seen;
+  return res;
+
+}
+
+}  // namespace
+}  // namespace gg

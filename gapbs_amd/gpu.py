@@ -60,7 +60,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_bfs_batch", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_tc",
+    "gg_bfs", "gg_bfs_batch", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
+        L.gg_bc_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
@@ -306,6 +307,17 @@ def bc(dg: DeviceGraph, roots, fp64: bool = False, kahan: bool = False,
     st = Stats()
     _check(lib().gg_bc(dg.handle, _p(roots, C.c_int32), len(roots), mode,
                        _p(out, C.c_float) if want_scores else None, C.byref(st)))
+    return BCResult(out, st.as_dict())
+
+
+def bc_batch(dg: DeviceGraph, roots, fp64: bool = False, want_scores: bool = True) -> BCResult:
+    """gg_bc_batch: up to 64 roots in one traversal; equals gg_bc over the
+    same roots (bit for bit with the reference's float arithmetic)."""
+    roots = np.ascontiguousarray(roots, dtype=np.int32)
+    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    st = Stats()
+    _check(lib().gg_bc_batch(dg.handle, _p(roots, C.c_int32), len(roots), BC_FP64 if fp64 else BC_FP32,
+                             _p(out, C.c_float) if want_scores else None, C.byref(st)))
     return BCResult(out, st.as_dict())
 
 

@@ -126,6 +126,18 @@ int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
 enum { GG_BC_FP32 = 0, GG_BC_FP64 = 1, GG_BC_FP32_KAHAN = 2 };
 int gg_bc(gg_graph* g, const int32_t* roots, int k, int mode, float* scores, gg_stats* stats);
 
+/* Batched Brandes: the k (1..64) roots traversed together (64-bit visit
+ * masks per vertex, path counts and dependencies stored per (vertex, root)).
+ * Replaces the reference's per-root loop in Brandes (src/bc.cc:108-140):
+ * the result equals gg_bc over the same roots -- bit for bit in
+ * GG_BC_FP32 (each root's dependencies are folded in successor order with
+ * the reference's arithmetic and added to the scores in root order), to
+ * FP64 rounding in GG_BC_FP64.  GG_BC_FP32_KAHAN returns GG_ENOTSUP.
+ * Device memory beyond the graph: about (8 x levels + 512 + 256) bytes per
+ * vertex in GG_BC_FP32 (1024 + 8 x levels in GG_BC_FP64).
+ * scores[n] (nullable) receives float scores divided by the largest. */
+int gg_bc_batch(gg_graph* g, const int32_t* roots, int k, int mode, float* scores, gg_stats* stats);
+
 /* Ordered triangle count of the uploaded undirected graph.
  * Replaces OrderedCount(const Graph&) (src/tc.cc:52-71); the Hybrid
  * relabel-by-degree decision (tc.cc:75-100) is made by the caller, which
