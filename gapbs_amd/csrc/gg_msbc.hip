@@ -365,7 +365,7 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
   const double N = static_cast<double>(n);
 
   BeginTiming(g);
-  MsTraversal t = MsTraverse(g, droots, k, 15, nullptr, true);
+  MsTraversal t = MsTraverse(g, droots, nullptr, k, k == 64 ? ~0ull : (1ull << k) - 1, 15, nullptr, true);
   const int levels = static_cast<int>(t.frontiers.size());
   int64_t launches = t.launches;
   Run(g, "msbc_clear", 0, MbClear, dim3(1), blk, ctr, 2 * kMbStripes + 1);

@@ -138,20 +138,19 @@ __global__ __launch_bounds__(kBS) void MsHeavyList(const int64_t* __restrict__ i
 }
 
 // Roots: seen and frontier bits, depth 0, the first active list (each
-// distinct root vertex once; a repeated root sets several bits on it).
-// One block of 64 threads.
-__global__ void MsRoots(const int32_t* __restrict__ roots, int k, uint64_t* seen, uint64_t* frontier,
-                        uint16_t* depth, int32_t* active, MsCounters* c) {
-  const int r = threadIdx.x;
-  if (r < k) {
-    const int32_t v = roots[r];
-    atomicOr(reinterpret_cast<unsigned long long*>(&seen[v]), 1ull << r);
-    atomicOr(reinterpret_cast<unsigned long long*>(&frontier[v]), 1ull << r);
-    if (depth) depth[int64_t(v) * 64 + r] = 0;
-    bool first = true;
-    for (int j = 0; j < r; j++)
-      if (roots[j] == v) first = false;
-    if (first) active[atomicAdd(&c->active_tail, 1)] = v;
+// distinct root vertex once: the thread whose bit lands first on it; a
+// repeated root sets several bits on it).  Root i takes bit bits[i], or bit
+// i when bits is null.
+__global__ void MsRoots(const int32_t* __restrict__ roots, const int32_t* __restrict__ bits, int count,
+                        uint64_t* seen, uint64_t* frontier, uint16_t* depth, int32_t* active, MsCounters* c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    const int32_t v = roots[i];
+    const int b = bits ? bits[i] : i;
+    atomicOr(reinterpret_cast<unsigned long long*>(&seen[v]), 1ull << b);
+    const uint64_t old = atomicOr(reinterpret_cast<unsigned long long*>(&frontier[v]), 1ull << b);
+    if (depth) depth[int64_t(v) * 64 + b] = 0;
+    if (old == 0) active[atomicAdd(&c->active_tail, 1)] = v;
   }
 }
 
@@ -551,12 +550,16 @@ struct MsTraversal {
   int64_t td = 0, bu = 0, examined = 0, launches = 0;
 };
 
-// Runs the level loop on g->stream from the k (1..64) roots in droots
-// (device memory, already validated).  depth16 (n x 64, may be null)
-// receives the depths.  With keep, every level's frontier array is kept for
-// the caller (workspace arrays named ms.level.<L>, valid until the next
-// call).  The caller has set the device and started timing.
-inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, int k, int alpha, uint16_t* depth16, bool keep) {
+// Runs the level loop on g->stream from the `count` roots in droots
+// (device memory, already validated).  Root i takes bit dbits[i] (device
+// memory; `full` is the OR of the bits used), or bit i when dbits is null
+// (count <= 64).  Roots sharing a bit must lie in different components.
+// depth16 (n x 64, may be null) receives the depths per bit.  With keep,
+// every level's frontier array is kept for the caller (workspace arrays
+// named ms.level.<L>, valid until the next call).  The caller has set the
+// device and started timing.
+inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t* dbits, int count, uint64_t full,
+                              int alpha, uint16_t* depth16, bool keep) {
   MsTraversal res;
   const int64_t n = g->n;
   hipStream_t st = g->stream;
@@ -600,7 +603,6 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, int k, int alp
   int32_t* touched = g->Scratch<int32_t>("ms.touched", n + 64);
   const bool use_chunks = g->max_degree >= kExpandBS;
   EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("ms.chunks", MaxChunks(n, g->out_entries)) : nullptr;
-  const uint64_t full = k == 64 ? ~0ull : ((1ull << k) - 1);
   const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
   const double N = static_cast<double>(n);
   int64_t& launches = res.launches;
@@ -614,8 +616,8 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, int k, int alp
 
   Run(g, "msbfs_init", 32 * N + 128 * N, MsInit, dim3(GridFor(n * 64, kBS, g->cu_count * 16)), blk, seen, fa,
       fb, acc, depth16, n, dc);
-  Run(g, "msbfs_roots", 0, MsRoots, dim3(1), dim3(64), static_cast<const int32_t*>(droots), k, seen, fa,
-      depth16, act_a, dc);
+  Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), static_cast<const int32_t*>(droots),
+      dbits, count, seen, fa, depth16, act_a, dc);
   launches += 2;
   publish();
   int64_t na = static_cast<int64_t>(mail[kMailActive]);

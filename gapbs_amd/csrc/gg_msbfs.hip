@@ -26,7 +26,7 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
     int32_t* droots = g->Scratch<int32_t>("ms.roots", 64);
     GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
     BeginTiming(g);
-    const MsTraversal t = MsTraverse(g, droots, k, alpha, depth16, false);
+    const MsTraversal t = MsTraverse(g, droots, nullptr, k, k == 64 ? ~0ull : (1ull << k) - 1, alpha, depth16, false);
     GG_HIP(hipGetLastError());
     FinishTiming(g, stats, wt);
     if (depths) {
@@ -35,6 +35,129 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
         MsDepthOut<<<grid_n, blk, 0, st>>>(depth16, n, r, d32);
         GG_HIP(hipMemcpyAsync(depths + int64_t(r) * n, d32, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
       }
+      GG_HIP(hipStreamSynchronize(st));
+    }
+    if (stats) {
+      unsigned long long* sums = g->Scratch<unsigned long long>("ms.reach", 2);
+      MsZero2<<<1, 64, 0, st>>>(sums);
+      MsReach<<<grid_n, blk, 0, st>>>(t.seen, g->out_off, n, sums);
+      unsigned long long h[2];
+      GG_HIP(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      stats->total_ms = wt.ms();
+      stats->iterations = t.td + t.bu;
+      stats->td_steps = t.td;
+      stats->bu_steps = t.bu;
+      stats->reached = static_cast<int64_t>(h[0]);
+      stats->reached_edges = static_cast<int64_t>(h[1]);
+      stats->edges_examined = t.examined;
+      stats->launches = t.launches;
+    }
+    GG_HIP(hipGetLastError());
+  });
+}
+
+namespace gg {
+namespace {
+
+// Checks that every vertex's neighbours lie in its own part (the lists are
+// sorted, so the first and last entries decide); sets *bad otherwise.
+__global__ __launch_bounds__(kBS) void PkCheckParts(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                    int64_t n, const int64_t* __restrict__ part, int nparts,
+                                                    int32_t* bad) {
+  for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
+    const int64_t b = off[v], e = off[v + 1];
+    if (b == e) continue;
+    int lo = 0, hi = nparts;  // part[lo] <= v < part[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (part[mid] <= v)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    if (dst[b] < part[lo] || dst[e - 1] >= part[lo + 1]) atomicOr(bad, 1);
+  }
+}
+
+// depth16[v][j] -> out[R * part[i] + j * |part i| + (v - part[i])] for the
+// part i holding v, -1 where unreached.
+__global__ __launch_bounds__(kBS) void PkDepthOut(const uint16_t* __restrict__ d16, int64_t n,
+                                                  const int64_t* __restrict__ part, int nparts, int R,
+                                                  int32_t* __restrict__ out) {
+  const int64_t total = n * R;
+  for (int64_t x = blockIdx.x * int64_t(kBS) + threadIdx.x; x < total; x += int64_t(gridDim.x) * kBS) {
+    const int64_t v = x / R;
+    const int j = static_cast<int>(x % R);
+    int lo = 0, hi = nparts;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (part[mid] <= v)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int64_t size = part[lo + 1] - part[lo];
+    const uint16_t d = d16[v * 64 + j];
+    out[R * part[lo] + j * size + (v - part[lo])] = d == kNoDepth ? -1 : d;
+  }
+}
+
+}  // namespace
+}  // namespace gg
+
+extern "C" int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const int32_t* roots, int roots_per_part,
+                             int alpha, int32_t* depths, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "BFS needs an unweighted graph upload");
+    GG_REQUIRE(nparts >= 1 && part && roots, GG_EINVAL, "null part or root list");
+    const int R = roots_per_part;
+    GG_REQUIRE(R >= 1 && R <= 64, GG_EINVAL, "1 to 64 roots per part");
+    GG_REQUIRE(alpha > 0, GG_EINVAL, "alpha must be positive");
+    GG_REQUIRE(part[0] == 0 && part[nparts] == g->n, GG_EINVAL, "parts must cover the vertices");
+    for (int i = 0; i < nparts; i++) {
+      GG_REQUIRE(part[i] < part[i + 1], GG_EINVAL, "parts must be non-empty and increasing");
+      for (int j = 0; j < R; j++) {
+        const int32_t r = roots[int64_t(i) * R + j];
+        GG_REQUIRE(r >= part[i] && r < part[i + 1], GG_EINVAL, "root outside its part");
+      }
+    }
+    GG_HIP(hipSetDevice(g->device));
+    const int64_t n = g->n;
+    const int64_t count = int64_t(nparts) * R;
+    GG_REQUIRE(count < (int64_t(1) << 30), GG_EINVAL, "too many roots");
+    hipStream_t st = g->stream;
+    const dim3 blk(kBS);
+    const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
+    int64_t* dpart = g->Scratch<int64_t>("pk.part", nparts + 1);
+    int32_t* droots = g->Scratch<int32_t>("pk.roots", count);
+    int32_t* dbits = g->Scratch<int32_t>("pk.bits", count);
+    int32_t* bad = g->Scratch<int32_t>("pk.bad", 1);
+    std::vector<int32_t> bits(count);
+    for (int64_t x = 0; x < count; x++) bits[x] = static_cast<int32_t>(x % R);
+    GG_HIP(hipMemcpyAsync(dpart, part, sizeof(int64_t) * (nparts + 1), hipMemcpyHostToDevice, st));
+    GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * count, hipMemcpyHostToDevice, st));
+    GG_HIP(hipMemcpyAsync(dbits, bits.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, st));
+    GG_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), st));
+    PkCheckParts<<<grid_n, blk, 0, st>>>(g->out_off, g->out_dst, n, dpart, nparts, bad);
+    if (g->directed) PkCheckParts<<<grid_n, blk, 0, st>>>(g->in_off, g->in_dst, n, dpart, nparts, bad);
+    int32_t hbad = 0;
+    GG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    GG_REQUIRE(hbad == 0, GG_EINVAL, "an edge joins two parts");
+    uint16_t* depth16 = g->Scratch<uint16_t>("ms.depth16", n * 64);
+    const uint64_t full = R == 64 ? ~0ull : (1ull << R) - 1;
+    BeginTiming(g);
+    const MsTraversal t = MsTraverse(g, droots, dbits, static_cast<int>(count), full, alpha, depth16, false);
+    GG_HIP(hipGetLastError());
+    FinishTiming(g, stats, wt);
+    if (depths) {
+      int32_t* d32 = g->Scratch<int32_t>("pk.depth_out", n * R);
+      PkDepthOut<<<dim3(GridFor(n * R, kBS, g->cu_count * 16)), blk, 0, st>>>(depth16, n, dpart, nparts, R, d32);
+      GG_HIP(hipMemcpyAsync(depths, d32, sizeof(int32_t) * n * R, hipMemcpyDeviceToHost, st));
       GG_HIP(hipStreamSynchronize(st));
     }
     if (stats) {

@@ -60,7 +60,7 @@ class Stats(C.Structure):
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_bfs_batch", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
+    "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
@@ -85,6 +85,7 @@ def lib() -> C.CDLL:
         L.gg_graph_device_bytes.restype = C.c_int64
         L.gg_bfs.argtypes = [vp, C.c_int32, C.c_int, C.c_int, _i32p, _i32p, C.POINTER(Stats)]
         L.gg_bfs_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(Stats)]
+        L.gg_bfs_packed.argtypes = [vp, _i64p, C.c_int, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
@@ -217,6 +218,29 @@ def bfs_batch(dg: DeviceGraph, roots, alpha: int = 15, want_depths: bool = True)
     _check(lib().gg_bfs_batch(dg.handle, _p(r, C.c_int32), len(r), alpha,
                               _p(depths, C.c_int32) if want_depths else None, C.byref(st)))
     return BatchBFSResult(depths, st.as_dict())
+
+
+@dataclass
+class PackedBFSResult:
+    depths: list | None  # per part: [R, |part|] int32, -1 where unreached
+    stats: dict
+
+
+def bfs_packed(dg: DeviceGraph, part, roots, alpha: int = 15, want_depths: bool = True) -> PackedBFSResult:
+    """gg_bfs_packed: one traversal over a disjoint union of graphs (vertex
+    ranges part[i]..part[i+1]); roots is [nparts, R] global vertex ids."""
+    part = np.ascontiguousarray(part, dtype=np.int64)
+    r = np.ascontiguousarray(roots, dtype=np.int32)
+    nparts, R = r.shape
+    n = dg.num_nodes
+    flat = np.empty(R * n, dtype=np.int32) if want_depths else None
+    st = Stats()
+    _check(lib().gg_bfs_packed(dg.handle, _p(part, C.c_int64), nparts, _p(r, C.c_int32), R, alpha,
+                               _p(flat, C.c_int32) if want_depths else None, C.byref(st)))
+    depths = None
+    if want_depths:
+        depths = [flat[R * part[i]:R * part[i + 1]].reshape(R, part[i + 1] - part[i]) for i in range(nparts)]
+    return PackedBFSResult(depths, st.as_dict())
 
 
 K_DIST_INF = (2**31 - 1) // 2  # src/sssp.cc:64

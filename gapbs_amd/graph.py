@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         L.gh_strip_weights.argtypes = [vp, C.POINTER(vp)]
         L.gh_relabel_by_degree.argtypes = [vp, C.POINTER(vp)]
         L.gh_worth_relabelling.argtypes = [vp]
+        L.gh_pack.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp), _i64p]
         L.gh_pick_sources.argtypes = [vp, C.c_int64, C.c_int, _i32p]
         L.gh_verify_bfs.argtypes = [vp, C.c_int32, _i32p]
         L.gh_verify_sssp.argtypes = [vp, C.c_int32, _i32p]
@@ -141,6 +142,16 @@ class Graph:
 
     def relabel_by_degree(self) -> "Graph":
         return Graph._make(lib().gh_relabel_by_degree, self._h)
+
+    @classmethod
+    def pack(cls, graphs) -> tuple["Graph", np.ndarray]:
+        """Disjoint union of unweighted graphs: (union, part bounds); graph
+        i's vertex v is bounds[i] + v in the union."""
+        hs = (C.c_void_p * len(graphs))(*[g._h for g in graphs])
+        bounds = np.empty(len(graphs) + 1, dtype=np.int64)
+        h = C.c_void_p()
+        _check(lib().gh_pack(hs, len(graphs), C.byref(h), _ptr(bounds, C.c_int64)))
+        return cls(h.value), bounds
 
     def worth_relabelling(self) -> bool:
         return bool(_check(lib().gh_worth_relabelling(self._h)))

@@ -173,6 +173,18 @@ int gh_relabel_by_degree(const gh_graph* g, gh_graph** out) {
   });
 }
 
+int gh_pack(const gh_graph* const* graphs, int count, gh_graph** out, int64_t* bounds) {
+  return Guard([&] {
+    if (count < 1 || !graphs) throw Error(-2, "no graphs to pack");
+    std::vector<const Graph*> parts;
+    for (int i = 0; i < count; i++) parts.push_back(&Unweighted(graphs[i]));
+    std::vector<int64_t> b;
+    *out = Wrap(PackGraphs(parts, &b));
+    if (bounds) std::copy(b.begin(), b.end(), bounds);
+    return 0;
+  });
+}
+
 int gh_worth_relabelling(const gh_graph* g) {
   return Guard([&] { return WorthRelabelling(Unweighted(g)) ? 1 : 0; });
 }

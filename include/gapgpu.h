@@ -89,6 +89,20 @@ int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* parent, in
  * (root, vertex) pairs; stats->reached_edges sums their out-degrees. */
 int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha, int32_t* depths, gg_stats* stats);
 
+/* Packed batched BFS: the upload is the disjoint union of nparts graphs,
+ * part i holding vertices [part[i], part[i+1]) (part[0] = 0, part[nparts] =
+ * n); no edge may join two parts (checked: GG_EINVAL).  roots[i * R + j]
+ * (R = roots_per_part, 1..64) is root j of part i and must lie in part i.
+ * All nparts * R roots run in one traversal: root j of every part takes
+ * visit-mask bit j (parts never meet, so bits are shared safely).  No
+ * reference counterpart: the reference runs one graph and one root per
+ * trial (src/benchmark.h:106); each root's depths equal its single-root BFS
+ * depths on its own graph.  depths[R * n] (nullable), part-major: part i's
+ * block starts at R * part[i] and holds R rows of |part i| depths (-1 if
+ * unreached), row j for root j. */
+int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const int32_t* roots, int roots_per_part,
+                  int alpha, int32_t* depths, gg_stats* stats);
+
 /* Delta-stepping single-source shortest paths from `source` with bucket
  * width `delta` on a weighted upload.
  * Replaces DeltaStep(const WGraph&, NodeID source, WeightT delta, bool

@@ -61,6 +61,11 @@ const void* gh_in_dests(const gh_graph* g);
 int gh_strip_weights(const gh_graph* g, gh_graph** out);
 /* Builder::RelabelByDegree (src/builder.h:367-402). */
 int gh_relabel_by_degree(const gh_graph* g, gh_graph** out);
+/* Disjoint union of `count` unweighted graphs (all undirected or all
+ * directed): graph i's vertex v becomes bounds[i] + v; bounds[count + 1]
+ * (nullable) receives the part boundaries, bounds[0] = 0.  No reference
+ * counterpart: it feeds gg_bfs_packed. */
+int gh_pack(const gh_graph* const* graphs, int count, gh_graph** out, int64_t* bounds);
 /* tc.cc:75-91 WorthRelabelling heuristic: 1 / 0. */
 int gh_worth_relabelling(const gh_graph* g);
 
