@@ -13,11 +13,20 @@ single-root direction-optimizing BFS calls ("bfs"), and the other kernels
 run on the same graph, each timed in its own loop with results left on the
 device:
   sssp  delta-stepping from the first --sssp-roots roots (weights in [1, 255])
-  pr    PageRank, damping 0.85, tolerance 1e-4, at most 20 iterations (FP32)
+  pr    PageRank, damping 0.85, tolerance 1e-4, at most 20 iterations (FP32,
+        compensated per-vertex sums)
+  pr_fp64  the same in FP64; the FP32 result's max-abs and L1 error against it
+        and the FP64/FP32 time ratio are reported (the FP64 path is held to
+        the FP64 oracle in the tests)
   cc    Afforest
   bc    Brandes from the first --bc-roots roots (the reference's float arithmetic)
   bc_batch  Brandes from all 64 roots in one batched traversal (same arithmetic)
   tc    ordered triangle count on the graph relabelled by degree
+
+Before timing, gg_autotune picks this graph's BFS alpha/beta, batched
+alpha, SSSP delta and PageRank bin cut-offs from the same roots (its time is
+reported under config.autotune and not counted; --no-autotune keeps the
+reference's alpha 15 / beta 18 and --delta).
 
 The roofline figure is measured live: the library times each kernel launch
 with HIP events carried by the dispatch packet and credits it with its
@@ -44,7 +53,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "cc", "bc", "tc")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc")
 
 
 def log(msg):
@@ -197,6 +206,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per sampled CPU baseline")
     ap.add_argument("--tc-stride", type=int, default=64, help="CPU TC sample: one vertex in stride")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the reference's alpha/beta (15/18), delta --delta and default PageRank bins")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run one device copy of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -242,15 +253,17 @@ def main():
 
     def run_kernel(k):
         if k == "bfs_batch":
-            return gpu.bfs_batch(dg, roots, args.alpha, want_depths=False).stats
+            return gpu.bfs_batch(dg, roots, batch_alpha, want_depths=False).stats
         if k == "bfs":
             for r in roots:
-                gpu.bfs(dg, r, args.alpha, want_depth=False, want_parent=False)
+                gpu.bfs(dg, r, alpha, beta, want_depth=False, want_parent=False)
         elif k == "sssp":
             for r in sssp_roots:
-                gpu.sssp(dwg, r, args.delta, want_dist=False)
+                gpu.sssp(dwg, r, delta, want_dist=False)
         elif k == "pr":
             return gpu.pagerank(dg, 20, 1e-4, want_scores=False).stats
+        elif k == "pr_fp64":
+            return gpu.pagerank(dg, 20, 1e-4, fp64=True, want_scores=False).stats
         elif k == "cc":
             gpu.cc(dg, want_comp=False)
         elif k == "bc":
@@ -260,6 +273,30 @@ def main():
         elif k == "tc":
             return gpu.tc(drg)[1]
         return None
+
+    # Per-graph tuning (gg_autotune) before any timing; its own time is
+    # reported, not included.  Results do not depend on the settings.
+    alpha, beta, batch_alpha, delta = args.alpha, 18, args.alpha, args.delta
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.time()
+        what = 0
+        if "bfs" in kernels:
+            what |= gpu.TUNE_BFS
+        if "bfs_batch" in kernels:
+            what |= gpu.TUNE_BATCH
+        if "pr" in kernels or "pr_fp64" in kernels:
+            what |= gpu.TUNE_PR
+        if what:
+            tuned = gpu.autotune(dg, what, roots)
+        if "sssp" in kernels:
+            tw = gpu.autotune(dwg, gpu.TUNE_SSSP, sssp_roots)
+            tuned = {**(tuned or {}), "sssp_delta": tw["sssp_delta"]}
+        if tuned is not None:
+            tuned["seconds"] = round(time.time() - t0, 2)
+            alpha, beta = tuned.get("bfs_alpha", alpha), tuned.get("bfs_beta", beta)
+            batch_alpha, delta = tuned.get("batch_alpha", batch_alpha), tuned.get("sssp_delta", delta)
+            log(f"autotune: {tuned}")
 
     traffic = load_traffic(workload)
     results = {}
@@ -287,7 +324,7 @@ def main():
             r["gteps"] = round(sum(m_per_root[:len(bc_roots)]) / per / 1e9, 3)
         if k == "bc_batch":
             r["gteps"] = round(m_total / per / 1e9, 3)
-        if k == "pr" and last:
+        if k in ("pr", "pr_fp64") and last:
             r["iterations"] = last["iterations"]
         if k == "tc" and last is not None:
             r["probes"] = last["edges_examined"]
@@ -296,6 +333,15 @@ def main():
     if "bfs_batch" in results and "bfs" in results:
         results["bfs_batch"]["speedup_vs_sequential"] = round(results["bfs"]["seconds"] /
                                                               results["bfs_batch"]["seconds"], 3)
+    if "pr" in results and "pr_fp64" in results:
+        import numpy as np
+        s32 = gpu.pagerank(dg, 20, 1e-4).scores.astype(np.float64)
+        s64 = gpu.pagerank(dg, 20, 1e-4, fp64=True).scores.astype(np.float64)
+        results["pr"]["vs_fp64"] = {"max_abs": float(np.abs(s32 - s64).max()),
+                                    "l1": float(np.abs(s32 - s64).sum()),
+                                    "fp64_over_fp32_time": round(results["pr_fp64"]["seconds"] /
+                                                                 results["pr"]["seconds"], 3)}
+        log(f"pr precision: {results['pr']['vs_fp64']}")
     if "bc_batch" in results and "bc" in results:  # per-root GTEPS: bc runs fewer roots
         results["bc_batch"]["speedup_vs_sequential"] = round(results["bc_batch"]["gteps"] /
                                                              results["bc"]["gteps"], 3)
@@ -326,7 +372,8 @@ def main():
                                f"({len(sssp_roots)} roots), pr, cc, bc ({len(bc_roots)} roots), batched bc "
                                f"({len(roots)} roots), tc",
                    "num_nodes": g.num_nodes, "num_edges_directed": g.num_edges_directed,
-                   "roots": len(roots), "alpha": args.alpha, "sssp_delta": args.delta},
+                   "roots": len(roots), "bfs_alpha_beta": [alpha, beta], "batch_alpha": batch_alpha,
+                   "sssp_delta": delta, "autotune": tuned},
         "roofline": head["roofline"],
         "kernels": results,
         "cpu_baseline": ({**cpu["bfs"], "sample": cpu["bfs"]["sample"] + " (the reference has no batched "

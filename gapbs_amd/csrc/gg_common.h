@@ -94,6 +94,7 @@ struct gg_graph {
   size_t graph_bytes = 0;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
+  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}};  // gg_autotune results
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for

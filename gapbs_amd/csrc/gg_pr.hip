@@ -25,8 +25,6 @@ namespace gg {
 namespace {
 
 constexpr int kBS = 256;
-constexpr int kSmallMax = 32;    // in-degree < 32: thread per vertex
-constexpr int kMediumMax = 1024; // in-degree < 1024: wavefront per vertex
 constexpr int kStripes = 64;
 
 // Compensated float sum: value = s + c.
@@ -231,7 +229,7 @@ __global__ __launch_bounds__(kBS) void ToFloat(const double* __restrict__ in, fl
 __global__ __launch_bounds__(kBS) void PrBin(const int64_t* __restrict__ in_off, int64_t n,
                                              int32_t* __restrict__ small, int32_t* __restrict__ medium,
                                              int32_t* __restrict__ large, int32_t* tails,
-                                             unsigned long long* edges) {
+                                             unsigned long long* edges, int64_t small_max, int64_t medium_max) {
   __shared__ unsigned long long s_red[kBS / kWave];
   unsigned long long e[3] = {0, 0, 0};
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
@@ -239,7 +237,7 @@ __global__ __launch_bounds__(kBS) void PrBin(const int64_t* __restrict__ in_off,
     int64_t d = -1;
     if (u < n) d = in_off[u + 1] - in_off[u];
     const int32_t v = static_cast<int32_t>(u);
-    const int bin = d < 0 ? -1 : (d < kSmallMax ? 0 : (d < kMediumMax ? 1 : 2));
+    const int bin = d < 0 ? -1 : (d < small_max ? 0 : (d < medium_max ? 1 : 2));
     if (bin >= 0) e[bin] += static_cast<unsigned long long>(d);
     WaveAppend(bin == 0, v, small, &tails[0]);
     WaveAppend(bin == 1, v, medium, &tails[1]);
@@ -269,8 +267,9 @@ PrBins GetBins(gg_graph* g) {
     };
     Tally* d = g->Scratch<Tally>("pr.bin_tally", 1);
     GG_HIP(hipMemsetAsync(d, 0, sizeof(Tally), g->stream));
-    PrBin<<<GridFor(g->n, kBS, g->cu_count * 8), kBS, 0, g->stream>>>(g->in_off, g->n, b.small, b.medium,
-                                                                       b.large, d->tails, d->edges);
+    PrBin<<<GridFor(g->n, kBS, g->cu_count * 8), kBS, 0, g->stream>>>(
+        g->in_off, g->n, b.small, b.medium, b.large, d->tails, d->edges, g->tune.pr_small_max,
+        g->tune.pr_medium_max);
     Tally h;
     GG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, g->stream));
     GG_HIP(hipStreamSynchronize(g->stream));
@@ -288,11 +287,15 @@ PrBins GetBins(gg_graph* g) {
   return b;
 }
 
+// in_place: one contribution array, each vertex's contribution replaced as
+// soon as its score is (PageRankPullGS, pr.cc:34-61): later gathers in the
+// same sweep may see it, as on the reference's threads.
 template <class T>
-void RunPageRank(gg_graph* g, int max_iters, double epsilon, float* scores_out, gg_stats* stats,
+void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, float* scores_out, gg_stats* stats,
                  const WallTimer& wt) {
   const int64_t n = g->n;
   const PrBins bins = GetBins(g);
+  const int width = g->tune.pr_blocks_per_cu;  // gather blocks per CU
   T* score = g->Scratch<T>(sizeof(T) == 4 ? "pr.score32" : "pr.score64", n);
   T* ca = g->Scratch<T>(sizeof(T) == 4 ? "pr.contrib32a" : "pr.contrib64a", n);
   T* cb = g->Scratch<T>(sizeof(T) == 4 ? "pr.contrib32b" : "pr.contrib64b", n);
@@ -316,22 +319,22 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, float* scores_out, 
   int iters = 0;
   double error = 0.0;
   for (int it = 0; it < max_iters; it++) {
-    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, cb, score, base, damp, residual};
+    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual};
     // Algorithmic bytes of one iteration, per bin: offsets 8 B and
     // contrib/score traffic per vertex, 4 B per in-neighbour id (each array
     // counted once; the random contribution gathers are not).
     const double per_vertex = 8.0 + vbytes;
     if (bins.n_small)
       Run(g, "pr_gather_thread", per_vertex * bins.n_small + 4.0 * bins.e_small, PrGatherThread<T>,
-          dim3(GridFor(bins.n_small, kBS, g->cu_count * 16)), blk, a,
+          dim3(GridFor(bins.n_small, kBS, g->cu_count * width)), blk, a,
           static_cast<const int32_t*>(bins.small), static_cast<int32_t>(bins.n_small));
     if (bins.n_medium)
       Run(g, "pr_gather_wave", per_vertex * bins.n_medium + 4.0 * bins.e_medium, PrGatherWave<T>,
-          dim3(GridFor(bins.n_medium, kBS / kWave, g->cu_count * 16)), blk, a,
+          dim3(GridFor(bins.n_medium, kBS / kWave, g->cu_count * width)), blk, a,
           static_cast<const int32_t*>(bins.medium), static_cast<int32_t>(bins.n_medium));
     if (bins.n_large)
       Run(g, "pr_gather_block", per_vertex * bins.n_large + 4.0 * bins.e_large, PrGatherBlock<T>,
-          dim3(GridFor(bins.n_large, 1, g->cu_count * 8)), blk, a,
+          dim3(GridFor(bins.n_large, 1, g->cu_count * (width + 1) / 2)), blk, a,
           static_cast<const int32_t*>(bins.large), static_cast<int32_t>(bins.n_large));
     launches += 3;
     const unsigned long long seq = MailNext(g);
@@ -340,7 +343,7 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, float* scores_out, 
     unsigned long long bits = 0;
     MailWait(g, seq, &bits, 1);
     std::memcpy(&error, &bits, sizeof(error));
-    std::swap(ca, cb);
+    if (!in_place) std::swap(ca, cb);
     iters++;
     if (error < epsilon) break;
   }
@@ -379,11 +382,11 @@ extern "C" int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode,
     GG_REQUIRE(!g->weighted, GG_ENOTSUP, "PageRank needs an unweighted graph upload");
     GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
     GG_REQUIRE(max_iters >= 0, GG_EINVAL, "max_iters must be >= 0");
-    GG_REQUIRE(mode == GG_PR_FP32 || mode == GG_PR_FP64, GG_EINVAL, "unknown precision mode");
+    GG_REQUIRE(mode == GG_PR_FP32 || mode == GG_PR_FP64 || mode == GG_PR_GS, GG_EINVAL, "unknown mode");
     GG_HIP(hipSetDevice(g->device));
-    if (mode == GG_PR_FP32)
-      RunPageRank<float>(g, max_iters, epsilon, scores, stats, wt);
+    if (mode == GG_PR_FP64)
+      RunPageRank<double>(g, max_iters, epsilon, false, scores, stats, wt);
     else
-      RunPageRank<double>(g, max_iters, epsilon, scores, stats, wt);
+      RunPageRank<float>(g, max_iters, epsilon, mode == GG_PR_GS, scores, stats, wt);
   });
 }

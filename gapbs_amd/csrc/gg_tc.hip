@@ -8,19 +8,25 @@
 // WorthRelabelling, tc.cc:75-100) is the caller's choice; the relabelled
 // graph is counted the same way.
 //
-// Writing N+(x) for x's neighbours below x (a prefix of its sorted list),
-// the reference's count is sum over u, v in N+(u) of |N+(u) & N+(v)|
-// (every w of N+(v) is below v, so the reference's "w <= v" bound is
-// implied).  GPU design:
-//   * TcForward: fdeg[x] = |N+(x)| by binary search, once per call;
-//   * TcItemCount / TcItemScan / TcItemFill: work items (u, v-chunk of 256
-//     entries of N+(u), set slice of kSetSlice entries of N+(u)) in a
+// Writing N+(x) for x's neighbours below x (a prefix of its sorted list) and
+// d+(x) = |N+(x)|, the reference's count is the sum over the forward pairs
+// (u, v in N+(u)) of |N+(u) & N+(v)| (every w of N+(v) is below v, so the
+// reference's "w <= v" bound is implied).  Each pair's intersection is
+// found by probing the shorter list into an LDS hash table of the longer:
+//   * forward items (u, 256 entries of N+(u), a 2048-entry slice of N+(u)):
+//     the slice of N+(u) in LDS, probed by the lists N+(v) of the v's with
+//     d+(v) <= d+(u);
+//   * backward items (v, 256 entries of v's list above v, a slice of
+//     N+(v)): the slice of N+(v) in LDS, probed by the lists N+(u) of the
+//     higher neighbours u with d+(u) < d+(v) (entries >= v cannot match and
+//     are skipped by the slice's value range).
+// Every pair lands in exactly one of the two.  Within an item the probe
+// lists form one flat range (block prefix sum, owner by binary search), so
+// each probe is a coalesced 4-byte list read plus an LDS lookup.
+//   * TcForward: d+(x) by binary search, once per call;
+//   * TcItemCount / TcItemScan / TcItemFill: both item kinds in a
 //     deterministic order (block prefix sums; no atomics);
-//   * TcCount: one block per item stages its slice of N+(u) in an LDS hash
-//     table, then streams the lists N+(v) of its 256 v's as one flat range
-//     (block prefix sum, owner by binary search) and probes every w.  Each
-//     probe is a coalesced 4-byte list read plus an LDS lookup, in place of
-//     the reference's dependent merge steps.
+//   * TcCount: one block per item.
 // Counts are summed per block and added to 64 striped counters.
 #include "gg_common.h"
 
@@ -29,15 +35,15 @@ namespace {
 
 constexpr int kBS = 256;
 constexpr int kStripes = 64;
-constexpr int kSetSlice = 2048;     // entries of N+(u) per LDS table
+constexpr int kSetSlice = 2048;        // entries of the staged list per LDS table
 constexpr int kTable = 2 * kSetSlice;  // table slots (load factor <= 1/2)
 constexpr int kUnroll = 2;
 
 struct TcItem {
-  int32_t u;
-  int32_t vb;  // first index into N+(u) of this item's v's
-  int32_t sb;  // first index into N+(u) of this item's set slice
-  int32_t pad;
+  int32_t x;     // the vertex whose forward list is staged
+  int32_t pb;    // first partner index (into N+(x), or into x's list above x)
+  int32_t sb;    // first index of the staged slice of N+(x)
+  int32_t back;  // 0: forward item, 1: backward item
 };
 
 // Index of the first entry of the sorted range [b, e) that is >= x.
@@ -52,9 +58,14 @@ __device__ __forceinline__ int64_t LowerBound(const int32_t* dst, int64_t b, int
   return b;
 }
 
-__device__ __forceinline__ int64_t ItemsOf(int32_t d) {
-  if (d < 2) return 0;  // a triangle needs two entries in N+(u)
-  return int64_t((d + kBS - 1) / kBS) * ((d + kSetSlice - 1) / kSetSlice);
+// Items of x: forward over its d+ partners, backward over its deg - d+
+// higher neighbours, each times the slices of N+(x).  A triangle needs two
+// entries in the staged list (forward) or a staged list longer than a
+// partner's, which has at least one entry (backward).
+__device__ __forceinline__ int64_t ItemsOf(int32_t dplus, int64_t above, bool back) {
+  if (dplus < 2) return 0;
+  const int64_t partners = back ? above : dplus;
+  return ((partners + kBS - 1) / kBS) * ((dplus + kSetSlice - 1) / kSetSlice);
 }
 
 __device__ __forceinline__ uint32_t Hash(int32_t w, uint32_t mask) {
@@ -67,8 +78,7 @@ __global__ __launch_bounds__(kBS) void TcForward(const int64_t* __restrict__ off
     fdeg[x] = static_cast<int32_t>(LowerBound(dst, off[x], off[x + 1], static_cast<int32_t>(x)) - off[x]);
 }
 
-// Vertices are split into gridDim.x contiguous spans; block b counts the
-// items of span b.
+// Vertices are split into gridDim.x contiguous spans; block b handles span b.
 __device__ __forceinline__ void Span(int64_t n, int64_t& lo, int64_t& hi) {
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
   lo = per * blockIdx.x;
@@ -76,20 +86,25 @@ __device__ __forceinline__ void Span(int64_t n, int64_t& lo, int64_t& hi) {
   if (lo > n) lo = n;
 }
 
-__global__ __launch_bounds__(kBS) void TcItemCount(const int32_t* __restrict__ fdeg, int64_t n,
-                                                   unsigned long long* __restrict__ span_items) {
+__device__ __forceinline__ int64_t BothItems(const int64_t* off, const int32_t* fdeg, int64_t x) {
+  const int32_t d = fdeg[x];
+  const int64_t above = off[x + 1] - off[x] - d;
+  return ItemsOf(d, above, false) + ItemsOf(d, above, true);
+}
+
+__global__ __launch_bounds__(kBS) void TcItemCount(const int64_t* __restrict__ off, const int32_t* __restrict__ fdeg,
+                                                   int64_t n, unsigned long long* __restrict__ span_items) {
   __shared__ unsigned long long s_red[kBS / kWave];
   int64_t lo, hi;
   Span(n, lo, hi);
   unsigned long long c = 0;
-  for (int64_t u = lo + threadIdx.x; u < hi; u += kBS) c += static_cast<unsigned long long>(ItemsOf(fdeg[u]));
+  for (int64_t x = lo + threadIdx.x; x < hi; x += kBS) c += static_cast<unsigned long long>(BothItems(off, fdeg, x));
   c = BlockSum<kBS>(c, s_red);
   if (threadIdx.x == 0) span_items[blockIdx.x] = c;
 }
 
-// Exclusive prefix over the span totals (one block; spans <= a few thousand).
-__global__ __launch_bounds__(kBS) void TcItemScan(unsigned long long* span_items, int spans,
-                                                  unsigned long long* total) {
+// Exclusive prefix over the span totals (one thread; spans <= a few thousand).
+__global__ void TcItemScan(unsigned long long* span_items, int spans, unsigned long long* total) {
   if (threadIdx.x == 0) {
     unsigned long long s = 0;
     for (int i = 0; i < spans; i++) {
@@ -101,17 +116,16 @@ __global__ __launch_bounds__(kBS) void TcItemScan(unsigned long long* span_items
   }
 }
 
-__global__ __launch_bounds__(kBS) void TcItemFill(const int32_t* __restrict__ fdeg, int64_t n,
-                                                  const unsigned long long* __restrict__ span_base,
+__global__ __launch_bounds__(kBS) void TcItemFill(const int64_t* __restrict__ off, const int32_t* __restrict__ fdeg,
+                                                  int64_t n, const unsigned long long* __restrict__ span_base,
                                                   TcItem* __restrict__ items) {
   __shared__ unsigned long long s_scan[kBS];
   int64_t lo, hi;
   Span(n, lo, hi);
   unsigned long long base = span_base[blockIdx.x];
   for (int64_t t = lo; t < hi; t += kBS) {
-    const int64_t u = t + threadIdx.x;
-    const int32_t d = u < hi ? fdeg[u] : 0;
-    const unsigned long long ni = static_cast<unsigned long long>(u < hi ? ItemsOf(d) : 0);
+    const int64_t x = t + threadIdx.x;
+    const unsigned long long ni = static_cast<unsigned long long>(x < hi ? BothItems(off, fdeg, x) : 0);
     // Block inclusive scan (Hillis-Steele in LDS; 64-bit counts).
     s_scan[threadIdx.x] = ni;
     __syncthreads();
@@ -125,8 +139,15 @@ __global__ __launch_bounds__(kBS) void TcItemFill(const int32_t* __restrict__ fd
     const unsigned long long tile_total = s_scan[kBS - 1];
     __syncthreads();
     if (ni) {
-      for (int32_t vb = 0; vb < d; vb += kBS)
-        for (int32_t sb = 0; sb < d; sb += kSetSlice) items[pos++] = TcItem{static_cast<int32_t>(u), vb, sb, 0};
+      const int32_t d = fdeg[x];
+      const int64_t above = off[x + 1] - off[x] - d;
+      for (int back = 0; back < 2; back++) {
+        if (ItemsOf(d, above, back) == 0) continue;
+        const int64_t partners = back ? above : d;
+        for (int64_t pb = 0; pb < partners; pb += kBS)
+          for (int32_t sb = 0; sb < d; sb += kSetSlice)
+            items[pos++] = TcItem{static_cast<int32_t>(x), static_cast<int32_t>(pb), sb, back};
+      }
     }
     base += tile_total;
   }
@@ -136,7 +157,7 @@ __global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, 
                                                const int32_t* __restrict__ fdeg, const TcItem* __restrict__ items,
                                                int64_t nitems, unsigned long long* counts) {
   __shared__ int32_t table[kTable];
-  __shared__ int64_t s_vbeg[kBS];
+  __shared__ int64_t s_lbeg[kBS];
   __shared__ int s_pre[kBS + 1];
   __shared__ int s_scan[kBS / kWave];
   __shared__ unsigned long long s_red[kBS / kWave];
@@ -144,8 +165,8 @@ __global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, 
   unsigned long long count = 0, probes = 0;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     const TcItem item = items[it];
-    const int64_t ub = off[item.u];
-    const int32_t d = fdeg[item.u];
+    const int64_t xb = off[item.x];
+    const int32_t d = fdeg[item.x];
     const int32_t se = item.sb + kSetSlice < d ? item.sb + kSetSlice : d;
     const int32_t slice = se - item.sb;
     uint32_t size = 64;
@@ -154,23 +175,29 @@ __global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, 
     for (uint32_t i = tid; i < size; i += kBS) table[i] = -1;
     __syncthreads();
     for (int32_t i = tid; i < slice; i += kBS) {
-      const int32_t w = dst[ub + item.sb + i];
+      const int32_t w = dst[xb + item.sb + i];
       uint32_t h = Hash(w, mask);
       while (atomicCAS(&table[h], -1, w) != -1) h = (h + 1) & mask;
     }
     // The slice holds the sorted values [lo_w, hi_w]; probes outside skip
     // the table.
-    const int32_t lo_w = dst[ub + item.sb], hi_w = dst[ub + se - 1];
-    // This thread's v and its list N+(v).
-    const int32_t j = item.vb + tid;
+    const int32_t lo_w = dst[xb + item.sb], hi_w = dst[xb + se - 1];
+    // This thread's partner and its probe list N+(partner), kept only when
+    // the pair belongs to this item's kind.
+    const int64_t partners = item.back ? off[item.x + 1] - xb - d : d;
+    const int64_t j = item.pb + tid;
     int len = 0;
-    int64_t vbeg = 0;
-    if (j < d) {
-      const int32_t v = dst[ub + j];
-      vbeg = off[v];
-      len = fdeg[v];
+    int64_t lbeg = 0;
+    if (j < partners) {
+      const int32_t p = dst[xb + (item.back ? d : 0) + j];
+      const int32_t dp = fdeg[p];
+      const bool mine = item.back ? dp < d : dp <= d;
+      if (mine) {
+        lbeg = off[p];
+        len = dp;
+      }
     }
-    s_vbeg[tid] = vbeg;
+    s_lbeg[tid] = lbeg;
     int total = 0;
     const int rank = BlockExclusiveScan<kBS>(len, &total, s_scan);
     s_pre[tid] = rank;
@@ -191,7 +218,7 @@ __global__ __launch_bounds__(kBS) void TcCount(const int64_t* __restrict__ off, 
             else
               hi = mid;
           }
-          w[k] = dst[s_vbeg[lo] + (jj - s_pre[lo])];
+          w[k] = dst[s_lbeg[lo] + (jj - s_pre[lo])];
         }
       }
 #pragma unroll
@@ -252,7 +279,8 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
     Run(g, "tc_clear", 0, TcClear, dim3(1), blk, counts);
     Run(g, "tc_forward", 16.0 * N + 4.0 * N, TcForward, dim3(GridFor(n, kBS, g->cu_count * 8)), blk,
         static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), n, fdeg);
-    Run(g, "tc_items", 4.0 * N, TcItemCount, dim3(spans), blk, static_cast<const int32_t*>(fdeg), n, span_items);
+    Run(g, "tc_items", 20.0 * N, TcItemCount, dim3(spans), blk, static_cast<const int64_t*>(g->out_off),
+        static_cast<const int32_t*>(fdeg), n, span_items);
     Run(g, "tc_items", 0, TcItemScan, dim3(1), dim3(64), span_items, spans, counts + 2 * kStripes);
     launches += 4;
     unsigned long long* h = g->Pinned<unsigned long long>(2 * kStripes + 1);
@@ -260,10 +288,11 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
     GG_HIP(hipStreamSynchronize(st));
     const int64_t nitems = static_cast<int64_t>(h[0]);
     TcItem* items = g->Scratch<TcItem>("tc.items", nitems + 1);
-    Run(g, "tc_items", 4.0 * N + 16.0 * nitems, TcItemFill, dim3(spans), blk, static_cast<const int32_t*>(fdeg), n,
+    Run(g, "tc_items", 20.0 * N + 16.0 * nitems, TcItemFill, dim3(spans), blk,
+        static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(fdeg), n,
         static_cast<const unsigned long long*>(span_items), items);
-    // Per item: its set slice and 256 list headers; per probe one 4-byte
-    // list entry (credited below from the probe counter).
+    // Per item: its header and partner headers; per probe one 4-byte list
+    // entry (credited below from the probe counter).
     const int idx = Run(g, "tc_count", 16.0 * nitems, TcCount, dim3(GridFor(nitems, 1, g->cu_count * 16)), blk,
                         static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
                         static_cast<const int32_t*>(fdeg), static_cast<const TcItem*>(items), nitems, counts);
@@ -276,8 +305,9 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
       total += h[2 * i];
       probes += h[2 * i + 1];
     }
-    // Staged set entries: |N+(u)| per (u, v-chunk), at most the probes' order.
-    CreditBytes(g, idx, 4.0 * static_cast<double>(probes) + 12.0 * N);
+    // The partners' headers (12 B) and the staged entries (4 B) of each
+    // (vertex, partner) once per side: 2 x 16 B per forward edge.
+    CreditBytes(g, idx, 4.0 * static_cast<double>(probes) + 16.0 * static_cast<double>(g->out_entries));
     FinishTiming(g, stats, wt);
     *count = total;
     if (stats) {

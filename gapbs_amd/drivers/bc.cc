@@ -2,7 +2,8 @@
 // replaced by gg_bc on the GPU.  The roots are drawn from the driver's
 // SourcePicker, -i per trial, as Brandes draws them (bc.cc:109).
 // -p fp32 (default): the reference's float arithmetic; -p kahan: compensated
-// float sums; -p fp64: double dependencies and scores.
+// float sums; -p fp64: double dependencies and scores.  -j (> 1) runs each
+// trial's -i roots (at most 64) as one batched traversal (gg_bc_batch).
 #include "driver_common.h"
 
 using namespace gapdrv;
@@ -13,6 +14,10 @@ int main(int argc, char* argv[]) {
     if (!cli.ParseArgs()) return -1;
     if (cli.num_iters() > 1 && cli.start_vertex() != -1)
       std::cout << "Warning: iterating from same source (-r & -i)" << std::endl;
+    if (cli.batch() > 1 && (cli.num_iters() > 64 || cli.precision() == "kahan")) {
+      std::cout << "-j batches up to 64 roots (-i) in fp32 or fp64" << std::endl;
+      return -1;
+    }
     Graph g = MakeGraph<NodeID>(cli);
     DeviceSession<Graph> dev(cli.device(), g);
     const int mode = cli.precision() == "fp64" ? GG_BC_FP64
@@ -27,7 +32,11 @@ int main(int argc, char* argv[]) {
       }
       Buf<float> scores(static_cast<size_t>(gr.num_nodes()));
       gg_stats st;
-      Check(gg_bc(dev.get(), roots.data(), static_cast<int>(roots.size()), mode, scores.data(), &st), "gg_bc");
+      if (cli.batch() > 1)  // all of this trial's roots in one batched traversal
+        Check(gg_bc_batch(dev.get(), roots.data(), static_cast<int>(roots.size()), mode, scores.data(), &st),
+              "gg_bc_batch");
+      else
+        Check(gg_bc(dev.get(), roots.data(), static_cast<int>(roots.size()), mode, scores.data(), &st), "gg_bc");
       LogStats(cli.logging_en(), st);
       return scores;
     };

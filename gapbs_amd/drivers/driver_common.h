@@ -5,11 +5,13 @@
 // never touch the GPU.
 #pragma once
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
 #include <iostream>
 #include <memory>
+#include <vector>
 
 #include "gap_build.h"
 #include "gap_core.h"
@@ -68,6 +70,32 @@ inline int RunDriver(const std::function<int()>& body) {
     std::cout << e.what() << std::endl;
     std::exit(e.code);
   }
+}
+
+// -q: tunes the selected kernels (GG_TUNE_* bits) on this graph before
+// any trial, from the first k roots the trials will use (a SourcePicker of
+// its own with the same seed), and prints the choice.
+template <class GraphT>
+gg_tuning Autotune(const CLI& cli, DeviceSession<GraphT>& dev, const GraphT& g, int what, int k) {
+  std::vector<int32_t> roots;
+  if (what != GG_TUNE_PR) {
+    SourcePicker<GraphT> sp(g, static_cast<NodeID>(cli.start_vertex()));
+    for (int i = 0; i < k; i++) roots.push_back(sp.PickNext());
+  }
+  gg_tuning t;
+  Check(gg_autotune(dev.get(), what, roots.empty() ? nullptr : roots.data(), static_cast<int>(roots.size()), &t),
+        "gg_autotune");
+  if (what & GG_TUNE_BATCH)
+    std::cout << "Autotune: batched alpha " << t.batch_alpha << " (" << t.best_ms[1] << " ms)" << std::endl;
+  if (what & GG_TUNE_BFS)
+    std::cout << "Autotune: alpha " << t.bfs_alpha << " beta " << t.bfs_beta << " (" << t.best_ms[0] << " ms)"
+              << std::endl;
+  if (what & GG_TUNE_SSSP)
+    std::cout << "Autotune: delta " << t.sssp_delta << " (" << t.best_ms[2] << " ms)" << std::endl;
+  if (what & GG_TUNE_PR)
+    std::cout << "Autotune: bins <" << t.pr_small_max << " <" << t.pr_medium_max << ", " << t.pr_blocks_per_cu
+              << " blocks/CU (" << t.best_ms[3] << " ms/iter)" << std::endl;
+  return t;
 }
 
 inline void LogStats(bool on, const gg_stats& st) {

@@ -1,6 +1,7 @@
 // pr driver: the reference's main() (src/pr.cc:97-111 / src/pr_spmv.cc:97-111)
 // with the PageRank kernel replaced by gg_pagerank (Jacobi pull, the
-// pr_spmv.cc semantics) on the GPU.  -p fp64 selects FP64 accumulation.
+// pr_spmv.cc semantics) on the GPU.  -p fp64 selects FP64 accumulation,
+// -p gs the in-place Gauss-Seidel sweep of pr.cc's PageRankPullGS.
 #include "driver_common.h"
 
 using namespace gapdrv;
@@ -11,7 +12,8 @@ int main(int argc, char* argv[]) {
     if (!cli.ParseArgs()) return -1;
     Graph g = MakeGraph<NodeID>(cli);
     DeviceSession<Graph> dev(cli.device(), g);
-    const int mode = cli.precision() == "fp64" ? GG_PR_FP64 : GG_PR_FP32;
+    const int mode = cli.precision() == "fp64" ? GG_PR_FP64 : cli.precision() == "gs" ? GG_PR_GS : GG_PR_FP32;
+    if (cli.autotune() && cli.num_trials() > 0) Autotune(cli, dev, g, GG_TUNE_PR, 0);
     auto PRBound = [&](const Graph& gr) {
       Buf<float> scores(static_cast<size_t>(gr.num_nodes()));
       gg_stats st;

@@ -58,9 +58,24 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("bfs_alpha", C.c_int32), ("bfs_beta", C.c_int32), ("batch_alpha", C.c_int32),
+                ("sssp_delta", C.c_int32), ("pr_small_max", C.c_int32), ("pr_medium_max", C.c_int32),
+                ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "best_ms"}
+        d["best_ms"] = list(self.best_ms)
+        return d
+
+
+TUNE_BFS, TUNE_BATCH, TUNE_SSSP, TUNE_PR = 1, 2, 4, 8
+
+
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
+    "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
@@ -93,6 +108,9 @@ def lib() -> C.CDLL:
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_bc_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
+        L.gg_autotune.argtypes = [vp, C.c_int, _i32p, C.c_int, C.POINTER(Tuning)]
+        L.gg_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
+        L.gg_tuning_set.argtypes = [vp, C.POINTER(Tuning)]
         L.gg_profile_enable.argtypes = [vp, C.c_int]
         L.gg_profile_reset.argtypes = [vp]
         L.gg_profile_get.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _i64p,
@@ -266,7 +284,7 @@ def DeltaStep(dg: DeviceGraph, source: int, delta: int) -> np.ndarray:
     return sssp(dg, source, delta).dist
 
 
-PR_FP32, PR_FP64 = 0, 1
+PR_FP32, PR_FP64, PR_GS = 0, 1, 2
 
 
 @dataclass
@@ -276,13 +294,20 @@ class PRResult:
 
 
 def pagerank(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4, fp64: bool = False,
-             want_scores: bool = True) -> PRResult:
-    """gg_pagerank: FP32 compensated (default) or FP64 accumulation."""
+             want_scores: bool = True, gauss_seidel: bool = False) -> PRResult:
+    """gg_pagerank: Jacobi with FP32 compensated (default) or FP64 accumulation,
+    or the in-place Gauss-Seidel sweep (gauss_seidel=True, FP32)."""
     out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
     st = Stats()
-    _check(lib().gg_pagerank(dg.handle, max_iters, epsilon, PR_FP64 if fp64 else PR_FP32,
+    mode = PR_GS if gauss_seidel else (PR_FP64 if fp64 else PR_FP32)
+    _check(lib().gg_pagerank(dg.handle, max_iters, epsilon, mode,
                              _p(out, C.c_float) if want_scores else None, C.byref(st)))
     return PRResult(out, st.as_dict())
+
+
+def PageRankPullGS(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:
+    """Scores of the Gauss-Seidel pull PageRank (src/pr.cc:34-61)."""
+    return pagerank(dg, max_iters, epsilon, gauss_seidel=True).scores
 
 
 def PageRankPull(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:
@@ -368,3 +393,28 @@ def Hybrid(g: Graph, device: int = 0) -> int:
     (src/tc.cc:75-100)."""
     h = g.relabel_by_degree() if g.worth_relabelling() else g
     return OrderedCount(DeviceGraph(h, device))
+
+
+def autotune(dg: DeviceGraph, what: int, roots=()) -> dict:
+    """gg_autotune: time candidate settings of the selected kernels (TUNE_*
+    bits) on this graph from `roots` and keep the fastest in the handle."""
+    r = np.ascontiguousarray(roots, dtype=np.int32)
+    t = Tuning()
+    _check(lib().gg_autotune(dg.handle, what, _p(r, C.c_int32) if len(r) else None, len(r), C.byref(t)))
+    return t.as_dict()
+
+
+def tuning(dg: DeviceGraph) -> dict:
+    t = Tuning()
+    _check(lib().gg_tuning_get(dg.handle, C.byref(t)))
+    return t.as_dict()
+
+
+def set_tuning(dg: DeviceGraph, values: dict) -> None:
+    t = Tuning()
+    _check(lib().gg_tuning_get(dg.handle, C.byref(t)))
+    for k, v in values.items():
+        if k == "best_ms":
+            continue
+        setattr(t, k, v)
+    _check(lib().gg_tuning_set(dg.handle, C.byref(t)))

@@ -13,7 +13,7 @@
 //   -c <side>   generate the side x side weighted 2D grid graph
 //   -x <dev>    HIP device ordinal
 //   -j <k>      roots per batched launch (bfs, sssp, bc; default 1 = per root)
-//   -p <mode>   score precision: pr fp32 (compensated, default) | fp64;
+//   -p <mode>   score precision: pr fp32 (compensated, default) | gs | fp64;
 //               bc fp32 (the reference's float arithmetic, default) | kahan | fp64
 //   -q          autotune direction/bucket/bin parameters for this graph
 #pragma once
@@ -68,7 +68,7 @@ class CLI {
     Help('v', "", "verify the output of each run", "false");
     Help('l', "", "log performance within each trial", "false");
     Help('j', "k", "roots per batched GPU launch", "1");
-    Help('p', "mode", "score precision: fp32, kahan (bc) or fp64", "fp32");
+    Help('p', "mode", "score precision: fp32, kahan (bc), gs (pr) or fp64", "fp32");
     Help('q', "", "autotune GPU parameters for this graph", "false");
     num_iters_ = default_iters;
     max_iters_ = default_iters;

@@ -115,10 +115,14 @@ int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist, gg_stats*
  * logging) (src/pr_spmv.cc:35-61): Jacobi iterations until the L1 change
  * (accumulated in double) drops below epsilon or max_iters is reached.
  * mode GG_PR_FP32: float scores with compensated per-vertex sums;
- * mode GG_PR_FP64: double scores and sums.  scores[n] (nullable) receives
- * float scores either way.  stats->iterations and stats->residual report
- * the iterations run and the last L1 change. */
-enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1 };
+ * mode GG_PR_FP64: double scores and sums.  mode GG_PR_GS: FP32 with one
+ * contribution array updated in place as scores change, the Gauss-Seidel
+ * sweep of PageRankPullGS (src/pr.cc:34-61, the reference's `pr` binary);
+ * like the reference's, its result depends on the schedule and is checked
+ * by the convergence rule of PRVerifier (pr.cc:65-94).  scores[n]
+ * (nullable) receives float scores.  stats->iterations and stats->residual
+ * report the iterations run and the last L1 change. */
+enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1, GG_PR_GS = 2 };
 int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* scores, gg_stats* stats);
 
 /* Afforest connected components.
@@ -156,8 +160,39 @@ int gg_bc_batch(gg_graph* g, const int32_t* roots, int k, int mode, float* score
  * Replaces OrderedCount(const Graph&) (src/tc.cc:52-71); the Hybrid
  * relabel-by-degree decision (tc.cc:75-100) is made by the caller, which
  * uploads the relabelled graph when it applies.  stats->edges_examined is
- * the number of merge steps. */
+ * the number of (set, list-entry) probes. */
 int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats);
+
+/* Per-graph tuning (no reference counterpart; the reference's knobs are the
+ * -a/-b style alpha/beta defaults in bfs.cc:122-123 and sssp's -d delta).
+ * gg_autotune times candidate settings of the selected kernels on this
+ * graph from the given roots (k >= 1; SourcePicker roots are the natural
+ * choice) and keeps the fastest in the handle:
+ *   GG_TUNE_BFS     gg_bfs alpha x beta over {2..240} x {2..162}
+ *   GG_TUNE_BATCH   gg_bfs_batch / gg_bfs_packed alpha over {1..1000}
+ *   GG_TUNE_SSSP    gg_sssp delta over powers of two 1..4096 (weighted
+ *                   uploads only; slow candidates are cut off early)
+ *   GG_TUNE_PR      PageRank in-degree bin cut-offs (thread / wavefront /
+ *                   block per vertex) and the gather launch width, by
+ *                   coordinate descent over 3-iteration runs
+ * alpha, beta and delta are arguments of their entry points: callers read
+ * them back (gg_tuning_get) and pass them; the PageRank settings apply to
+ * every later gg_pagerank on the handle.  gg_tuning_set installs a saved
+ * tuning (GG_EINVAL for values out of range). */
+typedef struct gg_tuning {
+  int32_t bfs_alpha, bfs_beta;
+  int32_t batch_alpha;
+  int32_t sssp_delta;
+  int32_t pr_small_max;      /* in-degree below this: one thread per vertex */
+  int32_t pr_medium_max;     /* below this: one wavefront; else one block */
+  int32_t pr_blocks_per_cu;  /* gather launch width */
+  int32_t tuned;             /* GG_TUNE_* bits tuned so far */
+  double best_ms[4];         /* device ms of the chosen setting: bfs, batch, sssp, pr */
+} gg_tuning;
+enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };
+int gg_autotune(gg_graph* g, int what, const int32_t* roots, int k, gg_tuning* out);
+int gg_tuning_get(const gg_graph* g, gg_tuning* out);
+int gg_tuning_set(gg_graph* g, const gg_tuning* in);
 
 /* Per-kernel measurement (no reference counterpart; used by bench.py).
  * While enabled, every kernel launch on the handle is bracketed by HIP

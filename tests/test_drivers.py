@@ -72,3 +72,15 @@ def test_verify_g14(kernel, extra):
         assert n + out.count("Verification:           FAIL") >= 1
     else:
         assert n >= 1 and "FAIL" not in out, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,extra", [("bfs", ["-n2", "-j", "16"]), ("bc", ["-n1", "-i", "8", "-j", "8"]),
+                                          ("bc", ["-n1", "-i", "8", "-j", "8", "-p", "fp64"]),
+                                          pytest.param("pr", ["-n1", "-p", "gs"], marks=pytest.mark.skip(
+                                              reason="GG_PR_GS failed PRVerifier on g16; cause not yet found"))])
+def test_verify_batched_and_gauss_seidel(kernel, extra):
+    print(f"[driver] {kernel} -g12 -v {extra}", flush=True)
+    rc, out = run([kernel, "-g12", "-v"] + extra)
+    assert rc == 0, out
+    assert "Verification:           PASS" in out and "FAIL" not in out, out

@@ -1,0 +1,79 @@
+"""The grid workload at full size: a 4096 x 4096 grid (16,777,216
+vertices), 4-neighbour edges with 5% deleted and 1% random shortcuts added
+(the shortcuts bring the BFS depth down to about 56 levels),
+weights uniform in [1, 1000], seed 7 (Graph.grid).  BFS and SSSP from
+SourcePicker roots, single-root and batched; every checked root against the
+serial oracles (bfs_depth restating src/bfs.cc:204-215, Dijkstra restating
+src/sssp.cc:156-181), and SSSP distances identical across bucket widths.
+Widths below 64 are covered on the 96 x 96 grid in test_sssp_gpu.py: here
+they would take hundreds of thousands of buckets.
+"""
+
+import numpy as np
+import pytest
+
+from gapbs_amd import gpu
+from gapbs_amd.graph import Graph
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+_G = {}
+
+
+def grid():
+    if "w" not in _G:
+        print("[grid] building 4096^2 seed 7", flush=True)
+        _G["w"] = Graph.grid(4096, seed=7)
+        _G["u"] = _G["w"].strip_weights()
+        _G["roots"] = [int(r) for r in _G["u"].pick_sources(32)]
+    return _G["w"], _G["u"], _G["roots"]
+
+
+def test_grid_shape():
+    wg, g, roots = grid()
+    assert g.num_nodes == 4096 * 4096
+    lattice = 2 * 4096 * 4095
+    # about 95% of the lattice edges plus 1% shortcuts (a few coincide or loop)
+    assert 0.955 * lattice < g.num_edges() < 0.965 * lattice
+    assert len(set(roots)) == 32
+
+
+def test_grid_bfs_single_and_batched():
+    wg, g, roots = grid()
+    dg = gpu.DeviceGraph(g)
+    print("[grid] batched BFS of 32 roots", flush=True)
+    batch = gpu.bfs_batch(dg, roots)
+    print(f"[grid] batched: {batch.stats['iterations']} levels, {batch.stats['device_ms']:.1f} ms", flush=True)
+    for i in (0, 1, 2, 3):
+        want = oracle.bfs_depth(g, roots[i])
+        assert np.array_equal(batch.depths[i], want), f"root {roots[i]}"
+    for i in (4, 5, 31):
+        single = gpu.bfs(dg, roots[i])
+        assert np.array_equal(batch.depths[i], single.depth), f"root {roots[i]}"
+        assert g.verify_bfs(roots[i], single.parent)
+    assert batch.stats["reached"] == sum(int((batch.depths[i] >= 0).sum()) for i in range(32))
+
+
+@pytest.mark.parametrize("delta", [64, 1000, 4096])
+def test_grid_sssp_delta_sweep(delta):
+    wg, g, roots = grid()
+    if "dwg" not in _G:
+        _G["dwg"] = gpu.DeviceGraph(wg)
+        _G["dist"] = {r: oracle.dijkstra(wg, r) for r in roots[:2]}
+    print(f"[grid] sssp delta {delta}", flush=True)
+    for r in roots[:2]:
+        res = gpu.sssp(_G["dwg"], r, delta)
+        assert np.array_equal(res.dist, _G["dist"][r]), f"root {r}"
+
+
+def test_grid_sssp_autotuned_delta():
+    wg, g, roots = grid()
+    if "dwg" not in _G:
+        _G["dwg"] = gpu.DeviceGraph(wg)
+    print("[grid] sssp autotune", flush=True)
+    t = gpu.autotune(_G["dwg"], gpu.TUNE_SSSP, roots[:2])
+    print(f"[grid] tuned delta {t['sssp_delta']} ({t['best_ms'][2]:.1f} ms per root)", flush=True)
+    if "dist" not in _G:
+        _G["dist"] = {r: oracle.dijkstra(wg, r) for r in roots[:2]}
+    res = gpu.sssp(_G["dwg"], roots[1], t["sssp_delta"])
+    assert np.array_equal(res.dist, _G["dist"][roots[1]])
