@@ -136,6 +136,10 @@ struct gg_graph {
 
 namespace gg {
 
+// A new undirected, unweighted handle on `device` with uninitialised device
+// arrays for n vertices and `entries` destinations (filled by the caller).
+gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries);
+
 // Host wall clock for gg_stats::total_ms.
 struct WallTimer {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();

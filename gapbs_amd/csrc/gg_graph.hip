@@ -216,6 +216,34 @@ void ValidateSide(const int64_t* off, const int32_t* dst, int64_t entries, int w
 
 }  // namespace
 
+gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries) {
+  GG_REQUIRE(n >= 0 && n < (int64_t(1) << 31), GG_EINVAL, "num_nodes out of the int32 id range");
+  gg_graph* g = new gg_graph;
+  try {
+    g->device = device;
+    GG_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    GG_HIP(hipGetDeviceProperties(&prop, device));
+    g->cu_count = prop.multiProcessorCount;
+    GG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    GG_HIP(hipEventCreate(&g->ev_start));
+    GG_HIP(hipEventCreate(&g->ev_stop));
+    g->n = n;
+    g->out_entries = g->in_entries = entries;
+    const size_t off_bytes = sizeof(int64_t) * (n + 1), dst_bytes = sizeof(int32_t) * entries;
+    g->out_off = static_cast<int64_t*>(DeviceAlloc(off_bytes));
+    g->out_dst = static_cast<int32_t*>(DeviceAlloc(dst_bytes));
+    g->in_off = g->out_off;
+    g->in_dst = g->out_dst;
+    g->graph_bytes = off_bytes + dst_bytes;
+    MailInit(g);
+  } catch (...) {
+    gg_graph_free(g);
+    throw;
+  }
+  return g;
+}
+
 }  // namespace gg
 
 using namespace gg;
@@ -299,6 +327,18 @@ int gg_graph_upload(int64_t num_nodes, const int64_t* out_offsets, const int32_t
       throw;
     }
     *out = g;
+  });
+}
+
+int gg_graph_copy_out(const gg_graph* g, int64_t* offsets, int32_t* dests) {
+  return Guard([&] {
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_HIP(hipSetDevice(g->device));
+    GG_HIP(hipStreamSynchronize(g->stream));
+    const int width = g->weighted ? 2 : 1;
+    if (offsets) GG_HIP(hipMemcpy(offsets, g->out_off, sizeof(int64_t) * (g->n + 1), hipMemcpyDeviceToHost));
+    if (dests && g->out_entries)
+      GG_HIP(hipMemcpy(dests, g->out_dst, sizeof(int32_t) * width * g->out_entries, hipMemcpyDeviceToHost));
   });
 }
 

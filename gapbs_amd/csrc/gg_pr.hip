@@ -26,6 +26,7 @@ namespace {
 
 constexpr int kBS = 256;
 constexpr int kStripes = 64;
+constexpr int kGsPasses = 4;  // Gauss-Seidel sweep: vertex-id ranges updated in turn
 
 // Compensated float sum: value = s + c.
 struct CompSum {
@@ -107,6 +108,7 @@ struct PrArgs {
   T* score;
   T base, damp;
   double* residual;  // striped
+  int32_t lo, hi;    // vertex ids updated by this launch: [lo, hi)
 };
 
 // Finishes vertex u: new score, next contribution, |change|.
@@ -125,13 +127,18 @@ __device__ __forceinline__ void AddResidual(double* residual, double r, double* 
   if (threadIdx.x == 0 && r != 0.0) atomicAdd(&residual[blockIdx.x & (kStripes - 1)], r);
 }
 
+// The three gathers as parts of one launch: blocks [0, nb_large) serve the
+// large bin (one block per vertex), the next nb_medium blocks the medium
+// bin (one wavefront per vertex), the rest the small bin (one thread per
+// vertex); `blk` and `nblk` are the block's index and count within its part.
 template <class T>
-__global__ __launch_bounds__(kBS) void PrGatherThread(PrArgs<T> a, const int32_t* __restrict__ verts, int32_t nv) {
-  __shared__ double s_red[kBS / kWave];
+__device__ __forceinline__ double GatherThreadPart(const PrArgs<T>& a, const int32_t* __restrict__ verts,
+                                                  int32_t nv, int64_t blk, int64_t nblk) {
   using A = typename Acc<T>::Type;
   double r = 0.0;
-  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < nv; i += int64_t(gridDim.x) * kBS) {
+  for (int64_t i = blk * kBS + threadIdx.x; i < nv; i += nblk * kBS) {
     const int32_t u = verts[i];
+    if (u < a.lo || u >= a.hi) continue;
     const int64_t b = a.in_off[u], e = a.in_off[u + 1];
     A acc = A::Zero();
     int64_t k = b;
@@ -146,37 +153,39 @@ __global__ __launch_bounds__(kBS) void PrGatherThread(PrArgs<T> a, const int32_t
     for (; k < e; k++) acc.Add(a.contrib_in[a.in_dst[k]]);
     r += Finish(a, u, static_cast<T>(acc.Value()));
   }
-  AddResidual(a.residual, r, s_red);
+  return r;
 }
 
 template <class T>
-__global__ __launch_bounds__(kBS) void PrGatherWave(PrArgs<T> a, const int32_t* __restrict__ verts, int32_t nv) {
-  __shared__ double s_red[kBS / kWave];
+__device__ __forceinline__ double GatherWavePart(const PrArgs<T>& a, const int32_t* __restrict__ verts, int32_t nv,
+                                                int64_t blk, int64_t nblk) {
   using A = typename Acc<T>::Type;
   const int lane = LaneId();
-  const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
-  const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
+  const int64_t wave = (blk * kBS + threadIdx.x) / kWave;
+  const int64_t nwaves = nblk * (kBS / kWave);
   double r = 0.0;
   for (int64_t i = wave; i < nv; i += nwaves) {
     const int32_t u = verts[i];
+    if (u < a.lo || u >= a.hi) continue;  // uniform per wavefront
     const int64_t b = a.in_off[u], e = a.in_off[u + 1];
     A acc = A::Zero();
     for (int64_t k = b + lane; k < e; k += kWave) acc.Add(a.contrib_in[a.in_dst[k]]);
     acc = WaveMerge(acc);
     if (lane == 0) r += Finish(a, u, static_cast<T>(acc.Value()));
   }
-  AddResidual(a.residual, r, s_red);
+  return r;
 }
 
 template <class T>
-__global__ __launch_bounds__(kBS) void PrGatherBlock(PrArgs<T> a, const int32_t* __restrict__ verts, int32_t nv) {
-  __shared__ double s_red[kBS / kWave];
+__device__ __forceinline__ double GatherBlockPart(const PrArgs<T>& a, const int32_t* __restrict__ verts, int32_t nv,
+                                                 int64_t blk, int64_t nblk,
+                                                 typename Acc<T>::Type* s_acc) {
   using A = typename Acc<T>::Type;
-  __shared__ A s_acc[kBS / kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
   double r = 0.0;
-  for (int64_t i = blockIdx.x; i < nv; i += gridDim.x) {
+  for (int64_t i = blk; i < nv; i += nblk) {
     const int32_t u = verts[i];
+    if (u < a.lo || u >= a.hi) continue;  // uniform per block
     const int64_t b = a.in_off[u], e = a.in_off[u + 1];
     A acc = A::Zero();
     for (int64_t k = b + threadIdx.x; k < e; k += kBS) acc.Add(a.contrib_in[a.in_dst[k]]);
@@ -190,6 +199,24 @@ __global__ __launch_bounds__(kBS) void PrGatherBlock(PrArgs<T> a, const int32_t*
     }
     __syncthreads();
   }
+  return r;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBS) void PrGather(PrArgs<T> a, const int32_t* __restrict__ small, int32_t n_small,
+                                                const int32_t* __restrict__ medium, int32_t n_medium,
+                                                const int32_t* __restrict__ large, int32_t n_large,
+                                                int32_t nb_large, int32_t nb_medium) {
+  __shared__ double s_red[kBS / kWave];
+  __shared__ typename Acc<T>::Type s_acc[kBS / kWave];
+  const int64_t b = blockIdx.x;
+  double r;
+  if (b < nb_large)
+    r = GatherBlockPart(a, large, n_large, b, nb_large, s_acc);
+  else if (b < nb_large + nb_medium)
+    r = GatherWavePart(a, medium, n_medium, b - nb_large, nb_medium);
+  else
+    r = GatherThreadPart(a, small, n_small, b - nb_large - nb_medium, int64_t(gridDim.x) - nb_large - nb_medium);
   AddResidual(a.residual, r, s_red);
 }
 
@@ -288,8 +315,12 @@ PrBins GetBins(gg_graph* g) {
 }
 
 // in_place: one contribution array, each vertex's contribution replaced as
-// soon as its score is (PageRankPullGS, pr.cc:34-61): later gathers in the
-// same sweep may see it, as on the reference's threads.
+// soon as its score is (PageRankPullGS, pr.cc:34-61), and each sweep runs as
+// kGsPasses launches over consecutive vertex-id ranges, so later ranges
+// read the new contributions of earlier ones -- the reference's threads
+// sweep ids in order within their 16384-vertex chunks.  (Ordering the
+// passes by degree bin instead converges more slowly than Jacobi on
+// Kronecker graphs.)
 template <class T>
 void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, float* scores_out, gg_stats* stats,
                  const WallTimer& wt) {
@@ -319,24 +350,30 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
   int iters = 0;
   double error = 0.0;
   for (int it = 0; it < max_iters; it++) {
-    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual};
-    // Algorithmic bytes of one iteration, per bin: offsets 8 B and
-    // contrib/score traffic per vertex, 4 B per in-neighbour id (each array
-    // counted once; the random contribution gathers are not).
-    const double per_vertex = 8.0 + vbytes;
-    if (bins.n_small)
-      Run(g, "pr_gather_thread", per_vertex * bins.n_small + 4.0 * bins.e_small, PrGatherThread<T>,
-          dim3(GridFor(bins.n_small, kBS, g->cu_count * width)), blk, a,
-          static_cast<const int32_t*>(bins.small), static_cast<int32_t>(bins.n_small));
-    if (bins.n_medium)
-      Run(g, "pr_gather_wave", per_vertex * bins.n_medium + 4.0 * bins.e_medium, PrGatherWave<T>,
-          dim3(GridFor(bins.n_medium, kBS / kWave, g->cu_count * width)), blk, a,
-          static_cast<const int32_t*>(bins.medium), static_cast<int32_t>(bins.n_medium));
-    if (bins.n_large)
-      Run(g, "pr_gather_block", per_vertex * bins.n_large + 4.0 * bins.e_large, PrGatherBlock<T>,
-          dim3(GridFor(bins.n_large, 1, g->cu_count * (width + 1) / 2)), blk, a,
-          static_cast<const int32_t*>(bins.large), static_cast<int32_t>(bins.n_large));
-    launches += 3;
+    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual, 0, 0};
+    const int passes = in_place ? kGsPasses : 1;
+    // Algorithmic bytes of one iteration: offsets 8 B and contrib/score
+    // traffic per vertex, 4 B per in-neighbour id (each array counted once;
+    // the random contribution gathers are not).  A Gauss-Seidel pass is
+    // credited its share.  One launch per pass: the three bins' blocks run
+    // side by side.
+    const double per_vertex = 8.0 + vbytes, share = 1.0 / passes;
+    const double bytes = per_vertex * n + 4.0 * (bins.e_small + bins.e_medium + bins.e_large);
+    const int32_t nb_large =
+        static_cast<int32_t>(bins.n_large ? GridFor(bins.n_large, 1, g->cu_count * (width + 1) / 2) : 0);
+    const int32_t nb_medium =
+        static_cast<int32_t>(bins.n_medium ? GridFor(bins.n_medium, kBS / kWave, g->cu_count * width) : 0);
+    const int32_t nb_small =
+        static_cast<int32_t>(bins.n_small ? GridFor(bins.n_small, kBS, g->cu_count * width) : 0);
+    for (int pass = 0; pass < passes; pass++) {
+      a.lo = static_cast<int32_t>(n * pass / passes);
+      a.hi = static_cast<int32_t>(n * (pass + 1) / passes);
+      Run(g, "pr_gather", share * bytes, PrGather<T>, dim3(nb_large + nb_medium + nb_small), blk, a,
+          static_cast<const int32_t*>(bins.small), static_cast<int32_t>(bins.n_small),
+          static_cast<const int32_t*>(bins.medium), static_cast<int32_t>(bins.n_medium),
+          static_cast<const int32_t*>(bins.large), static_cast<int32_t>(bins.n_large), nb_large, nb_medium);
+      launches++;
+    }
     const unsigned long long seq = MailNext(g);
     Run(g, "pr_publish", 0, PrPublish, dim3(1), blk, residual, g->mail_dev, seq);
     launches++;

@@ -1,0 +1,270 @@
+// Relabelling by degree on the device (gg_relabel_by_degree).
+//
+// Reference: BuilderBase::RelabelByDegree (src/builder.h:367-402), which
+// TC's Hybrid runs inside its timed trial (src/tc.cc:97): vertices get new
+// ids in the order of descending (degree, old id) -- std::sort with
+// std::greater over (degree, id) pairs, so equal degrees keep the larger
+// old id first -- every list is rewritten through the new ids and sorted.
+// The result here is byte-identical to that.
+//
+// Device steps:
+//   1. key = ~((degree << 32) | id) per vertex, radix-sorted ascending over
+//      the id bits and the degree bits (gg_sort.h): descending (degree, id);
+//   2. new_id[old] = rank, new offsets = exclusive scan of the sorted
+//      degrees;
+//   3. every list copied through new_id into its new row (tile expansion,
+//      hubs spread over the grid in chunks);
+//   4. rows sorted: the rows are in descending degree order, so rows with
+//      more than kMediumRow entries are a prefix -- sorted together as
+//      (row, value) keys by the radix sort; rows of kSmallRow+1 ..
+//      kMediumRow entries by a block bitonic sort in LDS; shorter rows by
+//      one thread each (insertion sort in LDS).
+#include "gg_expand.h"
+#include "gg_sort.h"
+
+namespace gg {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kSmallRow = 32;
+constexpr int kMediumRow = 2048;
+
+int BitsFor(uint64_t max_value) {
+  int b = 0;
+  while (b < 64 && (max_value >> b) != 0) b++;
+  return b;
+}
+
+__global__ void RlZero(int32_t* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ __launch_bounds__(kBS) void RlKeys(const int64_t* __restrict__ off, int64_t n, uint64_t* __restrict__ keys) {
+  for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
+    const uint64_t deg = static_cast<uint64_t>(off[v + 1] - off[v]);
+    keys[v] = ~((deg << 32) | static_cast<uint64_t>(v));
+  }
+}
+
+// new_id[old] = rank; row_len[rank] = degree (scanned into offsets next).
+__global__ __launch_bounds__(kBS) void RlRanks(const uint64_t* __restrict__ sorted, int64_t n,
+                                               int32_t* __restrict__ new_id, int64_t* __restrict__ row_len) {
+  for (int64_t p = blockIdx.x * int64_t(kBS) + threadIdx.x; p < n; p += int64_t(gridDim.x) * kBS) {
+    const uint64_t k = ~sorted[p];
+    new_id[static_cast<uint32_t>(k)] = static_cast<int32_t>(p);
+    row_len[p] = static_cast<int64_t>(k >> 32);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) row_len[n] = 0;
+}
+
+struct RlCopy {
+  const int64_t* old_off;
+  const int32_t* old_dst;
+  const int64_t* new_off;
+  const int32_t* new_id;
+  int32_t* new_dst;
+  __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
+    if (!valid) return;
+    new_dst[new_off[new_id[u]] + (e - old_off[u])] = new_id[old_dst[e]];
+  }
+};
+
+__global__ __launch_bounds__(kBS) void RlCopyRows(const int64_t* __restrict__ old_off,
+                                                  const int32_t* __restrict__ old_dst,
+                                                  const int64_t* __restrict__ new_off,
+                                                  const int32_t* __restrict__ new_id, int32_t* new_dst, int64_t n,
+                                                  EdgeChunk* chunks, int32_t* nchunks) {
+  __shared__ ExpandShared sh;
+  RlCopy op{old_off, old_dst, new_off, new_id, new_dst};
+  for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < n; tile += int64_t(gridDim.x) * kBS) {
+    const int64_t u = tile + threadIdx.x;
+    int64_t b = 0, e = 0;
+    if (u < n) {
+      b = old_off[u];
+      e = old_off[u + 1];
+    }
+    ExpandTile(sh, static_cast<int32_t>(u), 0, b, e, op, chunks, nchunks, kChunkEdges);
+  }
+}
+
+__global__ __launch_bounds__(kBS) void RlCopyChunks(const int64_t* __restrict__ old_off,
+                                                    const int32_t* __restrict__ old_dst,
+                                                    const int64_t* __restrict__ new_off,
+                                                    const int32_t* __restrict__ new_id, int32_t* new_dst,
+                                                    const EdgeChunk* chunks, const int32_t* nchunks) {
+  RlCopy op{old_off, old_dst, new_off, new_id, new_dst};
+  ExpandChunks(chunks, *nchunks, op);
+}
+
+// bounds[0] = first row with <= kMediumRow entries, bounds[1] = first row
+// with <= kSmallRow entries (rows are in descending length order).
+__global__ void RlBounds(const int64_t* __restrict__ off, int64_t n, int64_t* bounds) {
+  if (threadIdx.x < 2) {
+    const int64_t limit = threadIdx.x == 0 ? kMediumRow : kSmallRow;
+    int64_t lo = 0, hi = n;  // first p with len(p) <= limit
+    while (lo < hi) {
+      const int64_t m = lo + ((hi - lo) >> 1);
+      if (off[m + 1] - off[m] <= limit)
+        hi = m;
+      else
+        lo = m + 1;
+    }
+    bounds[threadIdx.x] = lo;
+  }
+}
+
+// Rows [first, n) with at most kSmallRow entries: one thread per row.
+__global__ __launch_bounds__(kBS) void RlSortSmall(const int64_t* __restrict__ off, int32_t* dst,
+                                                   const int64_t* __restrict__ bounds, int64_t n) {
+  __shared__ int32_t buf[kBS * kSmallRow];
+  int32_t* mine = buf + threadIdx.x * kSmallRow;
+  for (int64_t r = bounds[1] + blockIdx.x * int64_t(kBS) + threadIdx.x; r < n; r += int64_t(gridDim.x) * kBS) {
+    const int64_t b = off[r];
+    const int len = static_cast<int>(off[r + 1] - b);
+    for (int i = 0; i < len; i++) {
+      const int32_t x = dst[b + i];
+      int j = i;
+      while (j > 0 && mine[j - 1] > x) {
+        mine[j] = mine[j - 1];
+        j--;
+      }
+      mine[j] = x;
+    }
+    for (int i = 0; i < len; i++) dst[b + i] = mine[i];
+  }
+}
+
+// Rows [bounds[0], bounds[1]): one block per row, bitonic sort in LDS.
+__global__ __launch_bounds__(kBS) void RlSortMedium(const int64_t* __restrict__ off, int32_t* dst,
+                                                    const int64_t* __restrict__ bounds) {
+  __shared__ int32_t a[kMediumRow];
+  const int64_t first = bounds[0], last = bounds[1];
+  for (int64_t r = first + blockIdx.x; r < last; r += gridDim.x) {
+    const int64_t b = off[r];
+    const int len = static_cast<int>(off[r + 1] - b);
+    int size = 64;
+    while (size < len) size <<= 1;
+    for (int i = threadIdx.x; i < size; i += kBS) a[i] = i < len ? dst[b + i] : INT32_MAX;
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < size; i += kBS) {
+          const int p = i ^ j;
+          if (p > i) {
+            const bool up = (i & k) == 0;
+            const int32_t x = a[i], y = a[p];
+            if ((x > y) == up) {
+              a[i] = y;
+              a[p] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < len; i += kBS) dst[b + i] = a[i];
+    __syncthreads();
+  }
+}
+
+// Rows [0, big_rows): keys (row << 32 | value) over the prefix [0, entries).
+__global__ __launch_bounds__(kBS) void RlBigKeys(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                 int64_t big_rows, uint64_t* __restrict__ keys) {
+  // One block per row, striding over its entries.
+  for (int64_t r = blockIdx.x; r < big_rows; r += gridDim.x) {
+    const int64_t b = off[r], e = off[r + 1];
+    for (int64_t k = b + threadIdx.x; k < e; k += kBS)
+      keys[k] = (static_cast<uint64_t>(r) << 32) | static_cast<uint32_t>(dst[k]);
+  }
+}
+
+__global__ __launch_bounds__(kBS) void RlBigValues(const uint64_t* __restrict__ keys, int64_t m,
+                                                   int32_t* __restrict__ dst) {
+  for (int64_t k = blockIdx.x * int64_t(kBS) + threadIdx.x; k < m; k += int64_t(gridDim.x) * kBS)
+    dst[k] = static_cast<int32_t>(static_cast<uint32_t>(keys[k]));
+}
+
+}  // namespace
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(out != nullptr, GG_EINVAL, "null output handle");
+    *out = nullptr;
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "relabelling needs an unweighted graph upload");
+    GG_REQUIRE(!g->directed, GG_ENOTSUP, "Cannot relabel directed graph");
+    GG_HIP(hipSetDevice(g->device));
+    const int64_t n = g->n, entries = g->out_entries;
+    GG_REQUIRE(n < (int64_t(1) << 31) && entries < (int64_t(1) << 32), GG_ENOTSUP, "graph too large to relabel");
+    gg_graph* r = NewDeviceGraph(g->device, n, entries);
+    try {
+      const dim3 blk(kBS), grid_n(GridFor(n, kBS, g->cu_count * 16));
+      uint64_t* keys = g->Scratch<uint64_t>("rl.keys", n + 1);
+      uint64_t* tmp = g->Scratch<uint64_t>("rl.tmp", n + 1);
+      int32_t* new_id = g->Scratch<int32_t>("rl.new_id", n + 1);
+      int64_t* bounds = g->Scratch<int64_t>("rl.bounds", 2);
+      int32_t* nchunks = g->Scratch<int32_t>("rl.nchunks", 1);
+      const bool use_chunks = g->max_degree >= kExpandBS;
+      EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("rl.chunks", MaxChunks(n, entries)) : nullptr;
+      const double N = static_cast<double>(n), E = static_cast<double>(entries);
+      BeginTiming(g);
+      // 1. descending (degree, id)
+      Run(g, "relabel_keys", 16.0 * N, RlKeys, grid_n, blk, static_cast<const int64_t*>(g->out_off), n, keys);
+      const int id_bits = BitsFor(n > 0 ? static_cast<uint64_t>(n - 1) : 0);
+      const int deg_bits = BitsFor(static_cast<uint64_t>(g->max_degree));
+      uint64_t* sorted = RadixSort(g, "rl.sort", keys, tmp, n, 0, id_bits);
+      sorted = RadixSort(g, "rl.sort", sorted, sorted == keys ? tmp : keys, n, 32, 32 + deg_bits);
+      // 2. ranks and the new offsets
+      Run(g, "relabel_ranks", 20.0 * N, RlRanks, grid_n, blk, static_cast<const uint64_t*>(sorted), n, new_id,
+          r->out_off);
+      DeviceScan<int64_t>(g, "rl.offscan", r->out_off, n + 1, static_cast<int64_t*>(nullptr));
+      // 3. lists through the new ids
+      Run(g, "relabel_clear", 0, RlZero, dim3(1), dim3(64), nchunks);
+      Run(g, "relabel_copy", 8.0 * N + 12.0 * E, RlCopyRows, dim3(GridFor(n, kBS, g->cu_count * 8)), blk,
+          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+          static_cast<const int64_t*>(r->out_off), static_cast<const int32_t*>(new_id), r->out_dst, n,
+          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks);
+      if (use_chunks)
+        Run(g, "relabel_copy", 0, RlCopyChunks, dim3(g->cu_count * 4), blk, static_cast<const int64_t*>(g->out_off),
+            static_cast<const int32_t*>(g->out_dst), static_cast<const int64_t*>(r->out_off),
+            static_cast<const int32_t*>(new_id), r->out_dst, static_cast<const EdgeChunk*>(chunks),
+            static_cast<const int32_t*>(nchunks));
+      // 4. sort the rows
+      Run(g, "relabel_bounds", 0, RlBounds, dim3(1), dim3(64), static_cast<const int64_t*>(r->out_off), n, bounds);
+      int64_t hb[2];
+      GG_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, g->stream));
+      GG_HIP(hipStreamSynchronize(g->stream));
+      int64_t big_entries = 0;
+      if (hb[0] > 0) GG_HIP(hipMemcpy(&big_entries, r->out_off + hb[0], sizeof(int64_t), hipMemcpyDeviceToHost));
+      Run(g, "relabel_sort_small", 8.0 * (n - hb[1]), RlSortSmall, dim3(GridFor(n - hb[1], kBS, g->cu_count * 8)),
+          blk, static_cast<const int64_t*>(r->out_off), r->out_dst, static_cast<const int64_t*>(bounds), n);
+      if (hb[1] > hb[0])
+        Run(g, "relabel_sort_medium", 0, RlSortMedium, dim3(GridFor(hb[1] - hb[0], 1, g->cu_count * 8)), blk,
+            static_cast<const int64_t*>(r->out_off), r->out_dst, static_cast<const int64_t*>(bounds));
+      if (hb[0] > 0) {
+        uint64_t* bk = g->Scratch<uint64_t>("rl.big_keys", big_entries);
+        uint64_t* bt = g->Scratch<uint64_t>("rl.big_tmp", big_entries);
+        Run(g, "relabel_big_keys", 12.0 * big_entries, RlBigKeys, dim3(GridFor(hb[0], 1, g->cu_count * 8)), blk,
+            static_cast<const int64_t*>(r->out_off), static_cast<const int32_t*>(r->out_dst), hb[0], bk);
+        uint64_t* s = RadixSort(g, "rl.big", bk, bt, big_entries, 0, id_bits);
+        s = RadixSort(g, "rl.big", s, s == bk ? bt : bk, big_entries, 32, 32 + BitsFor(hb[0] - 1));
+        Run(g, "relabel_big_values", 12.0 * big_entries, RlBigValues,
+            dim3(GridFor(big_entries, kBS, g->cu_count * 16)), blk, static_cast<const uint64_t*>(s), big_entries,
+            r->out_dst);
+      }
+      GG_HIP(hipGetLastError());
+      FinishTiming(g, stats, wt);
+      r->max_degree = g->max_degree;
+    } catch (...) {
+      gg_graph_free(r);
+      throw;
+    }
+    *out = r;
+    if (stats) stats->total_ms = wt.ms();
+  });
+}

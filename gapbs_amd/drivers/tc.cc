@@ -1,7 +1,7 @@
 // tc driver: the reference's main() (src/tc.cc:131-143) with OrderedCount
 // replaced by gg_tc on the GPU.  As in Hybrid (tc.cc:95-100), a graph worth
-// relabelling is relabelled by degree inside the trial; the relabelled graph
-// is then uploaded and counted (also inside the trial).
+// relabelling is relabelled by degree inside the trial -- on the device
+// (gg_relabel_by_degree) -- and the relabelled graph is counted.
 #include "driver_common.h"
 
 using namespace gapdrv;
@@ -17,20 +17,17 @@ int main(int argc, char* argv[]) {
     }
     DeviceSession<Graph> dev(cli.device(), g);
     const bool relabel = WorthRelabelling(g);
-    Graph relabelled;
-    std::unique_ptr<DeviceSession<Graph>> rdev;
     auto Hybrid = [&](const Graph&) {
       uint64_t count = 0;
       gg_stats st;
       if (relabel) {
-        Timer t;
-        t.Start();
-        relabelled = RelabelByDegree(g);
-        t.Stop();
-        PrintTime("Relabel", t.Seconds());
-        rdev.reset(new DeviceSession<Graph>(cli.device(), relabelled));
-        Check(gg_tc(rdev->get(), &count, &st), "gg_tc");
-        rdev.reset();
+        gg_graph* r = nullptr;
+        gg_stats rs;
+        Check(gg_relabel_by_degree(dev.get(), &r, &rs), "gg_relabel_by_degree");
+        PrintTime("Relabel", rs.total_ms / 1e3);
+        const int rc = gg_tc(r, &count, &st);
+        gg_graph_free(r);
+        Check(rc, "gg_tc");
       } else {
         Check(gg_tc(dev.get(), &count, &st), "gg_tc");
       }

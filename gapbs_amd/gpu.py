@@ -75,7 +75,7 @@ TUNE_BFS, TUNE_BATCH, TUNE_SSSP, TUNE_PR = 1, 2, 4, 8
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
-    "gg_autotune", "gg_tuning_get", "gg_tuning_set",
+    "gg_relabel_by_degree", "gg_graph_copy_out", "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
@@ -108,6 +108,8 @@ def lib() -> C.CDLL:
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_bc_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
+        L.gg_relabel_by_degree.argtypes = [vp, C.POINTER(vp), C.POINTER(Stats)]
+        L.gg_graph_copy_out.argtypes = [vp, _i64p, _i32p]
         L.gg_autotune.argtypes = [vp, C.c_int, _i32p, C.c_int, C.POINTER(Tuning)]
         L.gg_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
         L.gg_tuning_set.argtypes = [vp, C.POINTER(Tuning)]
@@ -157,9 +159,28 @@ class DeviceGraph:
                                      int(g.weighted), int(g.directed), ioff, idst, C.byref(h)))
         self._h = h
 
+    @classmethod
+    def _adopt(cls, handle: C.c_void_p, num_nodes: int) -> "DeviceGraph":
+        """Wraps a handle the library created (undirected, unweighted)."""
+        self = cls.__new__(cls)
+        self.host = None
+        self.num_nodes = num_nodes
+        self.weighted = False
+        self.directed = False
+        self._h = handle
+        return self
+
     @property
     def handle(self):
         return self._h
+
+    def copy_out(self) -> tuple[np.ndarray, np.ndarray]:
+        """(offsets[n + 1], dests) of the device CSR (gg_graph_copy_out)."""
+        off = np.empty(self.num_nodes + 1, dtype=np.int64)
+        _check(lib().gg_graph_copy_out(self._h, _p(off, C.c_int64), None))
+        dst = np.empty(int(off[-1]) * (2 if self.weighted else 1), dtype=np.int32)
+        _check(lib().gg_graph_copy_out(self._h, None, _p(dst, C.c_int32)))
+        return off, dst
 
     def device_bytes(self) -> int:
         return lib().gg_graph_device_bytes(self._h)
@@ -391,8 +412,11 @@ def OrderedCount(dg: DeviceGraph) -> int:
 def Hybrid(g: Graph, device: int = 0) -> int:
     """Relabel by degree when the reference's heuristic says so, then count
     (src/tc.cc:75-100)."""
-    h = g.relabel_by_degree() if g.worth_relabelling() else g
-    return OrderedCount(DeviceGraph(h, device))
+    dg = DeviceGraph(g, device)
+    if not g.worth_relabelling():
+        return OrderedCount(dg)
+    dr, _ = relabel_by_degree(dg)  # on the device, as part of the count
+    return OrderedCount(dr)
 
 
 def autotune(dg: DeviceGraph, what: int, roots=()) -> dict:
@@ -418,3 +442,12 @@ def set_tuning(dg: DeviceGraph, values: dict) -> None:
             continue
         setattr(t, k, v)
     _check(lib().gg_tuning_set(dg.handle, C.byref(t)))
+
+
+def relabel_by_degree(dg: DeviceGraph) -> tuple[DeviceGraph, dict]:
+    """gg_relabel_by_degree: a new device graph numbered by descending
+    (degree, id) with sorted lists (src/builder.h:367-402), and the stats."""
+    h = C.c_void_p()
+    st = Stats()
+    _check(lib().gg_relabel_by_degree(dg.handle, C.byref(h), C.byref(st)))
+    return DeviceGraph._adopt(h, dg.num_nodes), st.as_dict()

@@ -163,6 +163,20 @@ int gg_bc_batch(gg_graph* g, const int32_t* roots, int k, int mode, float* score
  * the number of (set, list-entry) probes. */
 int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats);
 
+/* Relabelling by degree on the device.  Replaces
+ * BuilderBase::RelabelByDegree (src/builder.h:367-402), which TC's Hybrid
+ * runs inside its timed trial (src/tc.cc:97): *out receives a new handle
+ * (undirected, unweighted) whose vertices are numbered by descending
+ * (degree, old id) and whose lists are sorted -- byte-identical to the
+ * reference's result.  Directed or weighted uploads: GG_ENOTSUP ("Cannot
+ * relabel directed graph", as the reference).  stats->device_ms is the
+ * relabel's device time.  Free *out with gg_graph_free. */
+int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats);
+
+/* Copies a handle's CSR back to host memory (offsets[n + 1] and dests, each
+ * nullable): the layout gg_graph_upload takes. */
+int gg_graph_copy_out(const gg_graph* g, int64_t* offsets, int32_t* dests);
+
 /* Per-graph tuning (no reference counterpart; the reference's knobs are the
  * -a/-b style alpha/beta defaults in bfs.cc:122-123 and sssp's -d delta).
  * gg_autotune times candidate settings of the selected kernels on this
