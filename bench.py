@@ -90,6 +90,7 @@ class Timed:
                 p = self.prof.setdefault(k, {"launches": 0, "ms": 0.0, "alg_bytes": 0.0})
                 for f in p:
                     p[f] += v[f]
+        for dg in self.dgs:
             dg.profile(False)
 
 
@@ -313,7 +314,7 @@ def main():
             tuned = gpu.autotune(dg, what, roots)
         if "sssp" in kernels:
             tw = gpu.autotune(dwg, gpu.TUNE_SSSP, sssp_roots)
-            tuned = {**(tuned or {}), "sssp_delta": tw["sssp_delta"]}
+            tuned = {**(tuned or {}), "sssp_delta": tw["sssp_delta"], "sssp_best_ms": tw["best_ms"][2]}
         if tuned is not None:
             tuned["seconds"] = round(time.time() - t0, 2)
             alpha, beta = tuned.get("bfs_alpha", alpha), tuned.get("bfs_beta", beta)
