@@ -3,7 +3,12 @@
 Workload (default): a Kronecker graph, scale 26, edge factor 16 (Graph500
 R-MAT A=.57 B=.19 C=.19, IDs permuted, symmetrised, generator seed 42; 67.1M
 vertices, 2.1B directed edges, int64 offsets), built on the host and
-uploaded to HBM before any timing.
+uploaded to HBM before any timing.  By default the device graphs are then
+renumbered by descending degree in HBM (gg_graph_order_by_degree: hubs
+together, so the per-vertex arrays the kernels gather through the lists
+stay cache-resident); the renumbering's device time is reported under
+config.order_ms and not counted, every call still takes and returns the
+generator's ids, and --layout upload keeps the generator's order.
 
 Headline step: one batched BFS of 64 SourcePicker roots (64-bit per-vertex
 visit masks, gg_bfs_batch).  value = aggregate GTEPS = sum over roots of m_r
@@ -224,6 +229,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the reference's alpha/beta (15/18), delta --delta and default PageRank bins")
     ap.add_argument("--copy-probe", type=int, default=0, help="only run one device copy of this many bytes")
+    ap.add_argument("--layout", choices=("degree", "upload"), default="degree",
+                    help="device vertex order: descending degree (default) or as generated")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the kernels "
@@ -241,18 +248,28 @@ def main():
     from gapbs_amd.graph import Graph
 
     workload = f"kronecker s{args.scale} ef16 seed {args.seed}"
+    if args.layout == "degree":
+        workload += ", degree-ordered layout"
+    ordered = args.layout == "degree"
+    order_ms = {}
     t0 = time.time()
     g = Graph.kronecker(args.scale, 16, seed=args.seed)
     log(f"{g.stats_line()} built in {time.time() - t0:.1f}s")
     roots = [int(r) for r in g.pick_sources(args.roots)]
-    dg = gpu.DeviceGraph(g)
+    dg = gpu.DeviceGraph(g, order_by_degree=ordered)
+    if ordered:
+        order_ms["graph"] = round(dg.order_stats["device_ms"], 3)
     wg = dwg = None
     relabel_tc = "tc" in kernels and g.worth_relabelling()
     if "sssp" in kernels:
         t0 = time.time()
         wg = Graph.kronecker(args.scale, 16, seed=args.seed, weighted=True)
-        dwg = gpu.DeviceGraph(wg)
+        dwg = gpu.DeviceGraph(wg, order_by_degree=ordered)
+        if ordered:
+            order_ms["weighted_graph"] = round(dwg.order_stats["device_ms"], 3)
         log(f"weighted graph built and uploaded in {time.time() - t0:.1f}s")
+    if ordered:
+        log(f"degree ordering (device ms): {order_ms}")
     log(f"device bytes {sum(d.device_bytes() for d in (dg, dwg) if d) / 1e9:.2f} GB")
 
     # m_r per root, from the device's own reach statistics.
@@ -398,7 +415,8 @@ def main():
                                f"({len(roots)} roots), tc",
                    "num_nodes": g.num_nodes, "num_edges_directed": g.num_edges_directed,
                    "roots": len(roots), "bfs_alpha_beta": [alpha, beta], "batch_alpha": batch_alpha,
-                   "sssp_delta": delta, "autotune": tuned},
+                   "sssp_delta": delta, "autotune": tuned, "layout": args.layout,
+                   "order_ms": order_ms or None},
         "roofline": head["roofline"],
         "kernels": results,
         "cpu_baseline": ({**cpu["bfs"], "sample": cpu["bfs"]["sample"] + " (the reference has no batched "

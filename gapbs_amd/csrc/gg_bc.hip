@@ -505,7 +505,7 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   launches += 2;
   GG_HIP(hipGetLastError());
   FinishTiming(g, stats, wt);
-  if (scores_out) GG_HIP(hipMemcpy(scores_out, out32, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (scores_out) CopyOutVertexF32(g, out32, scores_out);
   if (stats) {
     stats->total_ms = wt.ms();
     stats->iterations = levels_total;
@@ -531,6 +531,9 @@ extern "C" int gg_bc(gg_graph* g, const int32_t* roots, int k, int mode, float* 
                "unknown precision mode");
     for (int i = 0; i < k; i++) GG_REQUIRE(roots[i] >= 0 && roots[i] < g->n, GG_EINVAL, "root out of range");
     GG_HIP(hipSetDevice(g->device));
+    std::vector<int32_t> mapped(roots, roots + k);
+    for (int32_t& r : mapped) r = DeviceVertex(g, r);
+    roots = mapped.data();
     if (mode == GG_BC_FP32)
       RunBrandes<float, false>(g, roots, k, scores, stats, wt);
     else if (mode == GG_BC_FP32_KAHAN)

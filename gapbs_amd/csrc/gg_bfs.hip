@@ -394,6 +394,7 @@ extern "C" int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t*
     GG_REQUIRE(source >= 0 && source < g->n, GG_EINVAL, "source out of range");
     GG_REQUIRE(alpha > 0 && beta > 0, GG_EINVAL, "alpha and beta must be positive");
     GG_HIP(hipSetDevice(g->device));
+    source = DeviceVertex(g, source);
     const int64_t n = g->n;
     const int64_t nwords = (n + 63) / 64;
     hipStream_t st = g->stream;
@@ -527,8 +528,8 @@ extern "C" int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t*
     launches++;
     GG_HIP(hipGetLastError());
     FinishTiming(g, stats, wt);
-    if (parent_out) GG_HIP(hipMemcpy(parent_out, parent, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    if (depth_out) GG_HIP(hipMemcpy(depth_out, depth, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (parent_out) CopyOutVertexI32(g, parent, parent_out, true);
+    if (depth_out) CopyOutVertexI32(g, depth, depth_out, false);
     if (stats) {
       stats->total_ms = wt.ms();
       stats->iterations = td + bu;

@@ -211,7 +211,7 @@ extern "C" int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_sta
     launches++;
     GG_HIP(hipGetLastError());
     FinishTiming(g, stats, wt);
-    if (comp_out) GG_HIP(hipMemcpy(comp_out, comp, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (comp_out) CopyOutComponents(g, comp, comp_out);
     if (stats) {
       unsigned long long h[64];
       GG_HIP(hipMemcpy(h, examined, sizeof(h), hipMemcpyDeviceToHost));

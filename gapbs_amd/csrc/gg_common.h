@@ -92,6 +92,11 @@ struct gg_graph {
   int64_t* in_off = nullptr;   // == out_off for undirected graphs
   int32_t* in_dst = nullptr;
   size_t graph_bytes = 0;
+  // Degree-ordered layout (gg_graph_order_by_degree): the lists above use
+  // device ids, new_id[original id] is a vertex's device id and
+  // old_id[device id] its original id.  Both null: ids as uploaded.
+  int32_t* new_id = nullptr;
+  int32_t* old_id = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
   gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}};  // gg_autotune results
@@ -139,6 +144,21 @@ namespace gg {
 // A new undirected, unweighted handle on `device` with uninitialised device
 // arrays for n vertices and `entries` destinations (filled by the caller).
 gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries);
+
+// Vertex ids across the C-ABI are the uploaded ones; a degree-ordered graph
+// maps them on the way in and its per-vertex results on the way out.
+// DeviceVertex: the device id of original vertex v (synchronous read).
+int32_t DeviceVertex(gg_graph* g, int32_t v);
+// In place on g->stream: k original ids at d (device memory) -> device ids.
+void DeviceVertices(gg_graph* g, int32_t* d, int64_t k);
+// Copies a per-vertex array in device order (src, device memory) to host in
+// original order; with ids, non-negative values are vertex ids and are
+// mapped too.  Synchronous.
+void CopyOutVertexI32(gg_graph* g, const int32_t* src, int32_t* host, bool ids);
+void CopyOutVertexF32(gg_graph* g, const float* src, float* host);
+// Component labels in device order, each the smallest device id of its
+// component -> host, each the smallest original id of its component.
+void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host);
 
 // Host wall clock for gg_stats::total_ms.
 struct WallTimer {

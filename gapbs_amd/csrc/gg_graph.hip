@@ -355,6 +355,8 @@ int gg_graph_free(gg_graph* g) {
       DeviceFree(g->in_off, off_bytes);
       DeviceFree(g->in_dst, sizeof(int32_t) * width * g->in_entries);
     }
+    DeviceFree(g->new_id, sizeof(int32_t) * (g->n > 0 ? g->n : 1));
+    DeviceFree(g->old_id, sizeof(int32_t) * (g->n > 0 ? g->n : 1));
     if (g->pinned) (void)hipHostFree(g->pinned);
     if (g->mail_host) (void)hipHostFree(const_cast<unsigned long long*>(g->mail_host));
     for (auto& p : g->pending) {

@@ -59,8 +59,9 @@ __device__ __forceinline__ void PopBits(uint64_t& any, int (&j)[B]) {
 }
 
 // Sigma of the vertices that gained bits at this level (cur = F_L, prev =
-// F_{L-1}).  Each wavefront scans 64 consecutive vertices and serves those
-// with new bits one at a time.  Heavy vertices (>= kHeavyEdges in-edges)
+// F_{L-1}).  Wavefront w of W scans the vertices w, w + W, w + 2W, ... 64 at
+// a time (consecutive ids on different wavefronts) and serves those with
+// new bits one at a time.  Heavy vertices (>= kHeavyEdges in-edges)
 // only get their entries zeroed here; MbSigmaHeavy adds their sums.
 __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
                                                  const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
@@ -71,8 +72,8 @@ __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_
   const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
   unsigned long long ex = 0;
-  for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
-    const int64_t vl = base + lane;
+  for (int64_t base = wave; base < n; base += nwaves * kWave) {
+    const int64_t vl = base + lane * nwaves;
     uint64_t fl = 0;
     int64_t bl = 0, el = 0;
     if (vl < n) {
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_
     while (act) {
       const int i = __builtin_ctzll(act);
       act &= act - 1;
-      const int64_t v = base + i;
+      const int64_t v = base + i * nwaves;
       const uint64_t fv = __shfl(fl, i);
       const int64_t b = __shfl(bl, i), e = __shfl(el, i);
       const bool mine = (fv >> lane) & 1;
@@ -255,10 +256,11 @@ __device__ __forceinline__ int64_t DeltaVertex(const int32_t* __restrict__ dst, 
 }
 
 // Dependencies of the vertices at level L (cur = F_L, next = F_{L+1}, null
-// at the deepest level).  Each wavefront scans 64 consecutive vertices and
-// serves those with bits at L one at a time.  The fold of each (vertex,
-// root) is sequential by definition, so a hub's wavefront is the longest
-// one; spread over the scan, hubs overlap with the other vertices' work.
+// at the deepest level).  Wavefront w of W scans the vertices w, w + W,
+// w + 2W, ... 64 at a time and serves those with bits at L one at a time.
+// The fold of each (vertex, root) is sequential by definition, so a hub's
+// wavefront is the longest one; the stride puts consecutive ids (in a
+// degree-ordered layout, hubs of similar size) on different wavefronts.
 template <class T>
 __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                  const uint64_t* __restrict__ cur, const uint64_t* __restrict__ next,
@@ -269,8 +271,8 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
   const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
   unsigned long long ex = 0;
-  for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
-    const int64_t ul = base + lane;
+  for (int64_t base = wave; base < n; base += nwaves * kWave) {
+    const int64_t ul = base + lane * nwaves;
     uint64_t fl = 0;
     int64_t bl = 0, el = 0;
     if (ul < n) {
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
       const int i = __builtin_ctzll(act);
       act &= act - 1;
       ex += static_cast<unsigned long long>(
-          DeltaVertex<T>(dst, next, sigma, delta, base + i, __shfl(fl, i), __shfl(bl, i), __shfl(el, i)));
+          DeltaVertex<T>(dst, next, sigma, delta, base + i * nwaves, __shfl(fl, i), __shfl(bl, i), __shfl(el, i)));
     }
   }
   ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
@@ -397,10 +399,7 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
   launches += 3;
   GG_HIP(hipGetLastError());
   FinishTiming(g, stats, wt);
-  if (scores) {
-    GG_HIP(hipMemcpyAsync(scores, out, sizeof(float) * n, hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-  }
+  if (scores) CopyOutVertexF32(g, out, scores);
   if (stats) {
     unsigned long long h[2 * kMbStripes];
     GG_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -441,6 +440,7 @@ extern "C" int gg_bc_batch(gg_graph* g, const int32_t* roots, int k, int mode, f
     GG_HIP(hipSetDevice(g->device));
     int32_t* droots = g->Scratch<int32_t>("ms.roots", 64);
     GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * k, hipMemcpyHostToDevice, g->stream));
+    DeviceVertices(g, droots, k);
     if (mode == GG_BC_FP32)
       RunBatchBrandes<float>(g, droots, k, scores, stats, wt);
     else

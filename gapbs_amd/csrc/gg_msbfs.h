@@ -535,11 +535,13 @@ __global__ void MsZero2(unsigned long long* p) {
   if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
-// depth16[v][r] -> depth32[v] of root r (-1 where unreached).
-__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16, int64_t n, int r,
+// depth16[v][r] -> depth32[v] of root r (-1 where unreached); with new_id,
+// out is indexed by original ids.
+__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16,
+                                                  const int32_t* __restrict__ new_id, int64_t n, int r,
                                                   int32_t* __restrict__ out) {
   for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
-    const uint16_t d = d16[v * 64 + r];
+    const uint16_t d = d16[(new_id ? int64_t(new_id[v]) : v) * 64 + r];
     out[v] = d == kNoDepth ? -1 : d;
   }
 }

@@ -25,6 +25,7 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
     uint16_t* depth16 = g->Scratch<uint16_t>("ms.depth16", n * 64);
     int32_t* droots = g->Scratch<int32_t>("ms.roots", 64);
     GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    DeviceVertices(g, droots, k);
     BeginTiming(g);
     const MsTraversal t = MsTraverse(g, droots, nullptr, k, k == 64 ? ~0ull : (1ull << k) - 1, alpha, depth16, false);
     GG_HIP(hipGetLastError());
@@ -32,7 +33,7 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
     if (depths) {
       int32_t* d32 = g->Scratch<int32_t>("ms.depth_out", n);
       for (int r = 0; r < k; r++) {
-        MsDepthOut<<<grid_n, blk, 0, st>>>(depth16, n, r, d32);
+        MsDepthOut<<<grid_n, blk, 0, st>>>(depth16, g->new_id, n, r, d32);
         GG_HIP(hipMemcpyAsync(depths + int64_t(r) * n, d32, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
       }
       GG_HIP(hipStreamSynchronize(st));
@@ -117,6 +118,7 @@ extern "C" int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const
     const int R = roots_per_part;
     GG_REQUIRE(R >= 1 && R <= 64, GG_EINVAL, "1 to 64 roots per part");
     GG_REQUIRE(alpha > 0, GG_EINVAL, "alpha must be positive");
+    GG_REQUIRE(!g->new_id, GG_ENOTSUP, "packed traversal needs the uploaded vertex order (parts are id ranges)");
     GG_REQUIRE(part[0] == 0 && part[nparts] == g->n, GG_EINVAL, "parts must cover the vertices");
     for (int i = 0; i < nparts; i++) {
       GG_REQUIRE(part[i] < part[i + 1], GG_EINVAL, "parts must be non-empty and increasing");

@@ -395,7 +395,7 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
   }
   GG_HIP(hipGetLastError());
   FinishTiming(g, stats, wt);
-  if (scores_out) GG_HIP(hipMemcpy(scores_out, out32, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (scores_out) CopyOutVertexF32(g, out32, scores_out);
   if (stats) {
     stats->total_ms = wt.ms();
     stats->iterations = iters;

@@ -46,17 +46,23 @@ __global__ __launch_bounds__(kBS) void RlKeys(const int64_t* __restrict__ off, i
   }
 }
 
-// new_id[old] = rank; row_len[rank] = degree (scanned into offsets next).
+// new_id[old] = rank, old_id[rank] = old (old_id may be null);
+// row_len[rank] = degree (scanned into offsets next).
 __global__ __launch_bounds__(kBS) void RlRanks(const uint64_t* __restrict__ sorted, int64_t n,
-                                               int32_t* __restrict__ new_id, int64_t* __restrict__ row_len) {
+                                               int32_t* __restrict__ new_id, int32_t* __restrict__ old_id,
+                                               int64_t* __restrict__ row_len) {
   for (int64_t p = blockIdx.x * int64_t(kBS) + threadIdx.x; p < n; p += int64_t(gridDim.x) * kBS) {
     const uint64_t k = ~sorted[p];
     new_id[static_cast<uint32_t>(k)] = static_cast<int32_t>(p);
+    if (old_id) old_id[p] = static_cast<int32_t>(static_cast<uint32_t>(k));
     row_len[p] = static_cast<int64_t>(k >> 32);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) row_len[n] = 0;
 }
 
+// Entry e of u's list -> the same position of new_id[u]'s list; a weighted
+// entry (width 2) keeps its weight.
+template <int W>
 struct RlCopy {
   const int64_t* old_off;
   const int32_t* old_dst;
@@ -65,17 +71,20 @@ struct RlCopy {
   int32_t* new_dst;
   __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
     if (!valid) return;
-    new_dst[new_off[new_id[u]] + (e - old_off[u])] = new_id[old_dst[e]];
+    const int64_t to = new_off[new_id[u]] + (e - old_off[u]);
+    new_dst[W * to] = new_id[old_dst[W * e]];
+    if (W == 2) new_dst[W * to + 1] = old_dst[W * e + 1];
   }
 };
 
+template <int W>
 __global__ __launch_bounds__(kBS) void RlCopyRows(const int64_t* __restrict__ old_off,
                                                   const int32_t* __restrict__ old_dst,
                                                   const int64_t* __restrict__ new_off,
                                                   const int32_t* __restrict__ new_id, int32_t* new_dst, int64_t n,
                                                   EdgeChunk* chunks, int32_t* nchunks) {
   __shared__ ExpandShared sh;
-  RlCopy op{old_off, old_dst, new_off, new_id, new_dst};
+  RlCopy<W> op{old_off, old_dst, new_off, new_id, new_dst};
   for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < n; tile += int64_t(gridDim.x) * kBS) {
     const int64_t u = tile + threadIdx.x;
     int64_t b = 0, e = 0;
@@ -87,12 +96,13 @@ __global__ __launch_bounds__(kBS) void RlCopyRows(const int64_t* __restrict__ ol
   }
 }
 
+template <int W>
 __global__ __launch_bounds__(kBS) void RlCopyChunks(const int64_t* __restrict__ old_off,
                                                     const int32_t* __restrict__ old_dst,
                                                     const int64_t* __restrict__ new_off,
                                                     const int32_t* __restrict__ new_id, int32_t* new_dst,
                                                     const EdgeChunk* chunks, const int32_t* nchunks) {
-  RlCopy op{old_off, old_dst, new_off, new_id, new_dst};
+  RlCopy<W> op{old_off, old_dst, new_off, new_id, new_dst};
   ExpandChunks(chunks, *nchunks, op);
 }
 
@@ -184,6 +194,76 @@ __global__ __launch_bounds__(kBS) void RlBigValues(const uint64_t* __restrict__ 
     dst[k] = static_cast<int32_t>(static_cast<uint32_t>(keys[k]));
 }
 
+// Steps 1-2 for g's (out-)lists: new_id/old_id (old_id may be null) and
+// the offsets of the relabelled lists in new_off (n + 1 entries).
+void RankByDegree(gg_graph* g, int32_t* new_id, int32_t* old_id, int64_t* new_off) {
+  const int64_t n = g->n;
+  const dim3 blk(kBS), grid_n(GridFor(n, kBS, g->cu_count * 16));
+  uint64_t* keys = g->Scratch<uint64_t>("rl.keys", n + 1);
+  uint64_t* tmp = g->Scratch<uint64_t>("rl.tmp", n + 1);
+  const double N = static_cast<double>(n);
+  Run(g, "relabel_keys", 16.0 * N, RlKeys, grid_n, blk, static_cast<const int64_t*>(g->out_off), n, keys);
+  const int id_bits = BitsFor(n > 0 ? static_cast<uint64_t>(n - 1) : 0);
+  const int deg_bits = BitsFor(static_cast<uint64_t>(g->max_degree));
+  uint64_t* sorted = RadixSort(g, "rl.sort", keys, tmp, n, 0, id_bits);
+  sorted = RadixSort(g, "rl.sort", sorted, sorted == keys ? tmp : keys, n, 32, 32 + deg_bits);
+  Run(g, "relabel_ranks", (old_id ? 24.0 : 20.0) * N, RlRanks, grid_n, blk, static_cast<const uint64_t*>(sorted),
+      n, new_id, old_id, new_off);
+  DeviceScan<int64_t>(g, "rl.offscan", new_off, n + 1, static_cast<int64_t*>(nullptr));
+}
+
+// Step 3: every list of g through new_id into new_dst (rows at new_off).
+template <int W>
+void CopyLists(gg_graph* g, const int64_t* new_off, const int32_t* new_id, int32_t* new_dst) {
+  const int64_t n = g->n, entries = g->out_entries;
+  const dim3 blk(kBS);
+  int32_t* nchunks = g->Scratch<int32_t>("rl.nchunks", 1);
+  const bool use_chunks = g->max_degree >= kExpandBS;
+  EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("rl.chunks", MaxChunks(n, entries)) : nullptr;
+  Run(g, "relabel_clear", 0, RlZero, dim3(1), dim3(64), nchunks);
+  Run(g, "relabel_copy", 8.0 * n + (8.0 * W + 4.0) * entries, RlCopyRows<W>, dim3(GridFor(n, kBS, g->cu_count * 8)),
+      blk, static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), new_off, new_id,
+      new_dst, n, use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks);
+  if (use_chunks)
+    Run(g, "relabel_copy", 0, RlCopyChunks<W>, dim3(g->cu_count * 4), blk, static_cast<const int64_t*>(g->out_off),
+        static_cast<const int32_t*>(g->out_dst), new_off, new_id, new_dst, static_cast<const EdgeChunk*>(chunks),
+        static_cast<const int32_t*>(nchunks));
+}
+
+// Step 4: sorts every row of unweighted lists whose rows are in descending
+// length order (rows at off, n rows).
+void SortRows(gg_graph* g, const int64_t* off, int32_t* dst) {
+  const int64_t n = g->n;
+  const dim3 blk(kBS);
+  int64_t* bounds = g->Scratch<int64_t>("rl.bounds", 2);
+  Run(g, "relabel_bounds", 0, RlBounds, dim3(1), dim3(64), off, n, bounds);
+  int64_t hb[2];
+  GG_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  int64_t big_entries = 0;
+  if (hb[0] > 0) {
+    GG_HIP(hipMemcpyAsync(&big_entries, off + hb[0], sizeof(int64_t), hipMemcpyDeviceToHost, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
+  }
+  Run(g, "relabel_sort_small", 8.0 * (n - hb[1]), RlSortSmall, dim3(GridFor(n - hb[1], kBS, g->cu_count * 8)), blk,
+      off, dst, static_cast<const int64_t*>(bounds), n);
+  if (hb[1] > hb[0])
+    Run(g, "relabel_sort_medium", 0, RlSortMedium, dim3(GridFor(hb[1] - hb[0], 1, g->cu_count * 8)), blk, off, dst,
+        static_cast<const int64_t*>(bounds));
+  if (hb[0] > 0) {
+    const int id_bits = BitsFor(n > 0 ? static_cast<uint64_t>(n - 1) : 0);
+    uint64_t* bk = g->Scratch<uint64_t>("rl.big_keys", big_entries);
+    uint64_t* bt = g->Scratch<uint64_t>("rl.big_tmp", big_entries);
+    Run(g, "relabel_big_keys", 12.0 * big_entries, RlBigKeys, dim3(GridFor(hb[0], 1, g->cu_count * 8)), blk, off,
+        static_cast<const int32_t*>(dst), hb[0], bk);
+    uint64_t* s = RadixSort(g, "rl.big", bk, bt, big_entries, 0, id_bits);
+    s = RadixSort(g, "rl.big", s, s == bk ? bt : bk, big_entries, 32, 32 + BitsFor(hb[0] - 1));
+    Run(g, "relabel_big_values", 12.0 * big_entries, RlBigValues, dim3(GridFor(big_entries, kBS, g->cu_count * 16)),
+        blk, static_cast<const uint64_t*>(s), big_entries, dst);
+  }
+  GG_HIP(hipGetLastError());
+}
+
 }  // namespace
 }  // namespace gg
 
@@ -203,61 +283,11 @@ extern "C" int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats
     GG_REQUIRE(n < (int64_t(1) << 31) && entries < (int64_t(1) << 32), GG_ENOTSUP, "graph too large to relabel");
     gg_graph* r = NewDeviceGraph(g->device, n, entries);
     try {
-      const dim3 blk(kBS), grid_n(GridFor(n, kBS, g->cu_count * 16));
-      uint64_t* keys = g->Scratch<uint64_t>("rl.keys", n + 1);
-      uint64_t* tmp = g->Scratch<uint64_t>("rl.tmp", n + 1);
       int32_t* new_id = g->Scratch<int32_t>("rl.new_id", n + 1);
-      int64_t* bounds = g->Scratch<int64_t>("rl.bounds", 2);
-      int32_t* nchunks = g->Scratch<int32_t>("rl.nchunks", 1);
-      const bool use_chunks = g->max_degree >= kExpandBS;
-      EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("rl.chunks", MaxChunks(n, entries)) : nullptr;
-      const double N = static_cast<double>(n), E = static_cast<double>(entries);
       BeginTiming(g);
-      // 1. descending (degree, id)
-      Run(g, "relabel_keys", 16.0 * N, RlKeys, grid_n, blk, static_cast<const int64_t*>(g->out_off), n, keys);
-      const int id_bits = BitsFor(n > 0 ? static_cast<uint64_t>(n - 1) : 0);
-      const int deg_bits = BitsFor(static_cast<uint64_t>(g->max_degree));
-      uint64_t* sorted = RadixSort(g, "rl.sort", keys, tmp, n, 0, id_bits);
-      sorted = RadixSort(g, "rl.sort", sorted, sorted == keys ? tmp : keys, n, 32, 32 + deg_bits);
-      // 2. ranks and the new offsets
-      Run(g, "relabel_ranks", 20.0 * N, RlRanks, grid_n, blk, static_cast<const uint64_t*>(sorted), n, new_id,
-          r->out_off);
-      DeviceScan<int64_t>(g, "rl.offscan", r->out_off, n + 1, static_cast<int64_t*>(nullptr));
-      // 3. lists through the new ids
-      Run(g, "relabel_clear", 0, RlZero, dim3(1), dim3(64), nchunks);
-      Run(g, "relabel_copy", 8.0 * N + 12.0 * E, RlCopyRows, dim3(GridFor(n, kBS, g->cu_count * 8)), blk,
-          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
-          static_cast<const int64_t*>(r->out_off), static_cast<const int32_t*>(new_id), r->out_dst, n,
-          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks);
-      if (use_chunks)
-        Run(g, "relabel_copy", 0, RlCopyChunks, dim3(g->cu_count * 4), blk, static_cast<const int64_t*>(g->out_off),
-            static_cast<const int32_t*>(g->out_dst), static_cast<const int64_t*>(r->out_off),
-            static_cast<const int32_t*>(new_id), r->out_dst, static_cast<const EdgeChunk*>(chunks),
-            static_cast<const int32_t*>(nchunks));
-      // 4. sort the rows
-      Run(g, "relabel_bounds", 0, RlBounds, dim3(1), dim3(64), static_cast<const int64_t*>(r->out_off), n, bounds);
-      int64_t hb[2];
-      GG_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, g->stream));
-      GG_HIP(hipStreamSynchronize(g->stream));
-      int64_t big_entries = 0;
-      if (hb[0] > 0) GG_HIP(hipMemcpy(&big_entries, r->out_off + hb[0], sizeof(int64_t), hipMemcpyDeviceToHost));
-      Run(g, "relabel_sort_small", 8.0 * (n - hb[1]), RlSortSmall, dim3(GridFor(n - hb[1], kBS, g->cu_count * 8)),
-          blk, static_cast<const int64_t*>(r->out_off), r->out_dst, static_cast<const int64_t*>(bounds), n);
-      if (hb[1] > hb[0])
-        Run(g, "relabel_sort_medium", 0, RlSortMedium, dim3(GridFor(hb[1] - hb[0], 1, g->cu_count * 8)), blk,
-            static_cast<const int64_t*>(r->out_off), r->out_dst, static_cast<const int64_t*>(bounds));
-      if (hb[0] > 0) {
-        uint64_t* bk = g->Scratch<uint64_t>("rl.big_keys", big_entries);
-        uint64_t* bt = g->Scratch<uint64_t>("rl.big_tmp", big_entries);
-        Run(g, "relabel_big_keys", 12.0 * big_entries, RlBigKeys, dim3(GridFor(hb[0], 1, g->cu_count * 8)), blk,
-            static_cast<const int64_t*>(r->out_off), static_cast<const int32_t*>(r->out_dst), hb[0], bk);
-        uint64_t* s = RadixSort(g, "rl.big", bk, bt, big_entries, 0, id_bits);
-        s = RadixSort(g, "rl.big", s, s == bk ? bt : bk, big_entries, 32, 32 + BitsFor(hb[0] - 1));
-        Run(g, "relabel_big_values", 12.0 * big_entries, RlBigValues,
-            dim3(GridFor(big_entries, kBS, g->cu_count * 16)), blk, static_cast<const uint64_t*>(s), big_entries,
-            r->out_dst);
-      }
-      GG_HIP(hipGetLastError());
+      RankByDegree(g, new_id, nullptr, r->out_off);
+      CopyLists<1>(g, r->out_off, new_id, r->out_dst);
+      SortRows(g, r->out_off, r->out_dst);
       FinishTiming(g, stats, wt);
       r->max_degree = g->max_degree;
     } catch (...) {
@@ -265,6 +295,63 @@ extern "C" int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats
       throw;
     }
     *out = r;
+    if (stats) stats->total_ms = wt.ms();
+  });
+}
+
+extern "C" int gg_graph_order_by_degree(gg_graph* g, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->directed, GG_ENOTSUP, "degree ordering needs an undirected graph");
+    GG_REQUIRE(!g->new_id, GG_EINVAL, "graph is already degree-ordered");
+    GG_HIP(hipSetDevice(g->device));
+    const int64_t n = g->n, entries = g->out_entries;
+    GG_REQUIRE(entries < (int64_t(1) << 32), GG_ENOTSUP, "graph too large to order");
+    const int width = g->weighted ? 2 : 1;
+    const size_t off_bytes = sizeof(int64_t) * (n + 1), dst_bytes = sizeof(int32_t) * width * entries;
+    const size_t id_bytes = sizeof(int32_t) * (n > 0 ? n : 1);
+    int64_t* new_off = nullptr;
+    int32_t *new_dst = nullptr, *new_id = nullptr, *old_id = nullptr;
+    try {
+      new_off = static_cast<int64_t*>(DeviceAlloc(off_bytes));
+      new_dst = static_cast<int32_t*>(DeviceAlloc(dst_bytes));
+      new_id = static_cast<int32_t*>(DeviceAlloc(id_bytes));
+      old_id = static_cast<int32_t*>(DeviceAlloc(id_bytes));
+      BeginTiming(g);
+      RankByDegree(g, new_id, old_id, new_off);
+      if (g->weighted) {
+        // Weighted rows keep their uploaded order under the new ids: no
+        // kernel result depends on the order within a weighted list.
+        CopyLists<2>(g, new_off, new_id, new_dst);
+      } else {
+        CopyLists<1>(g, new_off, new_id, new_dst);
+        SortRows(g, new_off, new_dst);
+      }
+      FinishTiming(g, stats, wt);
+    } catch (...) {
+      (void)hipStreamSynchronize(g->stream);
+      DeviceFree(new_off, off_bytes);
+      DeviceFree(new_dst, dst_bytes);
+      DeviceFree(new_id, id_bytes);
+      DeviceFree(old_id, id_bytes);
+      throw;
+    }
+    DeviceFree(g->out_off, off_bytes);
+    DeviceFree(g->out_dst, dst_bytes);
+    g->out_off = g->in_off = new_off;
+    g->out_dst = g->in_dst = new_dst;
+    g->new_id = new_id;
+    g->old_id = old_id;
+    g->graph_bytes += 2 * id_bytes;
+    // Everything cached from the old layout is rebuilt on demand.
+    g->no_in_count = -1;
+    g->pr_bins[0] = g->pr_bins[1] = g->pr_bins[2] = -1;
+    g->ms_heavy = -1;
+    g->ms_heavy_chunks = 0;
+    g->cc_samples_n = -1;
+    g->ws.Release();
     if (stats) stats->total_ms = wt.ms();
   });
 }

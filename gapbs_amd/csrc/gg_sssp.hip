@@ -286,6 +286,7 @@ extern "C" int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist
     GG_REQUIRE(source >= 0 && source < g->n, GG_EINVAL, "source out of range");
     GG_REQUIRE(delta > 0, GG_EINVAL, "delta must be positive");
     GG_HIP(hipSetDevice(g->device));
+    source = DeviceVertex(g, source);
     const int64_t n = g->n;
     hipStream_t st = g->stream;
     const int2* wdst = reinterpret_cast<const int2*>(g->out_dst);
@@ -379,7 +380,7 @@ extern "C" int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist
     }
     GG_HIP(hipGetLastError());
     FinishTiming(g, stats, wt);
-    if (dist_out) GG_HIP(hipMemcpy(dist_out, dist, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (dist_out) CopyOutVertexI32(g, dist, dist_out, false);
     if (stats) {
       unsigned long long* acc = g->Scratch<unsigned long long>("sssp.reach", 2);
       GG_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(unsigned long long), st));

@@ -75,7 +75,8 @@ TUNE_BFS, TUNE_BATCH, TUNE_SSSP, TUNE_PR = 1, 2, 4, 8
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
-    "gg_relabel_by_degree", "gg_graph_copy_out", "gg_autotune", "gg_tuning_get", "gg_tuning_set",
+    "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
+    "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
 
@@ -110,6 +111,8 @@ def lib() -> C.CDLL:
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
         L.gg_relabel_by_degree.argtypes = [vp, C.POINTER(vp), C.POINTER(Stats)]
         L.gg_graph_copy_out.argtypes = [vp, _i64p, _i32p]
+        L.gg_graph_order_by_degree.argtypes = [vp, C.POINTER(Stats)]
+        L.gg_graph_vertex_order.argtypes = [vp, _i32p]
         L.gg_autotune.argtypes = [vp, C.c_int, _i32p, C.c_int, C.POINTER(Tuning)]
         L.gg_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
         L.gg_tuning_set.argtypes = [vp, C.POINTER(Tuning)]
@@ -142,9 +145,11 @@ def _p(a: np.ndarray, ct):
 class DeviceGraph:
     """A graph copied into HBM (int64 offsets, int32 ids, AoS weights)."""
 
-    def __init__(self, g: Graph, device: int = 0):
+    def __init__(self, g: Graph, device: int = 0, order_by_degree: bool = False):
         init(device)
         self.host = g
+        self.ordered = False
+        self.order_stats = None
         self.num_nodes = g.num_nodes
         self.weighted = g.weighted
         self.directed = g.directed
@@ -158,12 +163,31 @@ class DeviceGraph:
         _check(lib().gg_graph_upload(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32),
                                      int(g.weighted), int(g.directed), ioff, idst, C.byref(h)))
         self._h = h
+        if order_by_degree:
+            self.order_by_degree()
+
+    def order_by_degree(self) -> dict:
+        """gg_graph_order_by_degree: renumbers the device layout by descending
+        degree; calls keep taking and returning uploaded ids."""
+        st = Stats()
+        _check(lib().gg_graph_order_by_degree(self._h, C.byref(st)))
+        self.ordered = True
+        self.order_stats = st.as_dict()
+        return self.order_stats
+
+    def vertex_order(self) -> np.ndarray:
+        """new_id[v]: the device id of uploaded vertex v."""
+        out = np.empty(self.num_nodes, dtype=np.int32)
+        _check(lib().gg_graph_vertex_order(self._h, _p(out, C.c_int32)))
+        return out
 
     @classmethod
     def _adopt(cls, handle: C.c_void_p, num_nodes: int) -> "DeviceGraph":
         """Wraps a handle the library created (undirected, unweighted)."""
         self = cls.__new__(cls)
         self.host = None
+        self.ordered = False
+        self.order_stats = None
         self.num_nodes = num_nodes
         self.weighted = False
         self.directed = False

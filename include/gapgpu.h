@@ -174,8 +174,31 @@ int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats);
 int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats);
 
 /* Copies a handle's CSR back to host memory (offsets[n + 1] and dests, each
- * nullable): the layout gg_graph_upload takes. */
+ * nullable): the layout gg_graph_upload takes.  A degree-ordered handle
+ * (below) returns its lists in device ids. */
 int gg_graph_copy_out(const gg_graph* g, int64_t* offsets, int32_t* dests);
+
+/* Degree-ordered device layout (no reference counterpart: the reference
+ * keeps the builder's ids for every kernel but TC).  Renumbers the handle's
+ * vertices in place by descending (degree, id) -- the order of
+ * gg_relabel_by_degree -- so the lists, and every per-vertex array the
+ * kernels gather through them, keep hubs together.  The handle still speaks
+ * uploaded ids: roots and sources are mapped on the way in, and every
+ * per-vertex result (parents, depths, distances, scores, component labels)
+ * comes back indexed by, and holding, uploaded ids.  Unweighted lists are
+ * sorted under the new ids; weighted lists keep their uploaded order.
+ * Exactness: BFS depths, SSSP distances, component labels (the smallest
+ * uploaded id per component) and triangle counts are unchanged; BFS parents
+ * remain a valid BFS tree; PageRank and BC sum in the new list order
+ * (within tolerance of the uploaded order; bit-exact with the uploaded-order
+ * result of the relabelled graph).  gg_bfs_packed needs the uploaded order
+ * (GG_ENOTSUP).  Undirected graphs only (GG_ENOTSUP otherwise); calling it
+ * twice is GG_EINVAL.  stats->device_ms is the renumbering's device time. */
+int gg_graph_order_by_degree(gg_graph* g, gg_stats* stats);
+
+/* new_id[v] = device id of uploaded vertex v (the identity unless the
+ * handle is degree-ordered). */
+int gg_graph_vertex_order(gg_graph* g, int32_t* new_id);
 
 /* Per-graph tuning (no reference counterpart; the reference's knobs are the
  * -a/-b style alpha/beta defaults in bfs.cc:122-123 and sssp's -d delta).
