@@ -28,7 +28,9 @@
 // a vertex is active at one level per distinct depth).  A top-down level
 // after a bottom-up one first compacts the dense frontier into a list.
 // Depths are kept vertex-major as uint16 (64 per vertex, 128 bytes),
-// 0xFFFF = unreached.
+// written per level in whole 16-byte chunks by a wavefront together; a
+// slot is defined only where the vertex's seen mask has the root's bit, so
+// the array needs no initialisation.
 #pragma once
 
 #include <string>
@@ -105,17 +107,13 @@ __global__ void MsPublish(MsCounters* c, unsigned long long* mb, unsigned long l
 }
 
 __global__ __launch_bounds__(kBS) void MsInit(uint64_t* __restrict__ seen, uint64_t* __restrict__ f0,
-                                              uint64_t* __restrict__ f1, uint64_t* __restrict__ acc,
-                                              uint16_t* __restrict__ depth, int64_t n, MsCounters* c) {
-  const int64_t total = depth ? n * 64 : n;
-  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < total; i += int64_t(gridDim.x) * kBS) {
-    if (depth) depth[i] = kNoDepth;
-    if (i < n) {
-      seen[i] = 0;
-      f0[i] = 0;
-      f1[i] = 0;
-      acc[i] = 0;
-    }
+                                              uint64_t* __restrict__ f1, uint64_t* __restrict__ acc, int64_t n,
+                                              MsCounters* c) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS) {
+    seen[i] = 0;
+    f0[i] = 0;
+    f1[i] = 0;
+    acc[i] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x < kStripes) {
     const int t = threadIdx.x;
@@ -154,13 +152,39 @@ __global__ void MsRoots(const int32_t* __restrict__ roots, const int32_t* __rest
   }
 }
 
-// Records `level` as the depth of every new bit of vertex w.
-__device__ __forceinline__ void WriteDepths(uint16_t* depth, int32_t w, uint64_t bits, uint16_t level) {
-  if (!depth) return;
-  uint16_t* row = depth + int64_t(w) * 64;
-  while (bits) {
-    row[__builtin_ctzll(bits)] = level;
-    bits &= bits - 1;
+// Records `level` as the depth of the new bits nb of vertex w.  Called by
+// every lane of a wavefront together, each with its own (w, nb, prior);
+// lanes without a vertex pass nb = 0.  prior is the vertex's seen mask
+// before this level.  In step i, lane l serves 16-byte chunk l % 8 (roots
+// 8c .. 8c+7) of the row of the vertex held by lane 8i + l / 8, so a step
+// covers eight whole 128-byte rows.  A chunk with no earlier bits is
+// written without being read: slots of roots that have not reached the
+// vertex are undefined, and readers mask every row with seen.
+__device__ __forceinline__ void WaveWriteDepths(uint16_t* depth, int32_t w, uint64_t nb, uint64_t prior,
+                                                uint16_t level) {
+  const int lane = LaneId();
+  const int c = lane & 7;
+  const uint32_t fill = static_cast<uint32_t>(level) * 0x10001u;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int src = 8 * i + (lane >> 3);
+    const uint64_t b = __shfl(nb, src);
+    const uint64_t p = __shfl(prior, src);
+    const int32_t v = __shfl(w, src);
+    const uint32_t bits = static_cast<uint32_t>(b >> (8 * c)) & 0xFFu;
+    if (bits) {
+      uint4* q = reinterpret_cast<uint4*>(depth + int64_t(v) * 64) + c;
+      uint4 x = ((p >> (8 * c)) & 0xFFu) ? *q : make_uint4(~0u, ~0u, ~0u, ~0u);
+      uint32_t m[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        m[k] = (((bits >> (2 * k)) & 1u) ? 0x0000FFFFu : 0u) | (((bits >> (2 * k + 1)) & 1u) ? 0xFFFF0000u : 0u);
+      x.x = (x.x & ~m[0]) | (fill & m[0]);
+      x.y = (x.y & ~m[1]) | (fill & m[1]);
+      x.z = (x.z & ~m[2]) | (fill & m[2]);
+      x.w = (x.w & ~m[3]) | (fill & m[3]);
+      *q = x;
+    }
   }
 }
 
@@ -248,21 +272,22 @@ __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touc
     const int64_t i = base + threadIdx.x;
     bool take = false;
     int32_t w = 0;
+    uint64_t s = 0, nb = 0;
     if (i < nt) {
       w = touched[i];
-      const uint64_t s = seen[w];
-      const uint64_t nb = acc[w] & ~s;
+      s = seen[w];
+      nb = acc[w] & ~s;
       acc[w] = 0;
       if (nb) {
         seen[w] = s | nb;
         fnext[w] = nb;
-        WriteDepths(depth, w, nb, level);
         scout += static_cast<unsigned long long>(off[w + 1] - off[w]);
         if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(in_off[w + 1] - in_off[w]);
         lv |= nb;
         take = true;
       }
     }
+    if (depth) WaveWriteDepths(depth, w, nb, s, level);
     LdsQueuePush(lq, take, w, active, &c->active_tail);
   }
   LdsQueueFlush(lq, active, &c->active_tail);
@@ -358,18 +383,19 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
     }
     if (tid == 0) ex += static_cast<unsigned long long>(total);
     __syncthreads();
+    uint64_t nb = 0;
     if (light) {
-      const uint64_t nb = s_got[tid] & ~s;
+      nb = s_got[tid] & ~s;
       fnext[i] = nb;
       if (nb) {
         seen[i] = s | nb;
-        WriteDepths(depth, static_cast<int32_t>(i), nb, level);
         scout += static_cast<unsigned long long>(out_off[i + 1] - out_off[i]);
         if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(d);
         lv |= nb;
         awake++;
       }
     }
+    if (depth) WaveWriteDepths(depth, static_cast<int32_t>(i), nb, s, level);
     __syncthreads();
   }
   awake = BlockSum<kBS>(awake, s_red);
@@ -460,21 +486,27 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
                                                      MsCounters* c) {
   __shared__ unsigned long long s_red[kBS / kWave];
   unsigned long long awake = 0, scout = 0, fin = 0, lv = 0;
-  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < nheavy; i += int64_t(gridDim.x) * kBS) {
-    const int32_t v = heavy[i];
-    const uint64_t a = acc[v];
-    const uint64_t s = seen[v];
-    const uint64_t nb = a & ~s;
-    if (a) acc[v] = 0;
-    fnext[v] = nb;
-    if (nb) {
-      seen[v] = s | nb;
-      WriteDepths(depth, v, nb, level);
-      scout += static_cast<unsigned long long>(out_off[v + 1] - out_off[v]);
-      if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(in_off[v + 1] - in_off[v]);
-      lv |= nb;
-      awake++;
+  // Block-uniform trip count: the depth writer needs whole wavefronts.
+  for (int64_t base = int64_t(blockIdx.x) * kBS; base < nheavy; base += int64_t(gridDim.x) * kBS) {
+    const int64_t i = base + threadIdx.x;
+    int32_t v = 0;
+    uint64_t s = 0, nb = 0;
+    if (i < nheavy) {
+      v = heavy[i];
+      const uint64_t a = acc[v];
+      s = seen[v];
+      nb = a & ~s;
+      if (a) acc[v] = 0;
+      fnext[v] = nb;
+      if (nb) {
+        seen[v] = s | nb;
+        scout += static_cast<unsigned long long>(out_off[v + 1] - out_off[v]);
+        if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(in_off[v + 1] - in_off[v]);
+        lv |= nb;
+        awake++;
+      }
     }
+    if (depth) WaveWriteDepths(depth, v, nb, s, level);
   }
   awake = BlockSum<kBS>(awake, s_red);
   scout = BlockSum<kBS>(scout, s_red);
@@ -535,13 +567,14 @@ __global__ void MsZero2(unsigned long long* p) {
   if (threadIdx.x < 2) p[threadIdx.x] = 0;
 }
 
-// depth16[v][r] -> depth32[v] of root r (-1 where unreached); with new_id,
-// out is indexed by original ids.
-__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16,
+// depth16[v][r] -> depth32[v] of root r (-1 where seen has no bit r: the
+// slot is undefined there); with new_id, out is indexed by original ids.
+__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16, const uint64_t* __restrict__ seen,
                                                   const int32_t* __restrict__ new_id, int64_t n, int r,
                                                   int32_t* __restrict__ out) {
   for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
-    const uint16_t d = d16[(new_id ? int64_t(new_id[v]) : v) * 64 + r];
+    const int64_t dv = new_id ? int64_t(new_id[v]) : v;
+    const uint16_t d = (seen[dv] >> r) & 1 ? d16[dv * 64 + r] : kNoDepth;
     out[v] = d == kNoDepth ? -1 : d;
   }
 }
@@ -616,8 +649,7 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
     MailWait(g, seq, mail, kMailValues);
   };
 
-  Run(g, "msbfs_init", 32 * N + 128 * N, MsInit, dim3(GridFor(n * 64, kBS, g->cu_count * 16)), blk, seen, fa,
-      fb, acc, depth16, n, dc);
+  Run(g, "msbfs_init", 32 * N, MsInit, grid_n, blk, seen, fa, fb, acc, n, dc);
   Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), static_cast<const int32_t*>(droots),
       dbits, count, seen, fa, depth16, act_a, dc);
   launches += 2;

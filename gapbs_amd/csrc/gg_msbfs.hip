@@ -33,7 +33,7 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
     if (depths) {
       int32_t* d32 = g->Scratch<int32_t>("ms.depth_out", n);
       for (int r = 0; r < k; r++) {
-        MsDepthOut<<<grid_n, blk, 0, st>>>(depth16, g->new_id, n, r, d32);
+        MsDepthOut<<<grid_n, blk, 0, st>>>(depth16, t.seen, g->new_id, n, r, d32);
         GG_HIP(hipMemcpyAsync(depths + int64_t(r) * n, d32, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
       }
       GG_HIP(hipStreamSynchronize(st));
@@ -82,8 +82,9 @@ __global__ __launch_bounds__(kBS) void PkCheckParts(const int64_t* __restrict__ 
 }
 
 // depth16[v][j] -> out[R * part[i] + j * |part i| + (v - part[i])] for the
-// part i holding v, -1 where unreached.
-__global__ __launch_bounds__(kBS) void PkDepthOut(const uint16_t* __restrict__ d16, int64_t n,
+// part i holding v, -1 where unreached (no bit j in seen[v]).
+__global__ __launch_bounds__(kBS) void PkDepthOut(const uint16_t* __restrict__ d16,
+                                                  const uint64_t* __restrict__ seen, int64_t n,
                                                   const int64_t* __restrict__ part, int nparts, int R,
                                                   int32_t* __restrict__ out) {
   const int64_t total = n * R;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kBS) void PkDepthOut(const uint16_t* __restrict__ d
         hi = mid;
     }
     const int64_t size = part[lo + 1] - part[lo];
-    const uint16_t d = d16[v * 64 + j];
+    const uint16_t d = (seen[v] >> j) & 1 ? d16[v * 64 + j] : kNoDepth;
     out[R * part[lo] + j * size + (v - part[lo])] = d == kNoDepth ? -1 : d;
   }
 }
@@ -158,7 +159,8 @@ extern "C" int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const
     FinishTiming(g, stats, wt);
     if (depths) {
       int32_t* d32 = g->Scratch<int32_t>("pk.depth_out", n * R);
-      PkDepthOut<<<dim3(GridFor(n * R, kBS, g->cu_count * 16)), blk, 0, st>>>(depth16, n, dpart, nparts, R, d32);
+      PkDepthOut<<<dim3(GridFor(n * R, kBS, g->cu_count * 16)), blk, 0, st>>>(depth16, t.seen, n, dpart, nparts, R,
+                                                                                d32);
       GG_HIP(hipMemcpyAsync(depths, d32, sizeof(int32_t) * n * R, hipMemcpyDeviceToHost, st));
       GG_HIP(hipStreamSynchronize(st));
     }
