@@ -50,6 +50,10 @@ constexpr uint16_t kNoDepth = 0xFFFF;
 constexpr int64_t kHeavyEdges = 512;  // bottom-up: light tier below, heavy tier from here
 constexpr int64_t kHeavyChunk = 2048;  // heavy in-edges per wavefront work item
 constexpr int kUnroll = 4;             // gathers in flight per lane
+// Light tier: list positions scanned per round (a vertex leaves once it
+// has every bit it can still gain).
+constexpr int kRounds = 3;
+__constant__ const int kRoundEnd[kRounds] = {16, 64, static_cast<int>(kHeavyEdges)};
 
 struct MsCounters {
   int32_t active_tail;   // top-down: next level's active list
@@ -308,11 +312,12 @@ __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touc
 // only gain bits of live roots, so it is skipped when it has all of them,
 // and the heavy tier stops scanning once it has found them.
 
-// Light tier: vertices below kHeavyEdges in-edges, 256 per block.  The
-// in-edges of the tile's vertices that still need live bits are laid out as
-// one flat range (block prefix sum); each thread gathers kUnroll
-// neighbours' frontier masks per step, locating their owner by binary
-// search over the prefix, and ORs them into the owner's LDS slot.  Writes
+// Light tier: vertices below kHeavyEdges in-edges, 256 per block.  In each
+// round, the next list positions of the tile's vertices that still need
+// live bits are laid out as one flat range (block prefix sum); each thread
+// gathers kUnroll neighbours' frontier masks per step, locating their owner
+// by binary search over the prefix, and ORs them into the owner's LDS slot.
+// A vertex leaves after the round that gave it every needed bit.  Writes
 // fnext for every vertex without a pending live bit and every light one.
 __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
                                                   const int64_t* __restrict__ out_off,
@@ -345,44 +350,54 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
         }
       }
     }
-    s_beg[tid] = b;
     s_got[tid] = 0;
-    int total = 0;
-    const int rank = BlockExclusiveScan<kBS>(d, &total, s_scan);
-    s_pre[tid] = rank;
-    if (tid == 0) s_pre[kBS] = total;
-    __syncthreads();
-    for (int j0 = 0; j0 < total; j0 += kBS * kUnroll) {
-      int32_t nbr[kUnroll];
-      int own[kUnroll];
+    const uint64_t need = live & ~s;
+    // Rounds over list positions [kRoundEnd[r-1], kRoundEnd[r]): a vertex
+    // takes part in a round only while it still lacks a needed bit (lists
+    // are sorted, so in a degree-ordered layout the hubs come first).
+    for (int round = 0, r_lo = 0; round < kRounds; r_lo = kRoundEnd[round], round++) {
+      int take = 0;
+      if (light && r_lo < d && (round == 0 || (s_got[tid] & need) != need))
+        take = (d < kRoundEnd[round] ? d : kRoundEnd[round]) - r_lo;
+      s_beg[tid] = b + r_lo;
+      int total = 0;
+      const int rank = BlockExclusiveScan<kBS>(take, &total, s_scan);
+      if (total == 0) break;  // block-uniform: no vertex has entries left
+      s_pre[tid] = rank;
+      if (tid == 0) s_pre[kBS] = total;
+      __syncthreads();
+      for (int j0 = 0; j0 < total; j0 += kBS * kUnroll) {
+        int32_t nbr[kUnroll];
+        int own[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int j = j0 + u * kBS + tid;
-        own[u] = -1;
-        nbr[u] = 0;
-        if (j < total) {
-          int lo = 0, hi = kBS;  // s_pre[lo] <= j < s_pre[hi]
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[mid] <= j)
-              lo = mid;
-            else
-              hi = mid;
+        for (int u = 0; u < kUnroll; u++) {
+          const int j = j0 + u * kBS + tid;
+          own[u] = -1;
+          nbr[u] = 0;
+          if (j < total) {
+            int lo = 0, hi = kBS;  // s_pre[lo] <= j < s_pre[hi]
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_pre[mid] <= j)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            own[u] = lo;
+            nbr[u] = in_dst[s_beg[lo] + (j - s_pre[lo])];
           }
-          own[u] = lo;
-          nbr[u] = in_dst[s_beg[lo] + (j - s_pre[lo])];
         }
-      }
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        if (own[u] >= 0) {
-          const uint64_t f = frontier[nbr[u]];
-          if (f) atomicOr(&s_got[own[u]], static_cast<unsigned long long>(f));
+        for (int u = 0; u < kUnroll; u++) {
+          if (own[u] >= 0) {
+            const uint64_t f = frontier[nbr[u]];
+            if (f) atomicOr(&s_got[own[u]], static_cast<unsigned long long>(f));
+          }
         }
       }
+      if (tid == 0) ex += static_cast<unsigned long long>(total);
+      __syncthreads();
     }
-    if (tid == 0) ex += static_cast<unsigned long long>(total);
-    __syncthreads();
     uint64_t nb = 0;
     if (light) {
       nb = s_got[tid] & ~s;
