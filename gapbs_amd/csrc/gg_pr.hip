@@ -111,6 +111,23 @@ struct PrArgs {
   int32_t lo, hi;    // vertex ids updated by this launch: [lo, hi)
 };
 
+// Adds contrib[in_dst[k]] for k = k0, k0 + step, ... < e to acc, with
+// four independent gathers in flight per lane.
+template <class T, class A>
+__device__ __forceinline__ void GatherStrided(const PrArgs<T>& a, A& acc, int64_t k0, int64_t e, int step) {
+  int64_t k = k0;
+  for (; k + 3 * step < e; k += 4 * step) {
+    const int32_t v0 = a.in_dst[k], v1 = a.in_dst[k + step], v2 = a.in_dst[k + 2 * step],
+                  v3 = a.in_dst[k + 3 * step];
+    const T x0 = a.contrib_in[v0], x1 = a.contrib_in[v1], x2 = a.contrib_in[v2], x3 = a.contrib_in[v3];
+    acc.Add(x0);
+    acc.Add(x1);
+    acc.Add(x2);
+    acc.Add(x3);
+  }
+  for (; k < e; k += step) acc.Add(a.contrib_in[a.in_dst[k]]);
+}
+
 // Finishes vertex u: new score, next contribution, |change|.
 template <class T>
 __device__ __forceinline__ double Finish(const PrArgs<T>& a, int32_t u, T sum) {
@@ -169,7 +186,7 @@ __device__ __forceinline__ double GatherWavePart(const PrArgs<T>& a, const int32
     if (u < a.lo || u >= a.hi) continue;  // uniform per wavefront
     const int64_t b = a.in_off[u], e = a.in_off[u + 1];
     A acc = A::Zero();
-    for (int64_t k = b + lane; k < e; k += kWave) acc.Add(a.contrib_in[a.in_dst[k]]);
+    GatherStrided(a, acc, b + lane, e, kWave);
     acc = WaveMerge(acc);
     if (lane == 0) r += Finish(a, u, static_cast<T>(acc.Value()));
   }
@@ -188,7 +205,7 @@ __device__ __forceinline__ double GatherBlockPart(const PrArgs<T>& a, const int3
     if (u < a.lo || u >= a.hi) continue;  // uniform per block
     const int64_t b = a.in_off[u], e = a.in_off[u + 1];
     A acc = A::Zero();
-    for (int64_t k = b + threadIdx.x; k < e; k += kBS) acc.Add(a.contrib_in[a.in_dst[k]]);
+    GatherStrided(a, acc, b + threadIdx.x, e, kBS);
     acc = WaveMerge(acc);
     if (lane == 0) s_acc[wid] = acc;
     __syncthreads();
