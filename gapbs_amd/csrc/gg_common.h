@@ -105,6 +105,9 @@ struct gg_graph {
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for
   int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
   int64_t ms_heavy_chunks = 0;           // and their in-edge chunks
+  hipGraphExec_t ms_exec = nullptr;      // batched traversal: two captured levels, replayed
+  std::vector<uintptr_t> ms_exec_key;    // the buffers and arguments captured in it
+  int ms_exec_kernels = 0;               // kernels per replay
   gg::Workspace ws;
   void* pinned = nullptr;  // small pinned host block for counter read-back
   size_t pinned_bytes = 0;

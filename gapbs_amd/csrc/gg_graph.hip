@@ -346,6 +346,7 @@ int gg_graph_free(gg_graph* g) {
   if (!g) return GG_OK;
   return Guard([&] {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->ms_exec) (void)hipGraphExecDestroy(g->ms_exec);
     g->ws.Release();
     const size_t off_bytes = sizeof(int64_t) * (g->n + 1);
     const int width = g->weighted ? 2 : 1;

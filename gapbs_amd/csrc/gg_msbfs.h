@@ -33,6 +33,7 @@
 // the array needs no initialisation.
 #pragma once
 
+#include <cstdint>
 #include <string>
 #include <utility>
 #include <vector>
@@ -70,7 +71,26 @@ struct MsCounters {
   unsigned long long live[kStripes];      // OR of the new bits
 };
 
-enum { kMailActive, kMailScout, kMailExamined, kMailHeavyEx, kMailFinished, kMailLive, kMailValues };
+// Level state, kept on the device and advanced by MsStep after every level,
+// so no level kernel needs a decision from the host: each reads the state
+// and returns at once unless its direction runs now.  This lets a run of
+// levels be replayed from a captured graph without host round trips.
+struct MsLevel {
+  int64_t na;          // vertices with bits in the current frontier
+  int64_t na_prev;     // the same for the frontier the last level consumed
+  int64_t unfinished;  // in-edges of vertices some root has not reached
+  int64_t td, bu;      // levels run in each direction
+  int64_t examined;    // adjacency entries read so far
+  uint64_t live;       // roots with bits in the current frontier
+  int32_t level;       // the level about to run (1-based)
+  int32_t bottom_up;   // its direction
+  int32_t listed;      // the current frontier is listed in the active list
+  int32_t clear;       // to clear in the array just consumed: 0 nothing, 1 its listed entries, 2 all
+  int32_t done;        // no frontier left
+};
+
+// Mail posted by MsStep for the host-driven loop.
+enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailValues };
 
 __device__ __forceinline__ uint64_t WaveOr(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
@@ -91,22 +111,73 @@ __device__ __forceinline__ unsigned long long BlockOr(unsigned long long v, unsi
   return t;
 }
 
-__global__ void MsPublish(MsCounters* c, unsigned long long* mb, unsigned long long seq) {
+// Starts the level state after the roots (their distinct vertices are the
+// first active list).
+__global__ void MsLevelInit(MsCounters* c, MsLevel* L, int64_t unfinished, uint64_t full) {
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, s = 0, ex = 0, hx = 0, fin = 0, lv = 0;
-    for (int i = 0; i < kStripes; i++) {
-      a += c->awake[i];
-      s += c->scout[i];
-      ex += c->examined[i];
-      hx += c->heavy_ex[i];
-      fin += c->finished[i];
-      lv |= c->live[i];
-      c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->finished[i] = c->live[i] = 0;
-    }
-    unsigned long long vals[kMailValues] = {a + static_cast<unsigned long long>(c->active_tail), s, ex, hx,
-                                            fin, lv};
+    const int64_t na = c->active_tail;
     c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
-    MailPost(mb, vals, kMailValues, seq);
+    L->na = na;
+    L->na_prev = 0;
+    L->unfinished = unfinished;
+    L->td = L->bu = L->examined = 0;
+    L->live = full;
+    L->level = 1;
+    L->bottom_up = 0;  // level 1 is top-down whatever the roots' degrees
+    L->listed = 1;
+    L->clear = 0;
+    L->done = na == 0;
+  }
+}
+
+// After a level: folds the striped counters into the level state and picks
+// the next direction -- bottom-up while the new frontier's out-edges exceed
+// unfinished / alpha -- and what the next level must clear first (a
+// top-down level writes only the entries that gain bits).  With mb, posts
+// the state for the host-driven loop.  A no-op once done.
+__global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned long long* mb,
+                       unsigned long long seq) {
+  if (threadIdx.x == 0) {
+    if (!L->done) {
+      unsigned long long a = 0, s = 0, ex = 0, hx = 0, fin = 0, lv = 0;
+      for (int i = 0; i < kStripes; i++) {
+        a += c->awake[i];
+        s += c->scout[i];
+        ex += c->examined[i];
+        hx += c->heavy_ex[i];
+        fin += c->finished[i];
+        lv |= c->live[i];
+        c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->finished[i] = c->live[i] = 0;
+      }
+      const int64_t na_next = static_cast<int64_t>(a) + c->active_tail;
+      c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
+      const int was_bu = L->bottom_up;
+      if (was_bu)
+        L->bu++;
+      else
+        L->td++;
+      L->examined += static_cast<int64_t>(ex + hx);
+      L->unfinished -= static_cast<int64_t>(fin);
+      const int next_bu = static_cast<int64_t>(s) > L->unfinished / alpha;
+      L->clear = (!keep && na_next > 0 && !next_bu) ? (L->listed ? 1 : 2) : 0;
+      L->na_prev = L->na;
+      L->listed = !was_bu;
+      L->bottom_up = next_bu;
+      L->na = na_next;
+      L->live = lv;
+      L->level++;
+      L->done = na_next == 0;
+      if (mb) {
+        const unsigned long long vals[kMailValues] = {static_cast<unsigned long long>(na_next),
+                                                      static_cast<unsigned long long>(next_bu),
+                                                      static_cast<unsigned long long>(L->done), ex, hx,
+                                                      static_cast<unsigned long long>(L->level)};
+        MailPost(mb, vals, kMailValues, seq);
+      }
+    } else if (mb) {
+      const unsigned long long vals[kMailValues] = {0, 0, 1, 0, 0, static_cast<unsigned long long>(L->level)};
+      MailPost(mb, vals, kMailValues, seq);
+    }
   }
 }
 
@@ -222,9 +293,13 @@ struct TdVisit {
 __global__ __launch_bounds__(kBS) void MsTopDown(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                  const uint64_t* __restrict__ frontier,
                                                  const uint64_t* __restrict__ seen, uint64_t* acc,
-                                                 const int32_t* __restrict__ active, int32_t na,
-                                                 int32_t* touched, MsCounters* c, EdgeChunk* chunks,
-                                                 int64_t chunk_min) {
+                                                 const int32_t* __restrict__ active, const MsLevel* L,
+                                                 int32_t* touched, MsCounters* c, EdgeChunk* chunks) {
+  if (L->done || L->bottom_up) return;
+  const int64_t na = L->na;
+  // A small frontier gets every vertex with >= kExpandBS edges chunked
+  // across the grid; a large one only the real hubs.
+  const int64_t chunk_min = na <= 4096 ? kExpandBS : kChunkEdges;
   __shared__ ExpandShared sh;
   __shared__ LdsQueue<kQueueCap> lq;
   __shared__ unsigned long long s_red[kBS / kWave];
@@ -251,7 +326,9 @@ __global__ __launch_bounds__(kBS) void MsTopDown(const int64_t* __restrict__ off
 __global__ __launch_bounds__(kBS) void MsTopDownChunks(const int32_t* __restrict__ dst,
                                                        const uint64_t* __restrict__ frontier,
                                                        const uint64_t* __restrict__ seen, uint64_t* acc,
-                                                       int32_t* touched, MsCounters* c, const EdgeChunk* chunks) {
+                                                       int32_t* touched, const MsLevel* L, MsCounters* c,
+                                                       const EdgeChunk* chunks) {
+  if (L->done || L->bottom_up) return;
   __shared__ LdsQueue<kQueueCap> lq;
   LdsQueueInit(lq);
   TdVisit visit{dst, frontier, seen, acc, &lq, touched, &c->touched_tail};
@@ -263,10 +340,12 @@ __global__ __launch_bounds__(kBS) void MsTopDownChunks(const int32_t* __restrict
 // vertices that gained any.
 __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touched, const MsCounters* cc,
                                                 uint64_t* __restrict__ acc, uint64_t* __restrict__ seen,
-                                                uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level,
+                                                uint64_t* __restrict__ fnext, uint16_t* depth, const MsLevel* L,
                                                 const int64_t* __restrict__ off,
                                                 const int64_t* __restrict__ in_off, uint64_t full, int32_t* active,
                                                 MsCounters* c) {
+  if (L->done || L->bottom_up) return;
+  const uint16_t level = static_cast<uint16_t>(L->level);
   __shared__ LdsQueue<kQueueCap> lq;
   __shared__ unsigned long long s_red[kBS / kWave];
   LdsQueueInit(lq);
@@ -322,8 +401,11 @@ __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touc
 __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
                                                   const int64_t* __restrict__ out_off,
                                                   const uint64_t* __restrict__ frontier, uint64_t* __restrict__ seen,
-                                                  uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level, int64_t n, uint64_t live,
-                                                  uint64_t full, MsCounters* c) {
+                                                  uint64_t* __restrict__ fnext, uint16_t* depth, int64_t n,
+                                                  const MsLevel* L, uint64_t full, MsCounters* c) {
+  if (L->done || !L->bottom_up) return;
+  const uint16_t level = static_cast<uint16_t>(L->level);
+  const uint64_t live = L->live;
   __shared__ int64_t s_beg[kBS];
   __shared__ int s_pre[kBS + 1];
   __shared__ unsigned long long s_got[kBS];
@@ -452,9 +534,11 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunkBuild(const int64_t* __restri
 // acc[v] for MsHeavyFinish.
 __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__ in_dst,
                                                      const uint64_t* __restrict__ frontier,
-                                                     const uint64_t* __restrict__ seen, uint64_t* acc, uint64_t live,
+                                                     const uint64_t* __restrict__ seen, uint64_t* acc, const MsLevel* L,
                                                      const EdgeChunk* __restrict__ chunks, int32_t nch,
                                                      MsCounters* c) {
+  if (L->done || !L->bottom_up) return;
+  const uint64_t live = L->live;
   __shared__ unsigned long long s_red[kBS / kWave];
   const int lane = LaneId();
   const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
@@ -495,10 +579,12 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
 // Writes fnext for every heavy vertex.
 __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__ heavy, int32_t nheavy,
                                                      uint64_t* __restrict__ seen, uint64_t* __restrict__ acc,
-                                                     uint64_t* __restrict__ fnext, uint16_t* depth, uint16_t level,
+                                                     uint64_t* __restrict__ fnext, uint16_t* depth, const MsLevel* L,
                                                      const int64_t* __restrict__ out_off,
                                                      const int64_t* __restrict__ in_off, uint64_t full,
                                                      MsCounters* c) {
+  if (L->done || !L->bottom_up) return;
+  const uint16_t level = static_cast<uint16_t>(L->level);
   __shared__ unsigned long long s_red[kBS / kWave];
   unsigned long long awake = 0, scout = 0, fin = 0, lv = 0;
   // Block-uniform trip count: the depth writer needs whole wavefronts.
@@ -536,9 +622,11 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
   }
 }
 
-// Dense frontier -> list of its non-zero entries.
+// Dense frontier -> list, before a top-down level that follows a bottom-up
+// one.
 __global__ __launch_bounds__(kBS) void MsCompact(const uint64_t* __restrict__ frontier, int64_t n, int32_t* out,
-                                                 MsCounters* c) {
+                                                 const MsLevel* L, MsCounters* c) {
+  if (L->done || L->bottom_up || L->listed) return;
   __shared__ LdsQueue<kQueueCap> lq;
   LdsQueueInit(lq);
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
@@ -548,13 +636,18 @@ __global__ __launch_bounds__(kBS) void MsCompact(const uint64_t* __restrict__ fr
   LdsQueueFlush(lq, out, &c->compact_tail);
 }
 
+// The clears MsStep asked for (L->clear) in the array the last level
+// consumed; with L null, MsClearAll clears unconditionally.
 __global__ __launch_bounds__(kBS) void MsClearList(uint64_t* __restrict__ frontier,
-                                                   const int32_t* __restrict__ active, int32_t na) {
+                                                   const int32_t* __restrict__ active, const MsLevel* L) {
+  if (L->clear != 1) return;
+  const int64_t na = L->na_prev;
   for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < na; i += int64_t(gridDim.x) * kBS)
     frontier[active[i]] = 0;
 }
 
-__global__ __launch_bounds__(kBS) void MsClearAll(uint64_t* __restrict__ frontier, int64_t n) {
+__global__ __launch_bounds__(kBS) void MsClearAll(uint64_t* __restrict__ frontier, int64_t n, const MsLevel* L) {
+  if (L && L->clear != 2) return;
   for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS)
     frontier[i] = 0;
 }
@@ -598,7 +691,80 @@ struct MsTraversal {
   uint64_t* seen = nullptr;          // final visit masks
   std::vector<uint64_t*> frontiers;  // keep mode: frontiers[L] holds the bits that arrived at level L
   int64_t td = 0, bu = 0, examined = 0, launches = 0;
+  bool graph = false;                // levels replayed from a captured graph
 };
+
+// Device buffers and fixed arguments of one traversal.
+struct MsBuffers {
+  MsCounters* dc;
+  MsLevel* L;
+  const int32_t* heavy;
+  const EdgeChunk* hchunks;
+  int32_t nheavy, nhchunks;
+  uint64_t* seen;
+  uint64_t* acc;
+  int32_t* touched;
+  EdgeChunk* chunks;  // null: no chunking (no vertex reaches kExpandBS edges)
+  uint16_t* depth16;
+  uint64_t full;
+};
+
+struct MsLaunched {
+  int td = -1, bu = -1, heavy = -1;  // profile records to credit once the counts are known
+  int kernels = 0;
+};
+
+// Enqueues one level, fcur -> fnext (act lists fcur when it is listed;
+// act_next receives a top-down level's new list): the top-down part, the
+// bottom-up part and the compaction before a top-down level, each only
+// when asked for.  Every kernel also checks the level state, so a captured
+// graph can hold all of them.
+inline MsLaunched MsLaunchLevel(gg_graph* g, const MsBuffers& b, const uint64_t* fcur, uint64_t* fnext,
+                                int32_t* act, int32_t* act_next, bool td, bool bu, bool compact) {
+  MsLaunched r;
+  const int64_t n = g->n;
+  const dim3 blk(kBS), grid_n(GridFor(n, kBS, g->cu_count * 16));
+  const double N = static_cast<double>(n);
+  if (compact) {
+    Run(g, "msbfs_compact", 8 * N, MsCompact, dim3(GridFor(n, kBS, g->cu_count * 4)), blk, fcur, n, act,
+        static_cast<const MsLevel*>(b.L), b.dc);
+    r.kernels++;
+  }
+  if (td) {
+    // 4 B per edge read, the neighbour's 8 B seen mask and 8 B accumulator.
+    r.td = Run(g, "msbfs_top_down", 0, MsTopDown, dim3(g->cu_count * 8), blk,
+               static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), fcur,
+               static_cast<const uint64_t*>(b.seen), b.acc, static_cast<const int32_t*>(act),
+               static_cast<const MsLevel*>(b.L), b.touched, b.dc, b.chunks);
+    r.kernels++;
+    if (b.chunks) {
+      Run(g, "msbfs_top_down_chunks", 0, MsTopDownChunks, dim3(g->cu_count * 4), blk,
+          static_cast<const int32_t*>(g->out_dst), fcur, static_cast<const uint64_t*>(b.seen), b.acc, b.touched,
+          static_cast<const MsLevel*>(b.L), b.dc, static_cast<const EdgeChunk*>(b.chunks));
+      r.kernels++;
+    }
+    Run(g, "msbfs_update", 0, MsUpdate, dim3(g->cu_count * 4), blk, static_cast<const int32_t*>(b.touched),
+        static_cast<const MsCounters*>(b.dc), b.acc, b.seen, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
+        static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, act_next, b.dc);
+    r.kernels++;
+  }
+  if (bu) {
+    // Per vertex: seen (8 B) and fnext (8 B); per in-edge gathered: 4 B
+    // and the neighbour's 8 B frontier mask (credited by the caller).
+    r.bu = Run(g, "msbfs_bottom_up", 16 * N, MsBottomUp, grid_n, blk, static_cast<const int64_t*>(g->in_off),
+               static_cast<const int32_t*>(g->in_dst), static_cast<const int64_t*>(g->out_off), fcur, b.seen,
+               fnext, b.depth16, n, static_cast<const MsLevel*>(b.L), b.full, b.dc);
+    r.heavy = Run(g, "msbfs_bottom_up_heavy", 0, MsHeavyChunks,
+                  dim3(GridFor(int64_t(b.nhchunks) * kWave, kBS, g->cu_count * 16)), blk,
+                  static_cast<const int32_t*>(g->in_dst), fcur, static_cast<const uint64_t*>(b.seen), b.acc,
+                  static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc);
+    Run(g, "msbfs_heavy_finish", 32.0 * b.nheavy, MsHeavyFinish, dim3(GridFor(b.nheavy, kBS, g->cu_count * 4)), blk,
+        b.heavy, b.nheavy, b.seen, b.acc, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
+        static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, b.dc);
+    r.kernels += 3;
+  }
+  return r;
+}
 
 // Runs the level loop on g->stream from the `count` roots in droots
 // (device memory, already validated).  Root i takes bit dbits[i] (device
@@ -608,6 +774,13 @@ struct MsTraversal {
 // every level's frontier array is kept for the caller (workspace arrays
 // named ms.level.<L>, valid until the next call).  The caller has set the
 // device and started timing.
+//
+// Two drivers share the kernels and the device level state: the host loop
+// (keep mode, or per-kernel profiling on) enqueues one level, waits for
+// MsStep's mail and enqueues the next; otherwise two levels (one per
+// frontier-buffer parity) are captured once into a hipGraph, cached in the
+// handle, and replayed until the state says done, with one host read of
+// the state per replay.
 inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t* dbits, int count, uint64_t full,
                               int alpha, uint16_t* depth16, bool keep) {
   MsTraversal res;
@@ -615,6 +788,7 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   hipStream_t st = g->stream;
   const dim3 blk(kBS);
   MsCounters* dc = g->Scratch<MsCounters>("ms.ctr", 1);
+  MsLevel* dl = g->Scratch<MsLevel>("ms.level_state", 1);
   int32_t* heavy = g->Scratch<int32_t>("ms.heavy", n + 1);
   // Heavy vertices and their in-edge chunks depend only on the graph.
   const int64_t max_chunks = g->in_entries / kHeavyChunk + g->in_entries / kHeavyEdges + 1;
@@ -636,8 +810,6 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
     g->ms_heavy = nh;
     g->ms_heavy_chunks = nc;
   }
-  const int32_t nheavy = static_cast<int32_t>(g->ms_heavy);
-  const int32_t nhchunks = static_cast<int32_t>(g->ms_heavy_chunks);
   uint64_t* seen = g->Scratch<uint64_t>("ms.seen", n);
   // keep: one frontier array per level, else two alternating ones.
   auto level_array = [&](int level) {
@@ -653,126 +825,133 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   int32_t* touched = g->Scratch<int32_t>("ms.touched", n + 64);
   const bool use_chunks = g->max_degree >= kExpandBS;
   EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("ms.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+  const MsBuffers b{dc,   dl,  heavy,   hchunks, static_cast<int32_t>(g->ms_heavy), static_cast<int32_t>(g->ms_heavy_chunks),
+                    seen, acc, touched, chunks,  depth16,                            full};
   const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
   const double N = static_cast<double>(n);
   int64_t& launches = res.launches;
   unsigned long long mail[kMailValues];
-  auto publish = [&]() {
-    const unsigned long long seq = MailNext(g);
-    Run(g, "msbfs_publish", 0, MsPublish, dim3(1), dim3(64), dc, g->mail_dev, seq);
-    launches++;
-    MailWait(g, seq, mail, kMailValues);
-  };
 
   Run(g, "msbfs_init", 32 * N, MsInit, grid_n, blk, seen, fa, fb, acc, n, dc);
-  Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), static_cast<const int32_t*>(droots),
-      dbits, count, seen, fa, depth16, act_a, dc);
-  launches += 2;
-  publish();
-  int64_t na = static_cast<int64_t>(mail[kMailActive]);
-  int64_t scout = 0;  // level 1 is top-down whatever the roots' degrees
-  int64_t unfinished = g->in_entries;
-  uint64_t* fcur = fa;
-  uint64_t* fnext = fb;
-  int32_t* act = act_a;
-  int32_t* act_next = act_b;
-  // Invariants at the top of a level: fcur holds the last level's bits
-  // (listed in act[0, na) when `listed`); fnext is all zero unless the
-  // level is bottom-up (which overwrites every entry).
-  bool listed = true;
-  bool bottom_up = false;
-  uint64_t live = full;  // roots with bits in fcur
-  int64_t& td = res.td;
-  int64_t& bu = res.bu;
-  int64_t& examined = res.examined;
-  for (int level = 1; na > 0; level++) {
-    GG_REQUIRE(level < 65535, GG_ENOTSUP, "more than 65534 levels");
-    const uint16_t lv = static_cast<uint16_t>(level);
-    if (keep) {
-      // A fresh array per level; a top-down level writes only the entries
-      // that gain bits, so it starts from zeros.
-      fnext = level_array(level);
-      res.frontiers.push_back(fnext);
-      if (!bottom_up) {
-        Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fnext, n);
+  Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), droots, dbits, count, seen, fa, depth16,
+      act_a, dc);
+  Run(g, "msbfs_level_init", 0, MsLevelInit, dim3(1), dim3(64), dc, dl, g->in_entries, full);
+  launches += 3;
+
+  MsLevel hl;
+  if (!keep && !g->profile) {
+    // Graph-replayed levels.
+    const std::vector<uintptr_t> key = {
+        reinterpret_cast<uintptr_t>(fa),      reinterpret_cast<uintptr_t>(fb),      reinterpret_cast<uintptr_t>(act_a),
+        reinterpret_cast<uintptr_t>(act_b),   reinterpret_cast<uintptr_t>(dc),      reinterpret_cast<uintptr_t>(dl),
+        reinterpret_cast<uintptr_t>(seen),    reinterpret_cast<uintptr_t>(acc),     reinterpret_cast<uintptr_t>(touched),
+        reinterpret_cast<uintptr_t>(chunks),  reinterpret_cast<uintptr_t>(hchunks), reinterpret_cast<uintptr_t>(heavy),
+        reinterpret_cast<uintptr_t>(depth16), static_cast<uintptr_t>(full),         static_cast<uintptr_t>(alpha),
+        static_cast<uintptr_t>(g->ms_heavy),  static_cast<uintptr_t>(g->ms_heavy_chunks)};
+    int per_replay = 0;
+    if (!g->ms_exec || g->ms_exec_key != key) {
+      if (g->ms_exec) {
+        (void)hipGraphExecDestroy(g->ms_exec);
+        g->ms_exec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int half = 0; half < 2; half++) {
+          uint64_t* fc = half ? fb : fa;
+          uint64_t* fn = half ? fa : fb;
+          int32_t* ac = half ? act_b : act_a;
+          int32_t* an = half ? act_a : act_b;
+          per_replay += MsLaunchLevel(g, b, fc, fn, ac, an, true, true, true).kernels;
+          hipLaunchKernelGGL(MsStep, dim3(1), dim3(64), 0, st, dc, dl, alpha, 0, static_cast<unsigned long long*>(nullptr),
+                             0ull);
+          hipLaunchKernelGGL(MsClearList, dim3(g->cu_count * 8), blk, 0, st, fc, static_cast<const int32_t*>(ac),
+                             static_cast<const MsLevel*>(dl));
+          hipLaunchKernelGGL(MsClearAll, grid_n, blk, 0, st, fc, n, static_cast<const MsLevel*>(dl));
+          per_replay += 3;
+        }
+        GG_HIP(hipGetLastError());
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      GG_HIP(hipStreamEndCapture(st, &graph));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      GG_HIP(e);
+      g->ms_exec = exec;
+      g->ms_exec_key = key;
+      g->ms_exec_kernels = per_replay;
+    }
+    per_replay = g->ms_exec_kernels;
+    res.graph = true;
+    for (;;) {
+      GG_HIP(hipGraphLaunch(g->ms_exec, st));
+      launches += per_replay;
+      GG_HIP(hipMemcpyAsync(&hl, dl, sizeof(hl), hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      if (hl.done) break;
+      GG_REQUIRE(hl.level < 65534, GG_ENOTSUP, "more than 65533 levels");
+    }
+  } else {
+    // Host-driven levels.
+    GG_HIP(hipMemcpyAsync(&hl, dl, sizeof(hl), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    int64_t na = hl.na;
+    bool listed = true, bottom_up = false;
+    uint64_t* fcur = fa;
+    uint64_t* fnext = fb;
+    int32_t* act = act_a;
+    int32_t* act_next = act_b;
+    for (int level = 1; na > 0; level++) {
+      GG_REQUIRE(level < 65535, GG_ENOTSUP, "more than 65534 levels");
+      if (keep) {
+        // A fresh array per level; a top-down level writes only the entries
+        // that gain bits, so it starts from zeros.
+        fnext = level_array(level);
+        res.frontiers.push_back(fnext);
+        if (!bottom_up) {
+          Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fnext, n, static_cast<const MsLevel*>(nullptr));
+          launches++;
+        }
+      }
+      const MsLaunched r = MsLaunchLevel(g, b, fcur, fnext, act, act_next, !bottom_up, bottom_up,
+                                         !bottom_up && !listed);
+      launches += r.kernels;
+      const unsigned long long seq = MailNext(g);
+      Run(g, "msbfs_step", 0, MsStep, dim3(1), dim3(64), dc, dl, alpha, keep ? 1 : 0, g->mail_dev, seq);
+      launches++;
+      MailWait(g, seq, mail, kMailValues);
+      if (r.td >= 0) CreditBytes(g, r.td, 20.0 * mail[kMailExamined]);
+      if (r.bu >= 0) CreditBytes(g, r.bu, 12.0 * mail[kMailExamined]);
+      if (r.heavy >= 0) CreditBytes(g, r.heavy, 12.0 * mail[kMailHeavyEx]);
+      const int64_t na_next = static_cast<int64_t>(mail[kMailNa]);
+      const bool next_bu = mail[kMailBottomUp] != 0;
+      // The clears MsStep chose for the array just consumed.
+      if (!keep && na_next > 0 && !next_bu) {
+        if (listed)
+          Run(g, "msbfs_clear", 12.0 * na, MsClearList, dim3(GridFor(na, kBS, g->cu_count * 8)), blk, fcur,
+              static_cast<const int32_t*>(act), static_cast<const MsLevel*>(dl));
+        else
+          Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fcur, n, static_cast<const MsLevel*>(dl));
         launches++;
       }
+      listed = !bottom_up;
+      bottom_up = next_bu;
+      na = na_next;
+      std::swap(fcur, fnext);
+      std::swap(act, act_next);
     }
-    if (bottom_up) {
-      // Per vertex: seen (8 B) and fnext (8 B); per in-edge gathered: 4 B
-      // and the neighbour's 8 B frontier mask.
-      const int idx = Run(g, "msbfs_bottom_up", 16 * N, MsBottomUp, grid_n, blk,
-                          static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
-                          static_cast<const int64_t*>(g->out_off), static_cast<const uint64_t*>(fcur), seen,
-                          fnext, depth16, lv, n, live, full, dc);
-      const int hidx = Run(g, "msbfs_bottom_up_heavy", 0, MsHeavyChunks,
-                           dim3(GridFor(int64_t(nhchunks) * kWave, kBS, g->cu_count * 16)), blk,
-                           static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(fcur),
-                           static_cast<const uint64_t*>(seen), acc, live,
-                           static_cast<const EdgeChunk*>(hchunks), nhchunks, dc);
-      Run(g, "msbfs_heavy_finish", 32.0 * nheavy, MsHeavyFinish, dim3(GridFor(nheavy, kBS, g->cu_count * 4)),
-          blk, static_cast<const int32_t*>(heavy), nheavy, seen, acc, fnext, depth16, lv,
-          static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), full, dc);
-      launches += 3;
-      bu++;
-      publish();
-      CreditBytes(g, idx, 12.0 * mail[kMailExamined]);
-      CreditBytes(g, hidx, 12.0 * mail[kMailHeavyEx]);
-    } else {
-      if (!listed) {
-        Run(g, "msbfs_compact", 8 * N, MsCompact, dim3(GridFor(n, kBS, g->cu_count * 4)), blk,
-            static_cast<const uint64_t*>(fcur), n, act, dc);
-        launches++;
-        listed = true;
-      }
-      const int64_t chunk_min = na <= 4096 ? kExpandBS : kChunkEdges;
-      const int idx = Run(g, "msbfs_top_down", 24.0 * na, MsTopDown, dim3(GridFor(na, kBS, g->cu_count * 8)),
-                          blk, static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
-                          static_cast<const uint64_t*>(fcur), static_cast<const uint64_t*>(seen), acc,
-                          static_cast<const int32_t*>(act), static_cast<int32_t>(na), touched, dc,
-                          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
-      launches++;
-      if (use_chunks) {
-        Run(g, "msbfs_top_down_chunks", 0, MsTopDownChunks, dim3(g->cu_count * 4), blk,
-            static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(fcur),
-            static_cast<const uint64_t*>(seen), acc, touched, dc, static_cast<const EdgeChunk*>(chunks));
-        launches++;
-      }
-      Run(g, "msbfs_update", 0, MsUpdate, dim3(g->cu_count * 4), blk, static_cast<const int32_t*>(touched),
-          static_cast<const MsCounters*>(dc), acc, seen, fnext, depth16, lv,
-          static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), full, act_next, dc);
-      launches++;
-      td++;
-      publish();
-      // 4 B per edge read, the neighbour's 8 B seen mask and 8 B accumulator.
-      CreditBytes(g, idx, 20.0 * mail[kMailExamined]);
-    }
-    examined += static_cast<int64_t>(mail[kMailExamined] + mail[kMailHeavyEx]);
-    const int64_t na_next = static_cast<int64_t>(mail[kMailActive]);
-    scout = static_cast<int64_t>(mail[kMailScout]);
-    unfinished -= static_cast<int64_t>(mail[kMailFinished]);
-    live = mail[kMailLive];
-    const bool next_bu = scout > unfinished / alpha;
-    // A top-down level writes only the entries that gain bits, so the
-    // array it will write (fcur, just consumed) must be cleared first.
-    if (!keep && na_next > 0 && !next_bu) {
-      if (listed) {
-        Run(g, "msbfs_clear", 12.0 * na, MsClearList, dim3(GridFor(na, kBS, g->cu_count * 8)), blk, fcur,
-            static_cast<const int32_t*>(act), static_cast<int32_t>(na));
-      } else {
-        Run(g, "msbfs_clear", 8 * N, MsClearAll, grid_n, blk, fcur, n);
-      }
-      launches++;
-    }
-    listed = !bottom_up;
-    bottom_up = next_bu;
-    na = na_next;
-    std::swap(fcur, fnext);
-    std::swap(act, act_next);
+    GG_HIP(hipMemcpyAsync(&hl, dl, sizeof(hl), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
   }
+  res.td = hl.td;
+  res.bu = hl.bu;
+  res.examined = hl.examined;
   res.seen = seen;
   return res;
-
 }
 
 }  // namespace
