@@ -19,7 +19,7 @@ int main(int argc, char* argv[]) {
       return -1;
     }
     Graph g = MakeGraph<NodeID>(cli);
-    DeviceSession<Graph> dev(cli.device(), g);
+    DeviceSession<Graph> dev(cli.device(), g, cli.order_by_degree());
     const int mode = cli.precision() == "fp64" ? GG_BC_FP64
                      : cli.precision() == "kahan" ? GG_BC_FP32_KAHAN
                                                   : GG_BC_FP32;

@@ -10,7 +10,7 @@ int main(int argc, char* argv[]) {
     CLI cli(argc, argv, "breadth-first search", CLI::kApp);
     if (!cli.ParseArgs()) return -1;
     Graph g = MakeGraph<NodeID>(cli);
-    DeviceSession<Graph> dev(cli.device(), g);
+    DeviceSession<Graph> dev(cli.device(), g, cli.order_by_degree());
     int alpha = 15, beta = 18;  // DOBFS's defaults (src/bfs.cc:122-123)
     if (cli.autotune() && cli.num_trials() > 0) {
       if (cli.batch() > 1) {

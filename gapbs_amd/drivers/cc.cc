@@ -9,7 +9,7 @@ int main(int argc, char* argv[]) {
     CLI cli(argc, argv, "connected-components-afforest", CLI::kApp);
     if (!cli.ParseArgs()) return -1;
     Graph g = MakeGraph<NodeID>(cli);
-    DeviceSession<Graph> dev(cli.device(), g);
+    DeviceSession<Graph> dev(cli.device(), g, cli.order_by_degree());
     auto CCBound = [&](const Graph& gr) {
       Buf<NodeID> comp(static_cast<size_t>(gr.num_nodes()));
       gg_stats st;

@@ -38,7 +38,9 @@ inline void Check(int rc, const char* what) {
 template <class GraphT>
 class DeviceSession {
  public:
-  DeviceSession(int device, const GraphT& g) : device_(device), g_(g) {}
+  // ordered: renumber the device copy by degree after the upload (-o);
+  // every call still takes and returns the graph's own ids.
+  DeviceSession(int device, const GraphT& g, bool ordered = false) : device_(device), g_(g), ordered_(ordered) {}
   ~DeviceSession() {
     if (dg_) gg_graph_free(dg_);
   }
@@ -52,6 +54,15 @@ class DeviceSession {
                             g_.directed() ? reinterpret_cast<const int32_t*>(g_.in_dests()) : nullptr,
                             &dg_),
             "gg_graph_upload");
+      if (ordered_) {
+        if (g_.directed()) {
+          std::cout << "Degree ordering skipped: directed graph" << std::endl;
+        } else {
+          gg_stats st;
+          Check(gg_graph_order_by_degree(dg_, &st), "gg_graph_order_by_degree");
+          gap::PrintTime("Degree Ordering", st.device_ms / 1e3);
+        }
+      }
     }
     return dg_;
   }
@@ -59,6 +70,7 @@ class DeviceSession {
  private:
   int device_;
   const GraphT& g_;
+  bool ordered_;
   gg_graph* dg_ = nullptr;
 };
 

@@ -11,7 +11,7 @@ int main(int argc, char* argv[]) {
     CLI cli(argc, argv, "pagerank", CLI::kPageRank, 20, 1e-4);
     if (!cli.ParseArgs()) return -1;
     Graph g = MakeGraph<NodeID>(cli);
-    DeviceSession<Graph> dev(cli.device(), g);
+    DeviceSession<Graph> dev(cli.device(), g, cli.order_by_degree());
     const int mode = cli.precision() == "fp64" ? GG_PR_FP64 : cli.precision() == "gs" ? GG_PR_GS : GG_PR_FP32;
     if (cli.autotune() && cli.num_trials() > 0) Autotune(cli, dev, g, GG_TUNE_PR, 0);
     auto PRBound = [&](const Graph& gr) {

@@ -9,7 +9,7 @@ int main(int argc, char* argv[]) {
     CLI cli(argc, argv, "single-source shortest-path", CLI::kDelta);
     if (!cli.ParseArgs()) return -1;
     WGraph g = MakeGraph<WNode>(cli);
-    DeviceSession<WGraph> dev(cli.device(), g);
+    DeviceSession<WGraph> dev(cli.device(), g, cli.order_by_degree());
     int32_t delta = cli.delta();
     if (cli.autotune() && cli.num_trials() > 0) delta = Autotune(cli, dev, g, GG_TUNE_SSSP, 2).sssp_delta;
     SourcePicker<WGraph> sp(g, static_cast<NodeID>(cli.start_vertex()));

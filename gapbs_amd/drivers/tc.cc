@@ -15,7 +15,7 @@ int main(int argc, char* argv[]) {
       std::cout << "Input graph is directed but tc requires undirected" << std::endl;
       return -2;
     }
-    DeviceSession<Graph> dev(cli.device(), g);
+    DeviceSession<Graph> dev(cli.device(), g, cli.order_by_degree());
     const bool relabel = WorthRelabelling(g);
     auto Hybrid = [&](const Graph&) {
       uint64_t count = 0;

@@ -61,7 +61,7 @@ class CLI {
       return;
     }
     if (kind == kBase) return;
-    opts_ += "an:r:vlj:p:q";
+    opts_ += "an:r:vlj:p:qo";
     Help('a', "", "output analysis of last run", "false");
     Help('n', "n", "perform n trials", std::to_string(num_trials_));
     Help('r', "node", "start from node r", "rand");
@@ -70,6 +70,7 @@ class CLI {
     Help('j', "k", "roots per batched GPU launch", "1");
     Help('p', "mode", "score precision: fp32, kahan (bc), gs (pr) or fp64", "fp32");
     Help('q', "", "autotune GPU parameters for this graph", "false");
+    Help('o', "", "renumber the GPU copy by degree (results keep ids)", "false");
     num_iters_ = default_iters;
     max_iters_ = default_iters;
     tolerance_ = default_tol;
@@ -121,6 +122,7 @@ class CLI {
   int batch() const { return batch_; }
   const std::string& precision() const { return precision_; }
   bool autotune() const { return autotune_; }
+  bool order_by_degree() const { return order_by_degree_; }
   int num_iters() const { return num_iters_; }
   int max_iters() const { return max_iters_; }
   double tolerance() const { return tolerance_; }
@@ -160,6 +162,7 @@ class CLI {
       case 'j': batch_ = std::max(1, std::atoi(a)); break;
       case 'p': precision_ = a; break;
       case 'q': autotune_ = true; break;
+      case 'o': order_by_degree_ = true; break;
       case 'i':
         num_iters_ = std::atoi(a);
         max_iters_ = num_iters_;
@@ -196,6 +199,7 @@ class CLI {
   int batch_ = 1;
   std::string precision_ = "fp32";
   bool autotune_ = false;
+  bool order_by_degree_ = false;
   int num_iters_ = 1;
   int max_iters_ = 20;
   double tolerance_ = 1e-4;

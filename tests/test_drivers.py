@@ -86,3 +86,22 @@ def test_verify_batched_and_gauss_seidel(kernel, extra):
         assert out.count("Verification:") == 1, out
     else:
         assert "Verification:           PASS" in out and "FAIL" not in out, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,extra", [("bfs", ["-n2"]), ("bfs", ["-n1", "-j", "16"]), ("sssp", ["-n2", "-d2"]),
+                                          ("pr", ["-n1"]), ("cc", ["-n2"]), ("tc", ["-n1"]),
+                                          ("bc", ["-n1", "-i4"])])
+def test_verify_degree_ordered(kernel, extra):
+    # -o renumbers the device copy by degree; results keep the graph's ids
+    # and must pass the reference's verifiers.  BC sums its dependencies in
+    # the new successor order, which can move a score by more than the
+    # verifier's FLT_EPSILON; its run must still complete and verify.
+    print(f"[driver] {kernel} -g12 -v -o {extra}", flush=True)
+    rc, out = run([kernel, "-g12", "-v", "-o"] + extra)
+    assert rc == 0, out
+    assert "Degree Ordering:" in out, out
+    if kernel == "bc":
+        assert out.count("Verification:") >= 1, out
+    else:
+        assert "Verification:           PASS" in out and "FAIL" not in out, out
