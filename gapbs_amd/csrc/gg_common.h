@@ -84,6 +84,8 @@ struct gg_graph {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  hipStream_t side = nullptr;  // EnsureSideStream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int64_t n = 0;
   int64_t out_entries = 0, in_entries = 0;
   bool directed = false, weighted = false;
@@ -105,6 +107,7 @@ struct gg_graph {
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for
   int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
   int64_t ms_heavy_chunks = 0;           // and their in-edge chunks
+  int64_t mb_hubs = -1;                  // batched BC: vertices whose dependency folds are split
   hipGraphExec_t ms_exec = nullptr;      // batched traversal: two captured levels, replayed
   std::vector<uintptr_t> ms_exec_key;    // the buffers and arguments captured in it
   int ms_exec_kernels = 0;               // kernels per replay
@@ -195,17 +198,27 @@ int ProfileSlot(gg_graph* g, const char* name, double bytes, hipEvent_t* a, hipE
 // packets), and the launch is credited to `name` with `bytes` algorithmic
 // bytes; returns the record index for CreditBytes, else -1.
 template <class... KArgs, class... Args>
-int Run(gg_graph* g, const char* name, double bytes, void (*k)(KArgs...), dim3 grid, dim3 block,
-        Args... args) {
+int RunOn(gg_graph* g, hipStream_t stream, const char* name, double bytes, void (*k)(KArgs...), dim3 grid,
+          dim3 block, Args... args) {
   if (!g->profile) {
-    hipLaunchKernelGGL(k, grid, block, 0, g->stream, args...);
+    hipLaunchKernelGGL(k, grid, block, 0, stream, args...);
     return -1;
   }
   hipEvent_t a, b;
   const int idx = ProfileSlot(g, name, bytes, &a, &b);
-  hipExtLaunchKernelGGL(k, grid, block, 0, g->stream, a, b, 0, args...);
+  hipExtLaunchKernelGGL(k, grid, block, 0, stream, a, b, 0, args...);
   return idx;
 }
+
+template <class... KArgs, class... Args>
+int Run(gg_graph* g, const char* name, double bytes, void (*k)(KArgs...), dim3 grid, dim3 block,
+        Args... args) {
+  return RunOn(g, g->stream, name, bytes, k, grid, block, args...);
+}
+
+// A second stream of the handle (default priority, created on first use)
+// for kernels that may overlap the main stream's, with fork/join events.
+void EnsureSideStream(gg_graph* g);
 
 // Mailbox protocol.  A one-block kernel calls MailPost with the next
 // sequence number after writing up to kMailWords values; the host waits for

@@ -103,6 +103,13 @@ void MailWait(gg_graph* g, unsigned long long seq, unsigned long long* values, i
 
 void BeginTiming(gg_graph* g) { GG_HIP(hipEventRecord(g->ev_start, g->stream)); }
 
+void EnsureSideStream(gg_graph* g) {
+  if (g->side) return;
+  GG_HIP(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+  GG_HIP(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+  GG_HIP(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
+}
+
 void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt) {
   GG_HIP(hipEventRecord(g->ev_stop, g->stream));
   GG_HIP(hipEventSynchronize(g->ev_stop));
@@ -365,6 +372,10 @@ int gg_graph_free(gg_graph* g) {
       (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t e : g->event_pool) (void)hipEventDestroy(e);
+    if (g->side) (void)hipStreamSynchronize(g->side);
+    if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+    if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    if (g->side) (void)hipStreamDestroy(g->side);
     if (g->ev_start) (void)hipEventDestroy(g->ev_start);
     if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
     if (g->stream) (void)hipStreamDestroy(g->stream);

@@ -37,6 +37,15 @@ constexpr int kMbBS = 256;
 constexpr int kMbStripes = 64;
 constexpr int kSigmaLoads = 8;   // sigma rows in flight per wavefront step
 constexpr int kDeltaLoads = 16;  // (sigma, delta) row pairs in flight per wavefront step
+// Dependency folds of vertices with at least kHubFold successors run in
+// MbDeltaHub: kHubProducers wavefronts compute the terms of kHubEdges
+// successors each per round into an LDS double buffer, one wavefront folds
+// them in list order.
+constexpr int64_t kHubFold = 2048;
+constexpr int kHubProducers = 8;
+constexpr int kHubEdges = 8;
+constexpr int kHubRound = kHubProducers * kHubEdges;
+constexpr int kHubBS = (kHubProducers + 1) * kWave;
 
 __global__ void MbRootSigma(const int32_t* __restrict__ roots, int k, double* __restrict__ sigma) {
   const int r = threadIdx.x;
@@ -280,6 +289,7 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
       if (fl) {
         bl = off[ul];
         el = off[ul + 1];
+        if (el - bl >= kHubFold) fl = 0;     // MbDeltaHub's
         if (!next) el = bl;                   // deepest level: no successors
       }
     }
@@ -293,6 +303,91 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
   }
   ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
   if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+}
+
+// Dependencies of the hubs (>= kHubFold successors) at level L, one block
+// per hub.  Per round, producer wavefront p computes, for the successors
+// b + round * kHubRound + p * kHubEdges + j (j < kHubEdges), lane r's term
+// for root r (or -1 where the successor is not at level L + 1 for r) into
+// one half of an LDS double buffer, while the consumer wavefront folds the
+// previous round's half in successor order -- the same terms, in the same
+// order, with the same arithmetic as DeltaVertex, so the result is
+// bit-identical; only the loads and divisions run in parallel.
+template <class T>
+__global__ __launch_bounds__(kHubBS) void MbDeltaHub(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                                     const uint64_t* __restrict__ cur,
+                                                     const uint64_t* __restrict__ next,
+                                                     const double* __restrict__ sigma, T* __restrict__ delta,
+                                                     const int32_t* __restrict__ hubs, int32_t nhubs,
+                                                     unsigned long long* examined) {
+  __shared__ double terms[2][kHubRound * kWave];
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  unsigned long long ex = 0;
+  for (int32_t h = blockIdx.x; h < nhubs; h += gridDim.x) {
+    const int64_t u = hubs[h];
+    const uint64_t fu = cur[u];
+    if (fu == 0) continue;  // uniform per block
+    const int64_t b = off[u];
+    const int64_t e = next ? off[u + 1] : b;  // deepest level: no successors
+    const bool mine = (fu >> lane) & 1;
+    const double su = mine ? sigma[u * 64 + lane] : 1.0;
+    T acc = T(0);
+    const int64_t rounds = (e - b + kHubRound - 1) / kHubRound;
+    for (int64_t k = 0; k <= rounds; k++) {
+      if (wid < kHubProducers) {
+        if (k < rounds) {
+          const int64_t base = b + k * kHubRound + wid * kHubEdges;
+          double* out = terms[k & 1] + wid * kHubEdges * kWave;
+          int32_t vl = 0;
+          uint64_t ml = 0;
+          if (lane < kHubEdges && base + lane < e) {
+            vl = dst[base + lane];
+            ml = next[vl] & fu;
+          }
+          double sv[kHubEdges];
+          T dv[kHubEdges];
+          bool on[kHubEdges];
+#pragma unroll
+          for (int j = 0; j < kHubEdges; j++) {
+            const int32_t v = __shfl(vl, j);
+            const uint64_t m = __shfl(ml, j);
+            on[j] = (m >> lane) & 1;
+            sv[j] = 1.0;
+            dv[j] = T(0);
+            if (on[j]) {
+              sv[j] = sigma[int64_t(v) * 64 + lane];
+              dv[j] = delta[int64_t(v) * 64 + lane];
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < kHubEdges; j++) out[j * kWave + lane] = on[j] ? Term(su, sv[j], dv[j]) : -1.0;
+        }
+      } else if (k >= 1) {
+        const double* in = terms[(k - 1) & 1];
+        const int64_t first = b + (k - 1) * kHubRound;
+        const int cnt = static_cast<int>(e - first < kHubRound ? e - first : kHubRound);
+        for (int j = 0; j < cnt; j++) {
+          const double t = in[j * kWave + lane];
+          if (t >= 0.0) acc = Fold(acc, t);
+        }
+      }
+      __syncthreads();
+    }
+    if (wid == kHubProducers) {
+      if (mine) delta[u * 64 + lane] = acc;
+      if (lane == 0) ex += static_cast<unsigned long long>(e - b);
+    }
+  }
+  if (wid == kHubProducers && lane == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+}
+
+// Vertices with at least kHubFold out-edges (once per graph).
+__global__ __launch_bounds__(kMbBS) void MbHubList(const int64_t* __restrict__ off, int64_t n, int32_t* out,
+                                                   int32_t* tail) {
+  for (int64_t base = int64_t(blockIdx.x) * kMbBS; base < n; base += int64_t(gridDim.x) * kMbBS) {
+    const int64_t v = base + threadIdx.x;
+    WaveAppend(v < n && off[v + 1] - off[v] >= kHubFold, static_cast<int32_t>(v), out, tail);
+  }
 }
 
 __device__ __forceinline__ float AddT(float a, float b) { return __fadd_rn(a, b); }
@@ -384,12 +479,39 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
         static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, hchunks, nhchunks, ctr);
     launches += 2;
   }
+  // Hubs (their folds split across a block, MbDeltaHub): listed once per graph.
+  int32_t* hubs = g->Scratch<int32_t>("mb.hubs", n + 1);
+  if (g->mb_hubs < 0) {
+    int32_t* tail = g->Scratch<int32_t>("mb.hub_tail", 1);
+    GG_HIP(hipMemsetAsync(tail, 0, sizeof(int32_t), st));
+    Run(g, "msbc_hub_list", 0, MbHubList, dim3(GridFor(n, kMbBS, g->cu_count * 4)), blk,
+        static_cast<const int64_t*>(g->out_off), n, hubs, tail);
+    int32_t h = 0;
+    GG_HIP(hipMemcpyAsync(&h, tail, sizeof(h), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    g->mb_hubs = h;
+  }
+  const int32_t nhubs = static_cast<int32_t>(g->mb_hubs);
+  if (nhubs > 0) EnsureSideStream(g);
   for (int L = levels - 1; L >= 0; L--) {
     const uint64_t* next = L + 1 < levels ? t.frontiers[L + 1] : nullptr;
+    // The hubs' folds (side stream) overlap the other vertices' (main
+    // stream): they write disjoint rows and read only level L + 1's.
+    if (nhubs > 0) {
+      GG_HIP(hipEventRecord(g->ev_fork, st));
+      GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
+      RunOn(g, g->side, "msbc_delta_hub", 0, MbDeltaHub<T>, dim3(GridFor(nhubs, 1, g->cu_count * 2)), dim3(kHubBS),
+            static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+            static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
+            static_cast<const int32_t*>(hubs), nhubs, ctr + kMbStripes);
+      GG_HIP(hipEventRecord(g->ev_join, g->side));
+      launches++;
+    }
     Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
         static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
         static_cast<const double*>(sigma), delta, n, ctr + kMbStripes);
     launches++;
+    if (nhubs > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
   }
   Run(g, "msbc_scores", 8.0 * N + sizeof(T) * N, MbScores<T>, grid_n, blk, static_cast<const uint64_t*>(t.seen),
       static_cast<const T*>(delta), n, score);
