@@ -44,6 +44,9 @@ constexpr int kQueueCap = 2048;
 constexpr int kStripes = 64;
 constexpr int64_t kHubEdges = 256;   // sigma pull: vertices with this many in-edges use chunks
 constexpr int64_t kWaveEdges = 64;   // delta: vertices with this many edges use a wavefront
+constexpr int64_t kBcHubEdges = 16384;  // ... and with this many, a block (BcDeltaHub)
+constexpr int kBcHubProducers = 15;
+constexpr int kBcHubBS = (kBcHubProducers + 1) * kWave;
 
 struct BcCounters {
   int32_t tail;     // queue length (all levels so far)
@@ -296,6 +299,7 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
   for (int64_t i = wave; i < nw; i += nwaves) {
     const int32_t u = wave_list[i];
     const int64_t b = off[u], e = off[u + 1];
+    if (e - b >= kBcHubEdges) continue;  // BcDeltaHub's (uniform per wavefront)
     const double su = sigma[u];
     typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
     for (int64_t base = b; base < e; base += kWave) {
@@ -317,6 +321,75 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
       }
     }
     if (lane == 0) {
+      const T d = acc.Value();
+      delta[u] = d;
+      AddScore<kKahan>(score, comp, u, d);
+    }
+  }
+}
+
+// Listed vertices with at least kBcHubEdges successors, one block each:
+// per round, producer wavefront p computes the terms of 64 consecutive
+// edges (b + round * 64 kBcHubProducers + 64 p + lane) and packs the
+// matched ones, in edge order, into one half of an LDS double buffer; the
+// last wavefront folds the previous round's terms in edge order.  Same
+// terms, same order, same arithmetic as BcDeltaWave; only the loads and
+// divisions run in parallel.
+template <class T, bool kKahan>
+__global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ dst,
+                                                       const int32_t* __restrict__ depth,
+                                                       const double* __restrict__ sigma, T* __restrict__ delta,
+                                                       T* __restrict__ score, T* __restrict__ comp,
+                                                       const int32_t* __restrict__ wave_list, int32_t level,
+                                                       const BcCounters* c) {
+  __shared__ double terms[2][kBcHubProducers][kWave];
+  __shared__ int counts[2][kBcHubProducers];
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  const int32_t nw = c->nwave;
+  constexpr int64_t kRound = int64_t(kBcHubProducers) * kWave;
+  for (int32_t i = blockIdx.x; i < nw; i += gridDim.x) {
+    const int32_t u = wave_list[i];
+    const int64_t b = off[u], e = off[u + 1];
+    if (e - b < kBcHubEdges) continue;  // uniform per block
+    const double su = sigma[u];
+    typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
+    const int64_t rounds = (e - b + kRound - 1) / kRound;
+    for (int64_t k = 0; k <= rounds; k++) {
+      if (wid < kBcHubProducers) {
+        if (k < rounds) {
+          const int64_t kk = b + k * kRound + int64_t(wid) * kWave + lane;
+          double term = 0.0;
+          bool has = false;
+          if (kk < e) {
+            const int32_t v = dst[kk];
+            if (depth[v] == level + 1) {
+              term = DeltaTerm(su, sigma[v], delta[v]);
+              has = true;
+            }
+          }
+          const uint64_t m = __ballot(has);
+          if (has) terms[k & 1][wid][__popcll(LanesBelow(m))] = term;
+          if (lane == 0) counts[k & 1][wid] = __popcll(m);
+        }
+      } else if (k >= 1) {
+        const int h = (k - 1) & 1;
+        for (int p = 0; p < kBcHubProducers; p++) {
+          const int cnt = counts[h][p];
+          int j = 0;
+          for (; j + 8 <= cnt; j += 8) {  // eight LDS reads in flight, then the ordered fold
+            double t[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = terms[h][p][j + q];
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc.Add(t[q]);
+          }
+          for (; j < cnt; j++) acc.Add(terms[h][p][j]);
+        }
+      }
+      __syncthreads();
+    }
+    if (wid == kBcHubProducers && lane == 0) {
       const T d = acc.Value();
       delta[u] = d;
       AddScore<kKahan>(score, comp, u, d);
@@ -488,10 +561,26 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
           wave_list, dc);
       launches++;
       if (g->max_degree >= kWaveEdges) {
+        // Hubs fold on the side stream beside the wavefront kernel (disjoint
+        // vertices; both only read level d + 1); both finish before the
+        // list is cleared.
+        const bool hubs = g->max_degree >= kBcHubEdges;
+        if (hubs) {
+          EnsureSideStream(g);
+          GG_HIP(hipEventRecord(g->ev_fork, st));
+          GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
+          RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count), dim3(kBcHubBS),
+                static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+                static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
+                static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
+          GG_HIP(hipEventRecord(g->ev_join, g->side));
+          launches++;
+        }
         Run(g, "bc_delta_wave", 0, BcDeltaWave<T, kKahan>, dim3(GridFor(qn, kBS / kWave, g->cu_count * 8)),
             blk, static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
             static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
             static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
+        if (hubs) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
         Run(g, "bc_clear_reduce", 0, BcClearReduce, dim3(1), blk, dc);
         launches += 2;
       }

@@ -366,7 +366,16 @@ __global__ __launch_bounds__(kHubBS) void MbDeltaHub(const int64_t* __restrict__
         const double* in = terms[(k - 1) & 1];
         const int64_t first = b + (k - 1) * kHubRound;
         const int cnt = static_cast<int>(e - first < kHubRound ? e - first : kHubRound);
-        for (int j = 0; j < cnt; j++) {
+        int j = 0;
+        for (; j + 8 <= cnt; j += 8) {  // eight LDS reads in flight, then the ordered fold
+          double t[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) t[q] = in[(j + q) * kWave + lane];
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            if (t[q] >= 0.0) acc = Fold(acc, t[q]);
+        }
+        for (; j < cnt; j++) {
           const double t = in[j * kWave + lane];
           if (t >= 0.0) acc = Fold(acc, t);
         }

@@ -73,3 +73,29 @@ def test_bc_repeated_root_and_disconnected():
     assert np.array_equal(gpu.bc(dg, roots).scores, oracle.brandes(g, roots))
     with pytest.raises(gpu.GpuError):
         gpu.bc(dg, [g.num_nodes])
+
+
+def test_bc_hub_folds():
+    # Vertices with 16384+ successors fold on a block (terms in parallel,
+    # folded in edge order): two such hubs sharing neighbours, a partial
+    # last round, roots around them; FP32 bit-exact with the reference's
+    # float arithmetic, Kahan and FP64 within the stated tolerances.
+    print("[bc] hub folds", flush=True)
+    rng = np.random.default_rng(23)
+    n = 40000
+    u, v = [], []
+    for h, d in ((0, 30000), (1, 16384)):
+        u.append(np.full(d, h))
+        v.append(rng.choice(np.arange(2, n), size=d, replace=False))
+    u.append(rng.integers(2, n, 50000))
+    v.append(rng.integers(2, n, 50000))
+    g = Graph.from_edges(np.concatenate(u), np.concatenate(v))
+    dg = gpu.DeviceGraph(g)
+    roots = np.array([0, 1, 7, 123, 4567, 30001], dtype=np.int32)
+    assert np.array_equal(gpu.bc(dg, roots).scores, oracle.brandes(g, roots))
+    want = oracle.brandes(g, roots, fp64=True)
+    assert np.abs(gpu.bc(dg, roots, fp64=True).scores - want).max() < 1e-6
+    assert np.abs(gpu.bc(dg, roots, kahan=True).scores - want).max() < 1e-5
+    dg.profile(True)
+    gpu.bc(dg, roots[:1])
+    assert "bc_delta_hub" in dg.profile_report()
