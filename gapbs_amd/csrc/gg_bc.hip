@@ -397,6 +397,56 @@ __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict
   }
 }
 
+
+// ---- discovery by DOBFS ------------------------------------------------------
+//
+// Depths are unique, so the direction-optimizing BFS (RunDobfs) gives the
+// levels of the reference's top-down PBFS at a fraction of the edges.  The
+// vertices are then bucketed by depth into the queue layout the sigma and
+// dependency passes read (level d at queue[bounds[d], bounds[d + 1]); the
+// order inside a level does not affect any result).
+constexpr int kBucketLevels = 4096;  // deeper traversals use the top-down discovery
+constexpr int64_t kBucketChunk = 65536;
+
+__global__ __launch_bounds__(kBS) void BcLevelCount(const int32_t* __restrict__ depth, int64_t n, int nlev,
+                                                    int32_t* counts) {
+  __shared__ int32_t h[kBucketLevels];
+  for (int i = threadIdx.x; i < nlev; i += kBS) h[i] = 0;
+  __syncthreads();
+  const int64_t lo = int64_t(blockIdx.x) * kBucketChunk;
+  const int64_t hi = lo + kBucketChunk < n ? lo + kBucketChunk : n;
+  for (int64_t v = lo + threadIdx.x; v < hi; v += kBS) {
+    const int32_t d = depth[v];
+    if (d >= 0 && d < nlev) atomicAdd(&h[d], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nlev; i += kBS)
+    if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+// cursor[d] starts at bounds[d]; each block reserves its ranges with one
+// atomic per level it holds.
+__global__ __launch_bounds__(kBS) void BcLevelScatter(const int32_t* __restrict__ depth, int64_t n, int nlev,
+                                                      int32_t* cursor, int32_t* __restrict__ queue) {
+  __shared__ int32_t h[kBucketLevels];
+  for (int i = threadIdx.x; i < nlev; i += kBS) h[i] = 0;
+  __syncthreads();
+  const int64_t lo = int64_t(blockIdx.x) * kBucketChunk;
+  const int64_t hi = lo + kBucketChunk < n ? lo + kBucketChunk : n;
+  for (int64_t v = lo + threadIdx.x; v < hi; v += kBS) {
+    const int32_t d = depth[v];
+    if (d >= 0 && d < nlev) atomicAdd(&h[d], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nlev; i += kBS)
+    if (h[i]) h[i] = atomicAdd(&cursor[i], h[i]);
+  __syncthreads();
+  for (int64_t v = lo + threadIdx.x; v < hi; v += kBS) {
+    const int32_t d = depth[v];
+    if (d >= 0 && d < nlev) queue[atomicAdd(&h[d], 1)] = static_cast<int32_t>(v);
+  }
+}
+
 __global__ __launch_bounds__(kBS) void BcClearReduce(BcCounters* c) {
   if (threadIdx.x == 0) {
     c->nred = 0;
@@ -490,6 +540,9 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   float* out32 = g->Scratch<float>("bc.out32", n);
   unsigned long long* max_bits = g->Scratch<unsigned long long>("bc.max", 1);
   BcCounters* dc = g->Scratch<BcCounters>("bc.ctr", 1);
+  int32_t* lvl_counts = g->Scratch<int32_t>("bc.level_counts", kBucketLevels);
+  int32_t* lvl_cursor = g->Scratch<int32_t>("bc.level_cursor", kBucketLevels);
+  std::vector<int32_t> h_counts(kBucketLevels);
   const int64_t entries = g->out_entries > g->in_entries ? g->out_entries : g->in_entries;
   EdgeChunk* chunks = g->Scratch<EdgeChunk>("bc.chunks", MaxChunks(n, entries));
   EdgeChunk* rchunks = g->Scratch<EdgeChunk>("bc.rchunks", MaxChunks(n, entries));
@@ -511,8 +564,52 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
     bounds.assign(1, 0);
     bounds.push_back(1);
     int32_t level = 0;
-    // Forward: discover each level, then pull its sigma.
-    while (true) {
+    // Forward, by DOBFS and bucketing when the levels fit the buckets.
+    const BfsRun br = RunDobfs(g, roots[r], 15, 18, depth);
+    launches += br.launches;
+    examined_total += br.examined;
+    const int nlev = static_cast<int>(br.td + br.bu + 1);
+    bool bucketed = false;
+    if (nlev <= kBucketLevels) {
+      GG_HIP(hipMemsetAsync(lvl_counts, 0, sizeof(int32_t) * nlev, st));
+      const dim3 grid_b(GridFor(n, static_cast<int>(kBucketChunk), 1u << 20));
+      Run(g, "bc_level_count", 4 * N, BcLevelCount, grid_b, blk, static_cast<const int32_t*>(depth), n, nlev,
+          lvl_counts);
+      GG_HIP(hipMemcpyAsync(h_counts.data(), lvl_counts, sizeof(int32_t) * nlev, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      int last = 0;
+      for (int d = 0; d < nlev; d++)
+        if (h_counts[d]) last = d;
+      bounds.assign(1, 0);
+      for (int d = 0; d <= last; d++) bounds.push_back(bounds.back() + h_counts[d]);
+      GG_HIP(hipMemcpyAsync(lvl_cursor, bounds.data(), sizeof(int32_t) * (last + 1), hipMemcpyHostToDevice, st));
+      Run(g, "bc_level_scatter", 8 * N, BcLevelScatter, grid_b, blk, static_cast<const int32_t*>(depth), n, last + 1,
+          lvl_cursor, queue);
+      launches += 2;
+      for (int32_t d = 1; d <= last; d++) {
+        const int32_t nb = bounds[d], nn = bounds[d + 1] - bounds[d];
+        Run(g, "bc_sigma", 20.0 * nn, BcSigma, dim3(GridFor(nn, kBS, g->cu_count * 16)), blk,
+            static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
+            static_cast<const int32_t*>(depth), sigma, static_cast<const int32_t*>(queue), nb, nn, d - 1, rchunks,
+            dc);
+        launches++;
+        if (hubs) {
+          Run(g, "bc_sigma_chunks", 0, BcSigmaChunks, grid_c, blk, static_cast<const int32_t*>(g->in_dst),
+              static_cast<const int32_t*>(depth), sigma, d - 1, static_cast<const EdgeChunk*>(rchunks),
+              static_cast<const BcCounters*>(dc));
+          Run(g, "bc_clear_reduce", 0, BcClearReduce, dim3(1), blk, dc);
+          launches += 2;
+        }
+      }
+      level = last;
+      bucketed = true;
+    } else {
+      // Too deep for the buckets: the top-down discovery, level by level
+      // (the BFS above only cost its own time; reset its outputs).
+      Run(g, "bc_reset_source", 4 * N, BcResetSource, grid_n, blk, depth, n, roots[r], sigma, queue, dc);
+      launches++;
+    }
+    while (!bucketed) {
       const int32_t qb = bounds[bounds.size() - 2], qn = bounds.back() - qb;
       const int64_t chunk_min = qn <= 4096 ? kExpandBS : kChunkEdges;
       const int idx = Run(g, "bc_discover", 8.0 * qn, BcDiscover, dim3(GridFor(qn, kBS, g->cu_count * 8)), blk,

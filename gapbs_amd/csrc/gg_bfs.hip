@@ -379,7 +379,164 @@ __global__ __launch_bounds__(kBS) void BfsFinish(int32_t* __restrict__ parent, i
     if (parent[i] < -1) parent[i] = -1;
 }
 
+// Vertices without in-neighbours, once per graph: bottom-up treats them as
+// visited (it could never discover them).  Synchronises the stream the
+// first time.
+uint64_t* NoInNeighbours(gg_graph* g) {
+  const int64_t n = g->n;
+  const int64_t nwords = (n + 63) / 64;
+  hipStream_t st = g->stream;
+  const dim3 blk(kBS);
+  const dim3 grid_few(GridFor(nwords, kBS / kWave, g->cu_count));
+  uint64_t* no_in = g->Scratch<uint64_t>("graph.no_in", nwords);
+  if (g->no_in_count < 0) {
+    unsigned long long* cnt = g->Scratch<unsigned long long>("graph.no_in_count", 1);
+    GG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
+    BfsNoInNeighbours<<<grid_few, blk, 0, st>>>(g->in_off, n, no_in, cnt);
+    unsigned long long h = 0;
+    GG_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+    g->no_in_count = static_cast<int64_t>(h);
+  }
+
+  return no_in;
+}
+
 }  // namespace
+
+// DOBFS from `source` (a device id) on g->stream, inside the caller's timed
+// region: parent in the "bfs.parent" workspace array, depth (device, may be
+// null) the level of every vertex (-1 unreached).
+BfsRun RunDobfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* depth) {
+  const int64_t n = g->n;
+  const int64_t nwords = (n + 63) / 64;
+  hipStream_t st = g->stream;
+  int32_t* parent = g->Scratch<int32_t>("bfs.parent", n);
+  int32_t* qin = g->Scratch<int32_t>("bfs.qa", n);
+  int32_t* qout = g->Scratch<int32_t>("bfs.qb", n);
+  uint64_t* front = g->Scratch<uint64_t>("bfs.front", nwords);
+  uint64_t* next = g->Scratch<uint64_t>("bfs.next", nwords);
+  uint64_t* visited = g->Scratch<uint64_t>("bfs.visited", nwords);
+  BfsCounters* dc = g->Scratch<BfsCounters>("bfs.ctr", 1);
+  const bool use_chunks = g->max_degree >= kExpandBS;
+  EdgeChunk* chunks =
+      use_chunks ? g->Scratch<EdgeChunk>("bfs.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+
+  const dim3 blk(kBS);
+  const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
+  const dim3 grid_w(GridFor(nwords, kBS / kWave, g->cu_count * 32));
+  const dim3 grid_few(GridFor(nwords, kBS / kWave, g->cu_count));
+  const double N = static_cast<double>(n), W = static_cast<double>(nwords);
+  int64_t launches = 0;
+  unsigned long long mail[kMailValues];
+  auto sum = [&](int k) { return static_cast<int64_t>(mail[k]); };
+  // Posts this step's counter totals to the host (and clears them).
+  auto publish = [&]() {
+    const unsigned long long seq = MailNext(g);
+    Run(g, "bfs_publish", 0, BfsPublish, dim3(1), blk, dc, g->mail_dev, seq);
+    launches++;
+    MailWait(g, seq, mail, kMailValues);
+  };
+
+  uint64_t* no_in = NoInNeighbours(g);
+  Run(g, "bfs_reset", 0, BfsResetCounters, dim3(1), blk, dc);
+  // Algorithmic bytes per launch: the compulsory traffic of each kernel
+  // (offsets 8 B, ids 4 B, per-vertex state 4 B, bitmaps N/8).
+  Run(g, "bfs_init", 8 * (N + 1) + 4 * N + (depth ? 4 * N : 0), BfsInit, grid_n, blk,
+      g->out_off, g->in_off, parent, depth, n, source, qin, dc);
+  launches += 2;
+  publish();
+  int64_t scout = sum(kScout);
+  int64_t reach = sum(kReach);
+  // Unvisited vertices without in-neighbours that the reference's
+  // bottom-up step examines (with zero edges) and this one skips.
+  const int64_t skipped_no_in = g->no_in_count - (mail[kMailSrcNoIn] ? 1 : 0);
+  int64_t examined = 0, vexamined = 0, reached = 1;
+  const int64_t e_dir = g->directed ? g->out_entries : 2 * (g->out_entries / 2);
+  int64_t edges_to_check = e_dir;
+  int64_t qsize = 1;
+  int32_t level = 1;
+  int64_t td = 0, bu = 0;
+  // Upper bound on the summed out-degree of the current frontier.
+  int64_t frontier_deg = scout;
+  while (qsize > 0) {
+    if (scout > edges_to_check / alpha) {
+      Run(g, "bfs_bitmap_clear", 8 * W, FillU64, grid_w, blk, front, nwords, uint64_t(0));
+      Run(g, "bfs_queue_to_bitmap", 4.0 * qsize, BfsQueueToBitmap,
+          dim3(GridFor(qsize, kBS, g->cu_count * 32)), blk, qin, static_cast<int32_t>(qsize),
+          reinterpret_cast<unsigned long long*>(front));
+      Run(g, "bfs_visited_from_parent", 4 * N + 16 * W, BfsVisitedFromParent, grid_w, blk, parent, n,
+          no_in, visited);
+      launches += 3;
+      int64_t awake = qsize, old_awake;
+      do {
+        old_awake = awake;
+        const int idx = Run(g, "bfs_bottom_up", 3 * W * 8, BfsBottomUp, grid_w, blk, g->in_off,
+                            g->in_dst, g->out_off, parent, depth, front, next, visited, n, dc, level);
+        launches++;
+        publish();
+        awake = sum(kAwake);
+        frontier_deg = sum(kReach);
+        reached += awake;
+        reach += frontier_deg;
+        examined += sum(kExamined);
+        vexamined += sum(kVexam) + skipped_no_in;
+        CreditBytes(g, idx, 4.0 * sum(kExamined) + 16.0 * sum(kVexam));
+        level++;
+        bu++;
+        std::swap(front, next);
+      } while (awake >= old_awake || awake > n / beta);
+      Run(g, "bfs_bitmap_to_queue", 16 * W + 4.0 * awake, BfsBitmapToQueue, grid_few, blk, front,
+          nwords, qin, &dc->bmtail);
+      launches++;
+      qsize = awake;  // the queue holds exactly the last step's discoveries
+      scout = 1;
+    } else {
+      edges_to_check -= scout;
+      const bool hubs = use_chunks && frontier_deg >= kExpandBS;
+      // A small frontier gets every vertex with >= 256 edges chunked across
+      // the grid; a large one only the real hubs.
+      const int64_t chunk_min = qsize <= 4096 ? kExpandBS : kChunkEdges;
+      const int idx = Run(g, "bfs_top_down", 16.0 * qsize, BfsTopDown,
+                          dim3(GridFor(qsize, kBS, g->cu_count * 8)), blk, g->out_off, g->out_dst,
+                          parent, depth, qin, static_cast<int32_t>(qsize), qout, dc, level,
+                          hubs ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
+      launches++;
+      if (hubs) {
+        const dim3 grid_c(GridFor(frontier_deg / kExpandBS + kChunkSlices, 1, g->cu_count * 8));
+        Run(g, "bfs_top_down_chunks", 0, BfsTopDownChunks, grid_c, blk, g->out_off, g->out_dst,
+            parent, depth, qout, dc, level, static_cast<const EdgeChunk*>(chunks));
+        launches++;
+      }
+      vexamined += qsize;
+      publish();
+      qsize = static_cast<int64_t>(mail[kMailQtail]);
+      reached += qsize;
+      scout = sum(kScout);
+      frontier_deg = scout;
+      reach += sum(kReach);
+      examined += sum(kExamined);
+      CreditBytes(g, idx, 4.0 * sum(kExamined));
+      level++;
+      td++;
+      std::swap(qin, qout);
+    }
+  }
+  Run(g, "bfs_finish", 4 * N, BfsFinish, grid_n, blk, parent, n);
+  launches++;
+  GG_HIP(hipGetLastError());
+  BfsRun r;
+  r.td = td;
+  r.bu = bu;
+  r.levels = level;
+  r.reached = reached;
+  r.reach = reach;
+  r.examined = examined;
+  r.vexamined = vexamined;
+  r.launches = launches;
+  return r;
+}
+
 }  // namespace gg
 
 using namespace gg;
@@ -395,151 +552,23 @@ extern "C" int gg_bfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t*
     GG_REQUIRE(alpha > 0 && beta > 0, GG_EINVAL, "alpha and beta must be positive");
     GG_HIP(hipSetDevice(g->device));
     source = DeviceVertex(g, source);
-    const int64_t n = g->n;
-    const int64_t nwords = (n + 63) / 64;
-    hipStream_t st = g->stream;
-
-    int32_t* parent = g->Scratch<int32_t>("bfs.parent", n);
-    int32_t* depth = depth_out ? g->Scratch<int32_t>("bfs.depth", n) : nullptr;
-    int32_t* qin = g->Scratch<int32_t>("bfs.qa", n);
-    int32_t* qout = g->Scratch<int32_t>("bfs.qb", n);
-    uint64_t* front = g->Scratch<uint64_t>("bfs.front", nwords);
-    uint64_t* next = g->Scratch<uint64_t>("bfs.next", nwords);
-    uint64_t* visited = g->Scratch<uint64_t>("bfs.visited", nwords);
-    BfsCounters* dc = g->Scratch<BfsCounters>("bfs.ctr", 1);
-    const bool use_chunks = g->max_degree >= kExpandBS;
-    EdgeChunk* chunks =
-        use_chunks ? g->Scratch<EdgeChunk>("bfs.chunks", MaxChunks(n, g->out_entries)) : nullptr;
-
-    const dim3 blk(kBS);
-    const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
-    const dim3 grid_w(GridFor(nwords, kBS / kWave, g->cu_count * 32));
-    const dim3 grid_few(GridFor(nwords, kBS / kWave, g->cu_count));
-    const double N = static_cast<double>(n), W = static_cast<double>(nwords);
-    int64_t launches = 0;
-    unsigned long long mail[kMailValues];
-    auto sum = [&](int k) { return static_cast<int64_t>(mail[k]); };
-    // Posts this step's counter totals to the host (and clears them).
-    auto publish = [&]() {
-      const unsigned long long seq = MailNext(g);
-      Run(g, "bfs_publish", 0, BfsPublish, dim3(1), blk, dc, g->mail_dev, seq);
-      launches++;
-      MailWait(g, seq, mail, kMailValues);
-    };
-
-    // Vertices without in-neighbours, once per graph: bottom-up treats them
-    // as visited (it could never discover them).
-    uint64_t* no_in = g->Scratch<uint64_t>("graph.no_in", nwords);
-    if (g->no_in_count < 0) {
-      unsigned long long* cnt = g->Scratch<unsigned long long>("graph.no_in_count", 1);
-      GG_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
-      BfsNoInNeighbours<<<grid_few, blk, 0, st>>>(g->in_off, n, no_in, cnt);
-      unsigned long long h = 0;
-      GG_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
-      GG_HIP(hipStreamSynchronize(st));
-      g->no_in_count = static_cast<int64_t>(h);
-    }
-
+    int32_t* depth = depth_out ? g->Scratch<int32_t>("bfs.depth", g->n) : nullptr;
+    NoInNeighbours(g);
     BeginTiming(g);
-    Run(g, "bfs_reset", 0, BfsResetCounters, dim3(1), blk, dc);
-    // Algorithmic bytes per launch: the compulsory traffic of each kernel
-    // (offsets 8 B, ids 4 B, per-vertex state 4 B, bitmaps N/8).
-    Run(g, "bfs_init", 8 * (N + 1) + 4 * N + (depth ? 4 * N : 0), BfsInit, grid_n, blk,
-        g->out_off, g->in_off, parent, depth, n, source, qin, dc);
-    launches += 2;
-    publish();
-    int64_t scout = sum(kScout);
-    int64_t reach = sum(kReach);
-    // Unvisited vertices without in-neighbours that the reference's
-    // bottom-up step examines (with zero edges) and this one skips.
-    const int64_t skipped_no_in = g->no_in_count - (mail[kMailSrcNoIn] ? 1 : 0);
-    int64_t examined = 0, vexamined = 0, reached = 1;
-    const int64_t e_dir = g->directed ? g->out_entries : 2 * (g->out_entries / 2);
-    int64_t edges_to_check = e_dir;
-    int64_t qsize = 1;
-    int32_t level = 1;
-    int64_t td = 0, bu = 0;
-    // Upper bound on the summed out-degree of the current frontier.
-    int64_t frontier_deg = scout;
-    while (qsize > 0) {
-      if (scout > edges_to_check / alpha) {
-        Run(g, "bfs_bitmap_clear", 8 * W, FillU64, grid_w, blk, front, nwords, uint64_t(0));
-        Run(g, "bfs_queue_to_bitmap", 4.0 * qsize, BfsQueueToBitmap,
-            dim3(GridFor(qsize, kBS, g->cu_count * 32)), blk, qin, static_cast<int32_t>(qsize),
-            reinterpret_cast<unsigned long long*>(front));
-        Run(g, "bfs_visited_from_parent", 4 * N + 16 * W, BfsVisitedFromParent, grid_w, blk, parent, n,
-            no_in, visited);
-        launches += 3;
-        int64_t awake = qsize, old_awake;
-        do {
-          old_awake = awake;
-          const int idx = Run(g, "bfs_bottom_up", 3 * W * 8, BfsBottomUp, grid_w, blk, g->in_off,
-                              g->in_dst, g->out_off, parent, depth, front, next, visited, n, dc, level);
-          launches++;
-          publish();
-          awake = sum(kAwake);
-          frontier_deg = sum(kReach);
-          reached += awake;
-          reach += frontier_deg;
-          examined += sum(kExamined);
-          vexamined += sum(kVexam) + skipped_no_in;
-          CreditBytes(g, idx, 4.0 * sum(kExamined) + 16.0 * sum(kVexam));
-          level++;
-          bu++;
-          std::swap(front, next);
-        } while (awake >= old_awake || awake > n / beta);
-        Run(g, "bfs_bitmap_to_queue", 16 * W + 4.0 * awake, BfsBitmapToQueue, grid_few, blk, front,
-            nwords, qin, &dc->bmtail);
-        launches++;
-        qsize = awake;  // the queue holds exactly the last step's discoveries
-        scout = 1;
-      } else {
-        edges_to_check -= scout;
-        const bool hubs = use_chunks && frontier_deg >= kExpandBS;
-        // A small frontier gets every vertex with >= 256 edges chunked across
-        // the grid; a large one only the real hubs.
-        const int64_t chunk_min = qsize <= 4096 ? kExpandBS : kChunkEdges;
-        const int idx = Run(g, "bfs_top_down", 16.0 * qsize, BfsTopDown,
-                            dim3(GridFor(qsize, kBS, g->cu_count * 8)), blk, g->out_off, g->out_dst,
-                            parent, depth, qin, static_cast<int32_t>(qsize), qout, dc, level,
-                            hubs ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
-        launches++;
-        if (hubs) {
-          const dim3 grid_c(GridFor(frontier_deg / kExpandBS + kChunkSlices, 1, g->cu_count * 8));
-          Run(g, "bfs_top_down_chunks", 0, BfsTopDownChunks, grid_c, blk, g->out_off, g->out_dst,
-              parent, depth, qout, dc, level, static_cast<const EdgeChunk*>(chunks));
-          launches++;
-        }
-        vexamined += qsize;
-        publish();
-        qsize = static_cast<int64_t>(mail[kMailQtail]);
-        reached += qsize;
-        scout = sum(kScout);
-        frontier_deg = scout;
-        reach += sum(kReach);
-        examined += sum(kExamined);
-        CreditBytes(g, idx, 4.0 * sum(kExamined));
-        level++;
-        td++;
-        std::swap(qin, qout);
-      }
-    }
-    Run(g, "bfs_finish", 4 * N, BfsFinish, grid_n, blk, parent, n);
-    launches++;
-    GG_HIP(hipGetLastError());
+    const BfsRun r = RunDobfs(g, source, alpha, beta, depth);
     FinishTiming(g, stats, wt);
-    if (parent_out) CopyOutVertexI32(g, parent, parent_out, true);
+    if (parent_out) CopyOutVertexI32(g, g->Scratch<int32_t>("bfs.parent", g->n), parent_out, true);
     if (depth_out) CopyOutVertexI32(g, depth, depth_out, false);
     if (stats) {
       stats->total_ms = wt.ms();
-      stats->iterations = td + bu;
-      stats->td_steps = td;
-      stats->bu_steps = bu;
-      stats->reached = reached;
-      stats->reached_edges = reach;
-      stats->edges_examined = examined;
-      stats->vertices_examined = vexamined;
-      stats->launches = launches;
+      stats->iterations = r.td + r.bu;
+      stats->td_steps = r.td;
+      stats->bu_steps = r.bu;
+      stats->reached = r.reached;
+      stats->reached_edges = r.reach;
+      stats->edges_examined = r.examined;
+      stats->vertices_examined = r.vexamined;
+      stats->launches = r.launches;
     }
   });
 }

@@ -166,6 +166,18 @@ void CopyOutVertexF32(gg_graph* g, const float* src, float* host);
 // component -> host, each the smallest original id of its component.
 void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host);
 
+// Direction-optimizing BFS (gg_bfs.hip) from device id `source` on
+// g->stream, inside the caller's timed region: parents in the workspace
+// array "bfs.parent", depth (device, may be null) the level of every vertex
+// (-1 unreached).  Also what BC's discovery uses: depths are unique, so any
+// BFS gives the reference's levels.
+struct BfsRun {
+  int64_t td = 0, bu = 0;  // levels run in each direction
+  int32_t levels = 0;      // deepest level + 1 ... as counted by the level loop
+  int64_t reached = 0, reach = 0, examined = 0, vexamined = 0, launches = 0;
+};
+BfsRun RunDobfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* depth);
+
 // Host wall clock for gg_stats::total_ms.
 struct WallTimer {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();

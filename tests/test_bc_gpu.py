@@ -99,3 +99,13 @@ def test_bc_hub_folds():
     dg.profile(True)
     gpu.bc(dg, roots[:1])
     assert "bc_delta_hub" in dg.profile_report()
+
+
+def test_bc_deep_graph_uses_level_by_level_discovery():
+    # More levels than the depth buckets hold (4096): discovery falls back to
+    # the top-down level loop; results as the reference's.
+    print("[bc] path of 5000", flush=True)
+    g = Graph.from_edges(np.arange(4999), np.arange(1, 5000))
+    dg = gpu.DeviceGraph(g)
+    roots = np.array([0, 2500, 4999], dtype=np.int32)
+    assert np.array_equal(gpu.bc(dg, roots).scores, oracle.brandes(g, roots))
