@@ -24,7 +24,9 @@ device:
         and the FP64/FP32 time ratio are reported (the FP64 path is held to
         the FP64 oracle in the tests)
   cc    Afforest
-  bc    Brandes from the first --bc-roots roots (the reference's float arithmetic)
+  bc    Brandes from the first --bc-roots roots (the reference's float arithmetic);
+        the FP32 and compensated (Kahan) scores' max-abs and L1 error against
+        the FP64 scores of the same roots are reported
   bc_batch  Brandes from all 64 roots in one batched traversal (same arithmetic)
   tc    Hybrid: relabel by degree on the device (when the reference's
         heuristic says so), then ordered triangle count
@@ -384,6 +386,15 @@ def main():
                                     "fp64_over_fp32_time": round(results["pr_fp64"]["seconds"] /
                                                                  results["pr"]["seconds"], 3)}
         log(f"pr precision: {results['pr']['vs_fp64']}")
+    if "bc" in results:
+        import numpy as np
+        s64 = gpu.bc(dg, bc_roots, fp64=True).scores.astype(np.float64)
+        err = {}
+        for name, kw in (("fp32", {}), ("kahan", {"kahan": True})):
+            d = np.abs(gpu.bc(dg, bc_roots, **kw).scores.astype(np.float64) - s64)
+            err[name] = {"max_abs": float(d.max()), "l1": float(d.sum())}
+        results["bc"]["vs_fp64"] = err
+        log(f"bc precision: {err}")
     if "bc_batch" in results and "bc" in results:  # per-root GTEPS: bc runs fewer roots
         results["bc_batch"]["speedup_vs_sequential"] = round(results["bc_batch"]["gteps"] /
                                                              results["bc"]["gteps"], 3)

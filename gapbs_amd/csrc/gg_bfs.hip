@@ -410,7 +410,6 @@ uint64_t* NoInNeighbours(gg_graph* g) {
 BfsRun RunDobfs(gg_graph* g, int32_t source, int alpha, int beta, int32_t* depth) {
   const int64_t n = g->n;
   const int64_t nwords = (n + 63) / 64;
-  hipStream_t st = g->stream;
   int32_t* parent = g->Scratch<int32_t>("bfs.parent", n);
   int32_t* qin = g->Scratch<int32_t>("bfs.qa", n);
   int32_t* qout = g->Scratch<int32_t>("bfs.qb", n);

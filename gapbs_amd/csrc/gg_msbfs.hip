@@ -5,6 +5,24 @@
 // equal what DOBFS gives for it alone (bfs.cc:122-179).
 #include "gg_msbfs.h"
 
+namespace gg {
+namespace {
+
+// depth16[v][r] -> depth32[v] of root r (-1 where seen has no bit r: the
+// slot is undefined there); with new_id, out is indexed by original ids.
+__global__ __launch_bounds__(kBS) void MsDepthOut(const uint16_t* __restrict__ d16, const uint64_t* __restrict__ seen,
+                                                  const int32_t* __restrict__ new_id, int64_t n, int r,
+                                                  int32_t* __restrict__ out) {
+  for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
+    const int64_t dv = new_id ? int64_t(new_id[v]) : v;
+    const uint16_t d = (seen[dv] >> r) & 1 ? d16[dv * 64 + r] : kNoDepth;
+    out[v] = d == kNoDepth ? -1 : d;
+  }
+}
+
+}  // namespace
+}  // namespace gg
+
 using namespace gg;
 
 extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha, int32_t* depths,
