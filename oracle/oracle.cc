@@ -209,6 +209,46 @@ void orc_dijkstra(int64_t n, const int64_t* off, const int32_t* wdst, int32_t sr
   }
 }
 
+// Shortest-path certificate over a symmetric graph with positive weights
+// (unit weights when wdst is null; dst then holds plain ids).  dist is the
+// distance vector from src iff
+//   (1) dist[src] == 0;
+//   (2) every edge (u, v, w) out of a reached u has v reached and
+//       dist[v] <= dist[u] + w (nothing relaxes, so dist <= the distances);
+//   (3) every reached v != src has an edge to a reached u with
+//       dist[u] + w == dist[v] (following such edges strictly lowers dist and
+//       ends at src, so dist >= the distances).
+// In a symmetric graph the edges out of v are the edges into v.  Returns the
+// number of vertices breaking a condition (0: certified).
+int64_t orc_distance_certificate(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* wdst,
+                                 int32_t src, const int32_t* dist, int32_t unreached) {
+  int64_t bad = (src < 0 || src >= n || dist[src] != 0) ? 1 : 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad)
+  for (int64_t u = 0; u < n; u++) {
+    const int64_t du = dist[u];
+    const bool reached = du != unreached;
+    if (reached && (du < 0 || (unreached >= 0 && du >= unreached))) {
+      bad++;
+      continue;
+    }
+    bool relaxes = false, tight = (u == src);
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = wdst ? wdst[2 * e] : dst[e];
+      const int64_t w = wdst ? wdst[2 * e + 1] : 1;
+      const int64_t dv = dist[v];
+      if (w <= 0) relaxes = true;
+      if (!reached) {
+        if (dv != unreached) relaxes = true;  // a reached neighbour would reach u
+        continue;
+      }
+      if (dv == unreached || dv > du + w) relaxes = true;
+      if (dv != unreached && dv + w == du) tight = true;
+    }
+    if (relaxes || (reached && !tight)) bad++;
+  }
+  return bad;
+}
+
 // Restates DeltaStep (src/sssp.cc:87-153) and RelaxEdges (68-85).
 double orc_delta_step(int64_t n, const int64_t* off, const int32_t* wdst, int64_t e_dir, int32_t src,
                       int32_t delta, int32_t* dist) {
@@ -420,46 +460,65 @@ void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_
 }  // extern "C"
 
 namespace {
+// One root of Brandes: depth-first order of discovery, path counts in FP64,
+// then dependencies farthest first into delta (zero where unreached).
+template <class T>
+void BrandesRoot(int64_t n, const int64_t* off, const int32_t* dst, int32_t src, std::vector<int32_t>& depth,
+                 std::vector<double>& sigma, std::vector<int32_t>& order, T* delta) {
+  std::fill(depth.begin(), depth.end(), -1);
+  std::fill(sigma.begin(), sigma.end(), 0.0);
+  std::fill(delta, delta + n, T(0));
+  order.clear();
+  depth[src] = 0;
+  sigma[src] = 1;
+  order.push_back(src);
+  for (size_t i = 0; i < order.size(); i++) {
+    const int32_t u = order[i];
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = dst[e];
+      if (depth[v] == -1) {
+        depth[v] = depth[u] + 1;
+        order.push_back(v);
+      }
+      if (depth[v] == depth[u] + 1) sigma[v] += sigma[u];
+    }
+  }
+  // Farthest first (reverse BFS order visits depths in decreasing order).
+  for (size_t i = order.size(); i-- > 0;) {
+    const int32_t u = order[i];
+    for (int64_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = dst[e];
+      if (depth[v] == depth[u] + 1) {
+        volatile double term = (sigma[u] / sigma[v]) * (1 + delta[v]);
+        delta[u] += term;
+      }
+    }
+  }
+}
+
+// Roots run in parallel, one per thread, each into its own delta row; the
+// rows are then added into scores in root order, so the sums are those of
+// the serial loop over roots (adding the zero of an unreached vertex leaves
+// a score unchanged).
 template <class T>
 void Brandes(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k, T* scores) {
   std::fill(scores, scores + n, T(0));
-  std::vector<int32_t> depth(static_cast<size_t>(n));
-  std::vector<double> sigma(static_cast<size_t>(n));
-  std::vector<T> delta(static_cast<size_t>(n));
-  std::vector<int32_t> order;
-  order.reserve(static_cast<size_t>(n));
-  for (int r = 0; r < k; r++) {
-    const int32_t src = roots[r];
-    std::fill(depth.begin(), depth.end(), -1);
-    std::fill(sigma.begin(), sigma.end(), 0.0);
-    std::fill(delta.begin(), delta.end(), T(0));
-    order.clear();
-    depth[src] = 0;
-    sigma[src] = 1;
-    order.push_back(src);
-    for (size_t i = 0; i < order.size(); i++) {
-      const int32_t u = order[i];
-      for (int64_t e = off[u]; e < off[u + 1]; e++) {
-        const int32_t v = dst[e];
-        if (depth[v] == -1) {
-          depth[v] = depth[u] + 1;
-          order.push_back(v);
-        }
-        if (depth[v] == depth[u] + 1) sigma[v] += sigma[u];
-      }
+  const int width = std::max(1, std::min(k, omp_get_max_threads()));
+  std::vector<T> rows(static_cast<size_t>(width) * static_cast<size_t>(n));
+  for (int r0 = 0; r0 < k; r0 += width) {
+    const int batch = std::min(width, k - r0);
+#pragma omp parallel num_threads(batch)
+    {
+      std::vector<int32_t> depth(static_cast<size_t>(n)), order;
+      std::vector<double> sigma(static_cast<size_t>(n));
+      order.reserve(static_cast<size_t>(n));
+#pragma omp for schedule(static, 1)
+      for (int j = 0; j < batch; j++)
+        BrandesRoot(n, off, dst, roots[r0 + j], depth, sigma, order, rows.data() + static_cast<size_t>(j) * n);
     }
-    // Farthest first (reverse BFS order visits depths in decreasing order).
-    for (size_t i = order.size(); i-- > 0;) {
-      const int32_t u = order[i];
-      for (int64_t e = off[u]; e < off[u + 1]; e++) {
-        const int32_t v = dst[e];
-        if (depth[v] == depth[u] + 1) {
-          volatile double term = (sigma[u] / sigma[v]) * (1 + delta[v]);
-          delta[u] += term;
-        }
-      }
-      scores[u] += delta[u];
-    }
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; v++)
+      for (int j = 0; j < batch; j++) scores[v] += rows[static_cast<size_t>(j) * n + v];
   }
   T biggest = 0;
   for (int64_t i = 0; i < n; i++) biggest = std::max(biggest, scores[i]);

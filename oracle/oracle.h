@@ -82,9 +82,20 @@ double orc_afforest(int64_t n, const int64_t* off, const int32_t* dst, int direc
  * smaller id): label = smallest vertex id of the component. */
 void orc_cc_min_labels(int64_t n, const int64_t* off, const int32_t* dst, int32_t* comp);
 
-/* Serial Brandes over roots[0 .. k), max-normalised, restating the
- * reference verifier's algorithm (src/bc.cc:169-220): double path counts,
- * successors from depths.  _f32: float dependencies and scores (the
+/* Certificate that dist holds the shortest-path distances from src in a
+ * symmetric graph with positive weights (wdst interleaved (dest, weight),
+ * or unit weights over dst when wdst is null; unreached vertices hold
+ * `unreached`): dist[src] == 0, no edge out of a reached vertex relaxes or
+ * leads to an unreached one, and every other reached vertex has a
+ * neighbour on a shortest path.  Returns how many vertices break it. */
+int64_t orc_distance_certificate(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* wdst,
+                                 int32_t src, const int32_t* dist, int32_t unreached);
+
+/* Brandes over roots[0 .. k), max-normalised, restating the reference
+ * verifier's algorithm (src/bc.cc:169-220): double path counts, successors
+ * from depths.  Each root is the serial algorithm; roots run one per thread
+ * and their dependencies are added in root order, so the scores are those
+ * of the serial loop over roots.  _f32: float dependencies and scores (the
  * reference's types, bc.cc:46-48); _f64: double throughout (FP64 oracle). */
 void orc_brandes_f32(int64_t n, const int64_t* off, const int32_t* dst, const int32_t* roots, int k,
                      float* scores);

@@ -70,6 +70,9 @@ def lib() -> C.CDLL:
         L.orc_ordered_count_sample.restype = C.c_uint64
         L.orc_tc_intersect.argtypes = [C.c_int64, _i64p, _i32p]
         L.orc_tc_intersect.restype = C.c_uint64
+        L.orc_distance_certificate.argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int32, _i32p,
+                                               C.c_int32]
+        L.orc_distance_certificate.restype = C.c_int64
         L.orc_depth_from_parent.argtypes = [C.c_int64, _i32p, C.c_int32, _i32p]
         _LIB = L
     return _LIB
@@ -137,6 +140,22 @@ def dijkstra(g, src: int) -> np.ndarray:
     dist = np.empty(g.num_nodes, dtype=np.int32)
     lib().orc_dijkstra(g.num_nodes, _p(off, C.c_int64), _p(wd, C.c_int32), src, _p(dist, C.c_int32))
     return dist
+
+
+def distance_certificate(g, src: int, dist: np.ndarray) -> int:
+    """Vertices breaking the shortest-path certificate of dist from src on a
+    symmetric graph: 0 iff dist equals the distances (weights of a weighted
+    graph, unreached = INT32_MAX/2; BFS depths of an unweighted one,
+    unreached = -1).  Linear in the edges and parallel, for graphs where a
+    serial Dijkstra or BFS per root is too slow."""
+    dist = np.ascontiguousarray(dist, dtype=np.int32)
+    if g.weighted:
+        off, wd = _wcsr(g)
+        return lib().orc_distance_certificate(g.num_nodes, _p(off, C.c_int64), None, _p(wd, C.c_int32),
+                                              src, _p(dist, C.c_int32), np.iinfo(np.int32).max // 2)
+    off, dst = _csr(g)
+    return lib().orc_distance_certificate(g.num_nodes, _p(off, C.c_int64), _p(dst, C.c_int32), None,
+                                          src, _p(dist, C.c_int32), -1)
 
 
 def delta_step(g, src: int, delta: int):

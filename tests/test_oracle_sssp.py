@@ -90,3 +90,59 @@ def test_planted_relaxation_mistake_is_caught():
     bad = d.copy()
     bad[np.flatnonzero(d > 0)[3]] += 1
     assert g.verify_sssp(r, d) and not g.verify_sssp(r, bad)
+
+
+@pytest.mark.parametrize("make", [lambda: Graph.kronecker(12, weighted=True),
+                                  lambda: Graph.uniform(11, weighted=True),
+                                  lambda: Graph.grid(40)])
+def test_distance_certificate_accepts_distances_and_rejects_mistakes(make):
+    # orc_distance_certificate checks full-size results where a serial
+    # Dijkstra or BFS per root is too slow.  It must accept exactly the
+    # Dijkstra distances and BFS depths, and reject each planted mistake: a
+    # distance one too high or too low, a reached vertex marked unreached, a
+    # source at distance 1, and two distances swapped.
+    wg = make()
+    g = wg.strip_weights()
+    rng = np.random.default_rng(1)
+    for r in wg.pick_sources(3):
+        r = int(r)
+        for gg, want, unreached in ((wg, oracle.dijkstra(wg, r), INF),
+                                    (g, oracle.bfs_depth(g, r), -1)):
+            assert oracle.distance_certificate(gg, r, want) == 0
+            far = int(np.argmax(np.where(want == unreached, -1, want)))
+            nearer = np.flatnonzero((want != unreached) & (want < want[far]) & (np.arange(len(want)) != r))
+            v = int(rng.choice(nearer))
+            for mutate in ("+1", "-1", "unreached", "source", "swap"):
+                x = want.copy()
+                if mutate == "+1":
+                    x[v] += 1
+                elif mutate == "-1":
+                    x[v] -= 1
+                elif mutate == "unreached":
+                    x[v] = unreached
+                elif mutate == "source":
+                    x[r] = 1
+                else:
+                    x[v], x[far] = want[far], want[v]
+                assert oracle.distance_certificate(gg, r, x) > 0, (r, mutate)
+
+
+def test_distance_certificate_unreachable_part():
+    # Two components: the far one must stay unreached; marking one of its
+    # vertices reached at any distance breaks the certificate.
+    u = np.array([0, 1, 2, 10, 11], np.int32)
+    v = np.array([1, 2, 3, 11, 12], np.int32)
+    w = np.array([5, 1, 7, 2, 2], np.int32)
+    wg = Graph.from_edges(u, v, w)
+    d = oracle.dijkstra(wg, 0)
+    assert oracle.distance_certificate(wg, 0, d) == 0
+    assert d[11] == INF
+    for value in (0, 1, 13, 2**20):
+        x = d.copy()
+        x[11] = value
+        assert oracle.distance_certificate(wg, 0, x) > 0
+    g = wg.strip_weights()
+    b = oracle.bfs_depth(g, 0)
+    assert oracle.distance_certificate(g, 0, b) == 0
+    b[12] = 2
+    assert oracle.distance_certificate(g, 0, b) > 0

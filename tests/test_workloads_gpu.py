@@ -1,0 +1,193 @@
+"""The named workloads at their full sizes, on the degree-ordered layout that
+the benchmark measures.
+
+Kronecker scale 22 (4,194,304 vertices, edge factor 16, seed 8675309), runs
+by default: all six kernels on the same graph and roots against the CPU
+oracle.
+  * BFS: depths equal the serial BFS (src/bfs.cc:204-215) and parents pass
+    the reference verifier (src/bfs.cc:204-241), single-root and batched.
+  * SSSP (weights uniform in [1, 255]): distances equal Dijkstra
+    (src/sssp.cc:156-181) at three bucket widths.
+  * PageRank: FP32 within 1e-6 of the FP64 oracle (src/pr_spmv.cc:35-61).
+  * CC: labels equal the smallest id per component.
+  * BC from 4 roots: bit-exact with the float Brandes (src/bc.cc:169-220)
+    on the relabelled graph, and FP64 within 1e-7 of the FP64 oracle.
+  * TC: equals OrderedCount (src/tc.cc:52-71) on the relabelled graph.
+
+Uniform scale 25 (33,554,432 vertices, degree 16, seed 1234, weights uniform
+in [1, 255]; the unweighted graph is the weighted one without its weights)
+and Kronecker scale 26 (67,108,864 vertices, seed 42) take about a minute
+each on a 16-core host (graph building and the CPU checks); GAPBS_FULL_SIZE=0
+skips them:
+  * U25: PageRank FP32 against the FP64 oracle (max-abs and L1 error
+    printed); BC from 16 roots, FP32 scores with FP64 path counts, against
+    the FP64 Brandes; SSSP distances certified exact
+    (oracle.distance_certificate) at two bucket widths.
+  * K26: the batched BFS of 64 roots equals 64 single-root BFS runs, with
+    four roots certified and one root's parents verified; the batched BC of
+    64 roots equals the single-root BC over the same roots bit for bit.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from gapbs_amd import gpu
+from gapbs_amd.graph import Graph
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+INF = np.iinfo(np.int32).max // 2
+full_size = pytest.mark.skipif(os.environ.get("GAPBS_FULL_SIZE") == "0",
+                               reason="GAPBS_FULL_SIZE=0 skips the U25 and K26 workloads")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+class TestKronecker22:
+    @pytest.fixture(scope="class")
+    def k22(self):
+        say("[k22] building kronecker 22 seed 8675309 (unweighted and weighted)")
+        g = Graph.kronecker(22, seed=8675309)
+        wg = Graph.kronecker(22, seed=8675309, weighted=True)
+        dg = gpu.DeviceGraph(g, order_by_degree=True)
+        dwg = gpu.DeviceGraph(wg, order_by_degree=True)
+        w = {"g": g, "wg": wg, "dg": dg, "dwg": dwg, "roots": [int(r) for r in g.pick_sources(4)],
+             "wroots": [int(r) for r in wg.pick_sources(2)]}
+        yield w
+        dg.free()
+        dwg.free()
+
+    def test_bfs(self, k22):
+        g, dg, roots = k22["g"], k22["dg"], k22["roots"]
+        batch = gpu.bfs_batch(dg, roots)
+        for i, r in enumerate(roots):
+            say(f"[k22] bfs root {r}")
+            want = oracle.bfs_depth(g, r)
+            res = gpu.bfs(dg, r)
+            assert np.array_equal(res.depth, want), r
+            assert np.array_equal(batch.depths[i], want), r
+            assert g.verify_bfs(r, res.parent), r
+
+    def test_sssp(self, k22):
+        wg, dwg = k22["wg"], k22["dwg"]
+        for r in k22["wroots"]:
+            want = oracle.dijkstra(wg, r)
+            for delta in (1, 32, 1024):
+                say(f"[k22] sssp root {r} delta {delta}")
+                assert np.array_equal(gpu.sssp(dwg, r, delta).dist, want), (r, delta)
+
+    def test_pagerank(self, k22):
+        g, dg = k22["g"], k22["dg"]
+        say("[k22] pagerank")
+        want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+        r32 = gpu.pagerank(dg, 20, 1e-4)
+        r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
+        assert r32.stats["iterations"] == it and r64.stats["iterations"] == it
+        assert np.abs(r32.scores.astype(np.float64) - want).max() < 1e-6
+        assert (np.abs(r64.scores.astype(np.float64) - want) / want).max() < 1e-6  # float output rounding
+        assert np.abs(gpu.pagerank(dg, 20, 1e-4, gauss_seidel=True).scores - want).max() < 1e-4
+        assert g.verify_pr(r32.scores, 1e-4)
+
+    def test_cc(self, k22):
+        say("[k22] cc")
+        assert np.array_equal(gpu.cc(k22["dg"]).comp, oracle.cc_min_labels(k22["g"]))
+
+    def test_bc(self, k22):
+        g, dg, roots = k22["g"], k22["dg"], k22["roots"]
+        say("[k22] bc 4 roots")
+        new_id = dg.vertex_order()
+        want = oracle.brandes(g.relabel_by_degree(), new_id[roots])[new_id]
+        assert np.array_equal(gpu.bc(dg, roots).scores, want)
+        assert np.array_equal(gpu.bc_batch(dg, roots).scores, want)
+        want64 = oracle.brandes(g, roots, fp64=True)
+        assert np.abs(gpu.bc(dg, roots, fp64=True).scores - want64).max() < 1e-7
+        assert np.abs(gpu.bc(dg, roots, kahan=True).scores - want64).max() < 2e-6
+
+    def test_tc(self, k22):
+        say("[k22] tc")
+        count, _ = oracle.ordered_count(k22["g"].relabel_by_degree())
+        assert gpu.tc(k22["dg"])[0] == count
+
+
+@full_size
+class TestUniform25:
+    @pytest.fixture(scope="class")
+    def u25(self):
+        say("[u25] building uniform 25 seed 1234 (weighted)")
+        wg = Graph.uniform(25, seed=1234, weighted=True)
+        g = wg.strip_weights()
+        say("[u25] uploading")
+        w = {"g": g, "wg": wg, "dg": gpu.DeviceGraph(g, order_by_degree=True),
+             "dwg": gpu.DeviceGraph(wg, order_by_degree=True)}
+        yield w
+        w["dg"].free()
+        w["dwg"].free()
+
+    def test_pagerank_fp32_against_fp64(self, u25):
+        say("[u25] pagerank")
+        want, it, _ = oracle.pagerank(u25["g"], 20, 1e-4, fp64=True)
+        res = gpu.pagerank(u25["dg"], 20, 1e-4)
+        err = np.abs(res.scores.astype(np.float64) - want)
+        say(f"[u25] pagerank FP32 vs FP64: max-abs {err.max():.3e}, L1 {err.sum():.3e}, {it} iterations")
+        assert res.stats["iterations"] == it
+        assert err.max() < 1e-6
+
+    def test_bc_16_roots_against_fp64(self, u25):
+        g, dg = u25["g"], u25["dg"]
+        roots = [int(r) for r in g.pick_sources(16)]
+        say("[u25] bc 16 roots")
+        got = gpu.bc_batch(dg, roots).scores
+        assert np.array_equal(gpu.bc(dg, roots).scores, got)
+        say("[u25] FP64 Brandes on the host")
+        want = oracle.brandes(g, roots, fp64=True)
+        err = np.abs(got.astype(np.float64) - want)
+        say(f"[u25] bc FP32 vs FP64: max-abs {err.max():.3e}, L1 {err.sum():.3e}")
+        assert err.max() < 1e-5
+
+    def test_sssp_certified(self, u25):
+        wg, dwg = u25["wg"], u25["dwg"]
+        for r in [int(r) for r in wg.pick_sources(2)]:
+            first = None
+            for delta in (32, 256):
+                say(f"[u25] sssp root {r} delta {delta}")
+                dist = gpu.sssp(dwg, r, delta).dist
+                assert oracle.distance_certificate(wg, r, dist) == 0, (r, delta)
+                if first is not None:
+                    assert np.array_equal(dist, first)
+                first = dist
+
+
+@full_size
+class TestKronecker26:
+    @pytest.fixture(scope="class")
+    def k26(self):
+        say("[k26] building kronecker 26 seed 42")
+        g = Graph.kronecker(26, seed=42)
+        say("[k26] uploading")
+        w = {"g": g, "dg": gpu.DeviceGraph(g, order_by_degree=True),
+             "roots": [int(r) for r in g.pick_sources(64)]}
+        yield w
+        w["dg"].free()
+
+    def test_batched_bfs_64_roots(self, k26):
+        g, dg, roots = k26["g"], k26["dg"], k26["roots"]
+        say("[k26] batched bfs of 64 roots")
+        batch = gpu.bfs_batch(dg, roots)
+        for i, r in enumerate(roots):
+            single = gpu.bfs(dg, r, want_parent=(i == 0))
+            assert np.array_equal(batch.depths[i], single.depth), r
+            if i < 4:
+                say(f"[k26] certifying root {r}")
+                assert oracle.distance_certificate(g, r, single.depth) == 0, r
+            if i == 0:
+                say(f"[k26] verifying parents of root {r}")
+                assert g.verify_bfs(r, single.parent)
+
+    def test_batched_bc_64_roots(self, k26):
+        say("[k26] batched bc of 64 roots against 64 single-root runs")
+        dg, roots = k26["dg"], k26["roots"]
+        assert np.array_equal(gpu.bc_batch(dg, roots).scores, gpu.bc(dg, roots).scores)
