@@ -40,12 +40,18 @@ constexpr int kDeltaLoads = 16;  // (sigma, delta) row pairs in flight per wavef
 // Dependency folds of vertices with at least kHubFold successors run in
 // MbDeltaHub: kHubProducers wavefronts compute the terms of kHubEdges
 // successors each per round into an LDS double buffer, one wavefront folds
-// them in list order.
+// them in list order.  Small blocks (three wavefronts, 16 KB of LDS, at
+// least kHubWavesPerSimd wavefronts per SIMD) put nine hubs on each CU at
+// once; at Kronecker scale 26 with 64 roots this shape took the batched BC
+// from 931 to 791 ms against 8 or 15 producers per block and 1 or 2 blocks
+// per CU.
 constexpr int64_t kHubFold = 2048;
-constexpr int kHubProducers = 8;
+constexpr int kHubProducers = 2;
 constexpr int kHubEdges = 8;
 constexpr int kHubRound = kHubProducers * kHubEdges;
 constexpr int kHubBS = (kHubProducers + 1) * kWave;
+constexpr int kHubWavesPerSimd = 7;
+constexpr int kHubBlocksPerCu = 8;
 
 __global__ void MbRootSigma(const int32_t* __restrict__ roots, int k, double* __restrict__ sigma) {
   const int r = threadIdx.x;
@@ -313,8 +319,14 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
 // previous round's half in successor order -- the same terms, in the same
 // order, with the same arithmetic as DeltaVertex, so the result is
 // bit-identical; only the loads and divisions run in parallel.
+//
+// A producer loads successor ids two rounds and their level masks one round
+// before use, so a round waits on one load, its sigma/delta rows.  Row loads
+// are unconditional (a lane whose successor is not on its root's next level
+// reads the hub's own row and its term is dropped), which keeps the round's
+// code free of branches and its one wait exact.
 template <class T>
-__global__ __launch_bounds__(kHubBS) void MbDeltaHub(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+__global__ __launch_bounds__(kHubBS, kHubWavesPerSimd) void MbDeltaHub(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                      const uint64_t* __restrict__ cur,
                                                      const uint64_t* __restrict__ next,
                                                      const double* __restrict__ sigma, T* __restrict__ delta,
@@ -329,42 +341,57 @@ __global__ __launch_bounds__(kHubBS) void MbDeltaHub(const int64_t* __restrict__
     if (fu == 0) continue;  // uniform per block
     const int64_t b = off[u];
     const int64_t e = next ? off[u + 1] : b;  // deepest level: no successors
-    const bool mine = (fu >> lane) & 1;
-    const double su = mine ? sigma[u * 64 + lane] : 1.0;
-    T acc = T(0);
     const int64_t rounds = (e - b + kHubRound - 1) / kHubRound;
-    for (int64_t k = 0; k <= rounds; k++) {
-      if (wid < kHubProducers) {
-        if (k < rounds) {
-          const int64_t base = b + k * kHubRound + wid * kHubEdges;
-          double* out = terms[k & 1] + wid * kHubEdges * kWave;
-          int32_t vl = 0;
-          uint64_t ml = 0;
-          if (lane < kHubEdges && base + lane < e) {
-            vl = dst[base + lane];
-            ml = next[vl] & fu;
-          }
-          double sv[kHubEdges];
-          T dv[kHubEdges];
-          bool on[kHubEdges];
+    if (wid < kHubProducers) {
+      const double su = ((fu >> lane) & 1) ? sigma[u * 64 + lane] : 1.0;
+      const int64_t mine = b + wid * kHubEdges;  // slot 0 of round 0
+      // Lane j < kHubEdges: successor slot j of round k (any in-range entry
+      // past the end; such slots are dropped below).
+      auto id_at = [&](int64_t k) -> int32_t {
+        const int64_t i = mine + k * kHubRound + lane;
+        return dst[lane < kHubEdges && i < e ? i : b];
+      };
+      int32_t v_cur = 0, v_nxt = 0;
+      uint64_t m_cur = 0;
+      if (rounds > 0) {
+        v_cur = id_at(0);
+        v_nxt = id_at(1);
+        m_cur = next[v_cur];
+      }
+      for (int64_t k = 0; k < rounds; k++) {
+        const int32_t v_nxt2 = id_at(k + 2);
+        const uint64_t m_nxt = next[v_nxt];
+        const int64_t first = mine + k * kHubRound;
+        double sv[kHubEdges];
+        T dv[kHubEdges];
+        uint32_t on = 0;
 #pragma unroll
-          for (int j = 0; j < kHubEdges; j++) {
-            const int32_t v = __shfl(vl, j);
-            const uint64_t m = __shfl(ml, j);
-            on[j] = (m >> lane) & 1;
-            sv[j] = 1.0;
-            dv[j] = T(0);
-            if (on[j]) {
-              sv[j] = sigma[int64_t(v) * 64 + lane];
-              dv[j] = delta[int64_t(v) * 64 + lane];
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < kHubEdges; j++) out[j * kWave + lane] = on[j] ? Term(su, sv[j], dv[j]) : -1.0;
+        for (int j = 0; j < kHubEdges; j++) {
+          const int32_t v = __shfl(v_cur, j);
+          const uint64_t m = __shfl(m_cur, j) & fu;
+          const bool o = first + j < e && ((m >> lane) & 1);
+          const int64_t row = (o ? int64_t(v) : u) * 64 + lane;
+          sv[j] = sigma[row];
+          dv[j] = delta[row];
+          on |= static_cast<uint32_t>(o) << j;
         }
-      } else if (k >= 1) {
-        const double* in = terms[(k - 1) & 1];
-        const int64_t first = b + (k - 1) * kHubRound;
+        double* out = terms[k & 1] + wid * kHubEdges * kWave;
+#pragma unroll
+        for (int j = 0; j < kHubEdges; j++) {
+          const double t = Term(su, sv[j], dv[j]);
+          out[j * kWave + lane] = ((on >> j) & 1) ? t : -1.0;
+        }
+        v_cur = v_nxt;
+        v_nxt = v_nxt2;
+        m_cur = m_nxt;
+        __syncthreads();
+      }
+      __syncthreads();  // the consumer's last fold
+    } else {
+      T acc = T(0);
+      auto fold = [&](int64_t k) {
+        const double* in = terms[k & 1];
+        const int64_t first = b + k * kHubRound;
         const int cnt = static_cast<int>(e - first < kHubRound ? e - first : kHubRound);
         int j = 0;
         for (; j + 8 <= cnt; j += 8) {  // eight LDS reads in flight, then the ordered fold
@@ -379,11 +406,14 @@ __global__ __launch_bounds__(kHubBS) void MbDeltaHub(const int64_t* __restrict__
           const double t = in[j * kWave + lane];
           if (t >= 0.0) acc = Fold(acc, t);
         }
+      };
+      for (int64_t k = 0; k < rounds; k++) {
+        if (k >= 1) fold(k - 1);
+        __syncthreads();
       }
+      if (rounds > 0) fold(rounds - 1);
       __syncthreads();
-    }
-    if (wid == kHubProducers) {
-      if (mine) delta[u * 64 + lane] = acc;
+      if ((fu >> lane) & 1) delta[u * 64 + lane] = acc;
       if (lane == 0) ex += static_cast<unsigned long long>(e - b);
     }
   }
@@ -509,7 +539,7 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
     if (nhubs > 0) {
       GG_HIP(hipEventRecord(g->ev_fork, st));
       GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
-      RunOn(g, g->side, "msbc_delta_hub", 0, MbDeltaHub<T>, dim3(GridFor(nhubs, 1, g->cu_count * 2)), dim3(kHubBS),
+      RunOn(g, g->side, "msbc_delta_hub", 0, MbDeltaHub<T>, dim3(GridFor(nhubs, 1, g->cu_count * kHubBlocksPerCu)), dim3(kHubBS),
             static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
             static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
             static_cast<const int32_t*>(hubs), nhubs, ctr + kMbStripes);
