@@ -84,8 +84,8 @@ struct gg_graph {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  hipStream_t side = nullptr;  // EnsureSideStream
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t side = nullptr, side2 = nullptr;  // EnsureSideStream
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
   int64_t n = 0;
   int64_t out_entries = 0, in_entries = 0;
   bool directed = false, weighted = false;
@@ -108,6 +108,7 @@ struct gg_graph {
   int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
   int64_t ms_heavy_chunks = 0;           // and their in-edge chunks
   int64_t mb_hubs = -1;                  // batched BC: vertices whose dependency folds are split
+  int64_t mb_big_hubs = -1;              // (those below and above the wide-block cut-off)
   hipGraphExec_t ms_exec = nullptr;      // batched traversal: two captured levels, replayed
   std::vector<uintptr_t> ms_exec_key;    // the buffers and arguments captured in it
   int ms_exec_kernels = 0;               // kernels per replay
@@ -228,7 +229,7 @@ int Run(gg_graph* g, const char* name, double bytes, void (*k)(KArgs...), dim3 g
   return RunOn(g, g->stream, name, bytes, k, grid, block, args...);
 }
 
-// A second stream of the handle (default priority, created on first use)
+// Two more streams of the handle (default priority, created on first use)
 // for kernels that may overlap the main stream's, with fork/join events.
 void EnsureSideStream(gg_graph* g);
 

@@ -106,8 +106,10 @@ void BeginTiming(gg_graph* g) { GG_HIP(hipEventRecord(g->ev_start, g->stream)); 
 void EnsureSideStream(gg_graph* g) {
   if (g->side) return;
   GG_HIP(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+  GG_HIP(hipStreamCreateWithFlags(&g->side2, hipStreamNonBlocking));
   GG_HIP(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
   GG_HIP(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
+  GG_HIP(hipEventCreateWithFlags(&g->ev_join2, hipEventDisableTiming));
 }
 
 void FinishTiming(gg_graph* g, gg_stats* s, const WallTimer& wt) {
@@ -373,9 +375,12 @@ int gg_graph_free(gg_graph* g) {
     }
     for (hipEvent_t e : g->event_pool) (void)hipEventDestroy(e);
     if (g->side) (void)hipStreamSynchronize(g->side);
+    if (g->side2) (void)hipStreamSynchronize(g->side2);
     if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
     if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+    if (g->ev_join2) (void)hipEventDestroy(g->ev_join2);
     if (g->side) (void)hipStreamDestroy(g->side);
+    if (g->side2) (void)hipStreamDestroy(g->side2);
     if (g->ev_start) (void)hipEventDestroy(g->ev_start);
     if (g->ev_stop) (void)hipEventDestroy(g->ev_stop);
     if (g->stream) (void)hipStreamDestroy(g->stream);

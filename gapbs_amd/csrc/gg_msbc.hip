@@ -36,22 +36,21 @@ namespace {
 constexpr int kMbBS = 256;
 constexpr int kMbStripes = 64;
 constexpr int kSigmaLoads = 8;   // sigma rows in flight per wavefront step
-constexpr int kDeltaLoads = 16;  // (sigma, delta) row pairs in flight per wavefront step
+constexpr int kDeltaLoads = 8;   // (sigma, delta) row pairs in flight per wavefront step
 // Dependency folds of vertices with at least kHubFold successors run in
-// MbDeltaHub: kHubProducers wavefronts compute the terms of kHubEdges
+// MbDeltaHub: P producer wavefronts compute the terms of kHubEdges
 // successors each per round into an LDS double buffer, one wavefront folds
-// them in list order.  Small blocks (three wavefronts, 16 KB of LDS, at
-// least kHubWavesPerSimd wavefronts per SIMD) put nine hubs on each CU at
-// once; at Kronecker scale 26 with 64 roots this shape took the batched BC
-// from 931 to 791 ms against 8 or 15 producers per block and 1 or 2 blocks
-// per CU.
+// them in list order.  Most hubs go to small blocks (two producers, three
+// wavefronts, 16 KB of LDS, at least seven wavefronts per SIMD: nine hubs on
+// a CU at once); hubs of at least kHubWide successors, whose fold would
+// otherwise outlast the rest of the level, go to wide blocks (fifteen
+// producers, 120 successors per round).
 constexpr int64_t kHubFold = 2048;
-constexpr int kHubProducers = 2;
+constexpr int64_t kHubWide = 65536;
 constexpr int kHubEdges = 8;
-constexpr int kHubRound = kHubProducers * kHubEdges;
-constexpr int kHubBS = (kHubProducers + 1) * kWave;
-constexpr int kHubWavesPerSimd = 7;
-constexpr int kHubBlocksPerCu = 8;
+constexpr int kHubSmall = 2;
+constexpr int kHubLarge = 15;
+constexpr int kHubSmallBlocksPerCu = 8;
 
 __global__ void MbRootSigma(const int32_t* __restrict__ roots, int k, double* __restrict__ sigma) {
   const int r = threadIdx.x;
@@ -325,13 +324,14 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
 // are unconditional (a lane whose successor is not on its root's next level
 // reads the hub's own row and its term is dropped), which keeps the round's
 // code free of branches and its one wait exact.
-template <class T>
-__global__ __launch_bounds__(kHubBS, kHubWavesPerSimd) void MbDeltaHub(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+template <class T, int P>
+__global__ __launch_bounds__((P + 1) * kWave, P <= kHubSmall ? 7 : 1) void MbDeltaHub(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                      const uint64_t* __restrict__ cur,
                                                      const uint64_t* __restrict__ next,
                                                      const double* __restrict__ sigma, T* __restrict__ delta,
                                                      const int32_t* __restrict__ hubs, int32_t nhubs,
                                                      unsigned long long* examined) {
+  constexpr int kHubRound = P * kHubEdges;
   __shared__ double terms[2][kHubRound * kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
   unsigned long long ex = 0;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kHubBS, kHubWavesPerSimd) void MbDeltaHub(const int
     const int64_t b = off[u];
     const int64_t e = next ? off[u + 1] : b;  // deepest level: no successors
     const int64_t rounds = (e - b + kHubRound - 1) / kHubRound;
-    if (wid < kHubProducers) {
+    if (wid < P) {
       const double su = ((fu >> lane) & 1) ? sigma[u * 64 + lane] : 1.0;
       const int64_t mine = b + wid * kHubEdges;  // slot 0 of round 0
       // Lane j < kHubEdges: successor slot j of round k (any in-range entry
@@ -417,15 +417,18 @@ __global__ __launch_bounds__(kHubBS, kHubWavesPerSimd) void MbDeltaHub(const int
       if (lane == 0) ex += static_cast<unsigned long long>(e - b);
     }
   }
-  if (wid == kHubProducers && lane == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+  if (wid == P && lane == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
 }
 
-// Vertices with at least kHubFold out-edges (once per graph).
-__global__ __launch_bounds__(kMbBS) void MbHubList(const int64_t* __restrict__ off, int64_t n, int32_t* out,
-                                                   int32_t* tail) {
+// Vertices with at least kHubFold out-edges, split at kHubWide (once per
+// graph).
+__global__ __launch_bounds__(kMbBS) void MbHubList(const int64_t* __restrict__ off, int64_t n, int32_t* small,
+                                                   int32_t* wide, int32_t* tails) {
   for (int64_t base = int64_t(blockIdx.x) * kMbBS; base < n; base += int64_t(gridDim.x) * kMbBS) {
     const int64_t v = base + threadIdx.x;
-    WaveAppend(v < n && off[v + 1] - off[v] >= kHubFold, static_cast<int32_t>(v), out, tail);
+    const int64_t d = v < n ? off[v + 1] - off[v] : 0;
+    WaveAppend(d >= kHubFold && d < kHubWide, static_cast<int32_t>(v), small, tails);
+    WaveAppend(d >= kHubWide, static_cast<int32_t>(v), wide, tails + 1);
   }
 }
 
@@ -518,39 +521,55 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
         static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, hchunks, nhchunks, ctr);
     launches += 2;
   }
-  // Hubs (their folds split across a block, MbDeltaHub): listed once per graph.
-  int32_t* hubs = g->Scratch<int32_t>("mb.hubs", n + 1);
+  // Hubs (their folds split across a block, MbDeltaHub): listed once per
+  // graph, at most m / kHubFold of them.
+  const int64_t max_hubs = std::min<int64_t>(n, g->out_entries / kHubFold + 1);
+  int32_t* hubs = g->Scratch<int32_t>("mb.hubs", max_hubs);
+  int32_t* wide = g->Scratch<int32_t>("mb.wide_hubs", std::min<int64_t>(n, g->out_entries / kHubWide + 1));
   if (g->mb_hubs < 0) {
-    int32_t* tail = g->Scratch<int32_t>("mb.hub_tail", 1);
-    GG_HIP(hipMemsetAsync(tail, 0, sizeof(int32_t), st));
+    int32_t* tails = g->Scratch<int32_t>("mb.hub_tails", 2);
+    GG_HIP(hipMemsetAsync(tails, 0, 2 * sizeof(int32_t), st));
     Run(g, "msbc_hub_list", 0, MbHubList, dim3(GridFor(n, kMbBS, g->cu_count * 4)), blk,
-        static_cast<const int64_t*>(g->out_off), n, hubs, tail);
-    int32_t h = 0;
-    GG_HIP(hipMemcpyAsync(&h, tail, sizeof(h), hipMemcpyDeviceToHost, st));
+        static_cast<const int64_t*>(g->out_off), n, hubs, wide, tails);
+    int32_t h[2] = {0, 0};
+    GG_HIP(hipMemcpyAsync(h, tails, sizeof(h), hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
-    g->mb_hubs = h;
+    g->mb_hubs = h[0];
+    g->mb_big_hubs = h[1];
   }
-  const int32_t nhubs = static_cast<int32_t>(g->mb_hubs);
-  if (nhubs > 0) EnsureSideStream(g);
+  const int32_t nhubs = static_cast<int32_t>(g->mb_hubs), nwide = static_cast<int32_t>(g->mb_big_hubs);
+  if (nhubs + nwide > 0) EnsureSideStream(g);
   for (int L = levels - 1; L >= 0; L--) {
     const uint64_t* next = L + 1 < levels ? t.frontiers[L + 1] : nullptr;
-    // The hubs' folds (side stream) overlap the other vertices' (main
-    // stream): they write disjoint rows and read only level L + 1's.
-    if (nhubs > 0) {
-      GG_HIP(hipEventRecord(g->ev_fork, st));
+    // The hubs' folds (side streams, wide blocks first) overlap the other
+    // vertices' (main stream): they write disjoint rows and read only level
+    // L + 1's.
+    if (nhubs + nwide > 0) GG_HIP(hipEventRecord(g->ev_fork, st));
+    if (nwide > 0) {
       GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
-      RunOn(g, g->side, "msbc_delta_hub", 0, MbDeltaHub<T>, dim3(GridFor(nhubs, 1, g->cu_count * kHubBlocksPerCu)), dim3(kHubBS),
+      RunOn(g, g->side, "msbc_delta_hub_wide", 0, MbDeltaHub<T, kHubLarge>, dim3(GridFor(nwide, 1, g->cu_count)),
+            dim3((kHubLarge + 1) * kWave), static_cast<const int64_t*>(g->out_off),
+            static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
+            static_cast<const double*>(sigma), delta, static_cast<const int32_t*>(wide), nwide, ctr + kMbStripes);
+      GG_HIP(hipEventRecord(g->ev_join, g->side));
+      launches++;
+    }
+    if (nhubs > 0) {
+      GG_HIP(hipStreamWaitEvent(g->side2, g->ev_fork, 0));
+      RunOn(g, g->side2, "msbc_delta_hub", 0, MbDeltaHub<T, kHubSmall>,
+            dim3(GridFor(nhubs, 1, g->cu_count * kHubSmallBlocksPerCu)), dim3((kHubSmall + 1) * kWave),
             static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
             static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
             static_cast<const int32_t*>(hubs), nhubs, ctr + kMbStripes);
-      GG_HIP(hipEventRecord(g->ev_join, g->side));
+      GG_HIP(hipEventRecord(g->ev_join2, g->side2));
       launches++;
     }
     Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
         static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
         static_cast<const double*>(sigma), delta, n, ctr + kMbStripes);
     launches++;
-    if (nhubs > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
+    if (nwide > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
+    if (nhubs > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join2, 0));
   }
   Run(g, "msbc_scores", 8.0 * N + sizeof(T) * N, MbScores<T>, grid_n, blk, static_cast<const uint64_t*>(t.seen),
       static_cast<const T*>(delta), n, score);

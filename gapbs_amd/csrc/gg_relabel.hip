@@ -352,6 +352,7 @@ extern "C" int gg_graph_order_by_degree(gg_graph* g, gg_stats* stats) {
     g->ms_heavy_chunks = 0;
     g->cc_samples_n = -1;
     g->mb_hubs = -1;
+    g->mb_big_hubs = -1;
     if (g->ms_exec) {
       (void)hipGraphExecDestroy(g->ms_exec);
       g->ms_exec = nullptr;

@@ -118,23 +118,24 @@ def test_batch_bc_errors():
 def test_batch_bc_hub_folds(ordered):
     # Vertices with 2048+ successors have their dependency folds split over
     # a block (terms computed in parallel, folded in list order by one
-    # wavefront): three such hubs sharing neighbours, a partial last round,
-    # and roots placed so the hubs sit at several levels.
+    # wavefront), in wide blocks from 65536 successors: four such hubs
+    # sharing neighbours (one wide), partial last rounds, and roots placed
+    # so the hubs sit at several levels.
     print(f"[msbc] hub folds ordered={ordered}", flush=True)
     rng = np.random.default_rng(17)
-    n = 30000
-    hubs = [0, 1, 2]
-    degs = [20000, 5000, 2048]
+    n = 150000
+    hubs = [0, 1, 2, 3]
+    degs = [70001, 20000, 5000, 2048]
     u, v = [], []
     for h, d in zip(hubs, degs):
-        nb = rng.choice(np.arange(3, n), size=d, replace=False)
+        nb = rng.choice(np.arange(4, n), size=d, replace=False)
         u.append(np.full(d, h))
         v.append(nb)
-    u.append(rng.integers(3, n, 60000))
-    v.append(rng.integers(3, n, 60000))
+    u.append(rng.integers(4, n, 300000))
+    v.append(rng.integers(4, n, 300000))
     g = Graph.from_edges(np.concatenate(u), np.concatenate(v))
     dg = gpu.DeviceGraph(g, order_by_degree=ordered)
-    roots = np.concatenate([[0, 1, 2], rng.choice(np.arange(3, n), 29, replace=False)]).astype(np.int32)
+    roots = np.concatenate([hubs, rng.choice(np.arange(4, n), 28, replace=False)]).astype(np.int32)
     if not ordered:
         r = check(g, dg, roots)
     else:
@@ -143,4 +144,5 @@ def test_batch_bc_hub_folds(ordered):
         assert np.array_equal(gpu.bc_batch(dg, roots).scores, want)
     dg.profile(True)
     gpu.bc_batch(dg, roots)
-    assert "msbc_delta_hub" in dg.profile_report()
+    rep = dg.profile_report()
+    assert "msbc_delta_hub" in rep and "msbc_delta_hub_wide" in rep
