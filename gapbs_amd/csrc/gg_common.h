@@ -148,6 +148,8 @@ struct gg_graph {
 
 namespace gg {
 
+// True once gg_init has run in this process.
+bool GpuInitialised();
 // A new undirected, unweighted handle on `device` with uninitialised device
 // arrays for n vertices and `entries` destinations (filled by the caller).
 gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries);

@@ -225,6 +225,8 @@ void ValidateSide(const int64_t* off, const int32_t* dst, int64_t entries, int w
 
 }  // namespace
 
+bool GpuInitialised() { return g_initialised; }
+
 gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries) {
   GG_REQUIRE(n >= 0 && n < (int64_t(1) << 31), GG_EINVAL, "num_nodes out of the int32 id range");
   gg_graph* g = new gg_graph;

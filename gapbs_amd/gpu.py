@@ -76,6 +76,7 @@ EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
+    "gg_graph_build", "gg_graph_copy_in",
     "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
@@ -111,6 +112,9 @@ def lib() -> C.CDLL:
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
         L.gg_relabel_by_degree.argtypes = [vp, C.POINTER(vp), C.POINTER(Stats)]
         L.gg_graph_copy_out.argtypes = [vp, _i64p, _i32p]
+        L.gg_graph_copy_in.argtypes = [vp, _i64p, _i32p]
+        L.gg_graph_build.argtypes = [_i32p, _i32p, _i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp),
+                                     _i64p, C.POINTER(Stats)]
         L.gg_graph_order_by_degree.argtypes = [vp, C.POINTER(Stats)]
         L.gg_graph_vertex_order.argtypes = [vp, _i32p]
         L.gg_autotune.argtypes = [vp, C.c_int, _i32p, C.c_int, C.POINTER(Tuning)]
@@ -204,6 +208,14 @@ class DeviceGraph:
         _check(lib().gg_graph_copy_out(self._h, _p(off, C.c_int64), None))
         dst = np.empty(int(off[-1]) * (2 if self.weighted else 1), dtype=np.int32)
         _check(lib().gg_graph_copy_out(self._h, None, _p(dst, C.c_int32)))
+        return off, dst
+
+    def copy_in(self) -> tuple[np.ndarray, np.ndarray]:
+        """(offsets[n + 1], dests) of the device in-lists (gg_graph_copy_in)."""
+        off = np.empty(self.num_nodes + 1, dtype=np.int64)
+        _check(lib().gg_graph_copy_in(self._h, _p(off, C.c_int64), None))
+        dst = np.empty(int(off[-1]) * (2 if self.weighted else 1), dtype=np.int32)
+        _check(lib().gg_graph_copy_in(self._h, None, _p(dst, C.c_int32)))
         return off, dst
 
     def device_bytes(self) -> int:
@@ -466,6 +478,33 @@ def set_tuning(dg: DeviceGraph, values: dict) -> None:
             continue
         setattr(t, k, v)
     _check(lib().gg_tuning_set(dg.handle, C.byref(t)))
+
+
+def build_graph(u, v, w=None, symmetrize: bool = True, weighted: bool = False,
+                device: int = 0) -> tuple[DeviceGraph, dict]:
+    """gg_graph_build: the squished CSR of the edge list u[i] -> v[i] built on
+    the device (the graph gh_build_edges / Graph.from_edges builds on the
+    host).  Returns (handle, stats)."""
+    init(device)
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    if u.shape != v.shape:
+        raise ValueError("u and v differ in length")
+    wp = None
+    if w is not None:
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        if w.shape != u.shape:
+            raise ValueError("w and u differ in length")
+        wp = _p(w, C.c_int32)
+    h = C.c_void_p()
+    n = C.c_int64()
+    st = Stats()
+    _check(lib().gg_graph_build(_p(u, C.c_int32), _p(v, C.c_int32), wp, len(u), int(symmetrize),
+                                int(weighted), C.byref(h), C.byref(n), C.byref(st)))
+    dg = DeviceGraph._adopt(h, n.value)
+    dg.weighted = bool(weighted)
+    dg.directed = not symmetrize
+    return dg, st.as_dict()
 
 
 def relabel_by_degree(dg: DeviceGraph) -> tuple[DeviceGraph, dict]:

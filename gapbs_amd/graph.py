@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         L.gh_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(vp)]
         L.gh_build_edges.argtypes = [_i32p, _i32p, _i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]
+        L.gh_generate_edges.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_int64),
+                                        _i32p, _i32p, _i32p]
         L.gh_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp)]
         L.gh_write.argtypes = [vp, C.c_char_p]
         L.gh_free.argtypes = [vp]
@@ -117,6 +119,19 @@ class Graph:
     @classmethod
     def grid(cls, side: int, seed: int = 7, weighted=True):
         return cls._make(lib().gh_generate, GRID, side, 0, seed, int(weighted))
+
+    @staticmethod
+    def generate_edges(kind: int, scale: int, degree: int = 16, seed: int = K_RAND_SEED, weighted=False):
+        """The generator's edge list before the build (gh_generate_edges):
+        (u, v, w or None), the list kronecker()/uniform()/grid() build from."""
+        m = C.c_int64(0)
+        _check(lib().gh_generate_edges(kind, scale, degree, seed, int(weighted), C.byref(m), None, None, None))
+        u = np.empty(m.value, dtype=np.int32)
+        v = np.empty(m.value, dtype=np.int32)
+        w = np.empty(m.value, dtype=np.int32) if weighted else None
+        _check(lib().gh_generate_edges(kind, scale, degree, seed, int(weighted), C.byref(m), _ptr(u, C.c_int32),
+                                       _ptr(v, C.c_int32), _ptr(w, C.c_int32) if weighted else None))
+        return u, v, w
 
     @classmethod
     def from_edges(cls, u, v, w=None, symmetrize=True) -> "Graph":

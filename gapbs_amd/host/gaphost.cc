@@ -87,6 +87,40 @@ int gh_generate(int kind, int scale, int degree, int64_t seed, int weighted, gh_
   });
 }
 
+int gh_generate_edges(int kind, int scale, int degree, int64_t seed, int weighted, int64_t* m, int32_t* u,
+                      int32_t* v, int32_t* w) {
+  return Guard([&] {
+    if (!m) throw Error(-2, "null edge count");
+    Buf<WeightT> wt;
+    Buf<Edge> el;
+    if (kind == GH_GRID) {
+      GridSpec sp;
+      sp.side = scale;
+      sp.seed = seed;
+      el = MakeGridEL(sp, &wt);
+    } else if (kind == GH_UNIFORM) {
+      el = MakeUniformEL(scale, degree, seed);
+    } else if (kind == GH_KRONECKER) {
+      el = MakeKroneckerEL(scale, degree, seed);
+    } else {
+      throw Error(-3, "unknown generator kind");
+    }
+    const int64_t count = static_cast<int64_t>(el.size());
+    if (u) {
+      if (*m < count || !v || (weighted && !w)) throw Error(-2, "edge arrays too small or missing");
+      if (weighted && kind != GH_GRID) wt = MakeWeights(count, seed);
+#pragma omp parallel for schedule(static)
+      for (int64_t e = 0; e < count; e++) {
+        u[e] = el[e].u;
+        v[e] = el[e].v;
+        if (weighted) w[e] = wt[e];
+      }
+    }
+    *m = count;
+    return 0;
+  });
+}
+
 int gh_build_edges(const int32_t* u, const int32_t* v, const int32_t* w, int64_t m, int symmetrize,
                    int weighted, gh_graph** out) {
   return Guard([&] {

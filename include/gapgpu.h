@@ -173,6 +173,25 @@ int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats);
  * relabel's device time.  Free *out with gg_graph_free. */
 int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats);
 
+/* Builds a graph on the device from an edge list (u[i] -> v[i], optional
+ * weights w[i], m edges in host memory).  Replaces the CSR half of the
+ * reference's builder: BuilderBase::MakeGraphFromEL / MakeCSR / SquishCSR /
+ * SquishGraph (src/builder.h:300-341, 141-186), with the vertex count taken
+ * as the largest endpoint + 1 (FindMaxNodeID, builder.h:69-78).  The handle
+ * holds exactly the CSR gh_build_edges builds from the same arguments:
+ * symmetrize != 0 stores both directions of every edge; otherwise the graph
+ * is directed with out- and in-lists; weighted != 0 stores {v, w} pairs
+ * (weights 1 when w is NULL); lists are sorted by (v, w), duplicates keep the
+ * smallest weight, self-loops are dropped.  *num_nodes (nullable) receives
+ * n.  GG_EINVAL for a negative id, GG_ENOTSUP when the lists hold 2^32
+ * entries or more or 2 * bits(n - 1) + bits(weight range) exceeds 64.
+ * stats->device_ms covers the build after the edge list reached HBM. */
+int gg_graph_build(const int32_t* u, const int32_t* v, const int32_t* w, int64_t m, int symmetrize,
+                   int weighted, gg_graph** out, int64_t* num_nodes, gg_stats* stats);
+/* Copies a handle's inverse CSR (in-lists; the out-lists for undirected
+ * graphs) back to host memory, like gg_graph_copy_out. */
+int gg_graph_copy_in(const gg_graph* g, int64_t* offsets, int32_t* dests);
+
 /* Copies a handle's CSR back to host memory (offsets[n + 1] and dests, each
  * nullable): the layout gg_graph_upload takes.  A degree-ordered handle
  * (below) returns its lists in device ids. */

@@ -31,6 +31,15 @@ enum gh_kind { GH_KRONECKER = 0, GH_UNIFORM = 1, GH_GRID = 2 };
  * kind GH_GRID: `scale` is the grid side, `degree` is ignored. */
 int gh_generate(int kind, int scale, int degree, int64_t seed, int weighted, gh_graph** out);
 
+/* The generator's edge list before the build: Generator::GenerateEL
+ * (src/generator.h:157-168) with an explicit seed, plus InsertWeights
+ * (generator.h:170-187) when weighted -- the list gh_generate builds.  On
+ * input *m is the capacity of u, v (and w when weighted); on output the
+ * edge count.  With u NULL only the count is returned (for GH_KRONECKER and
+ * GH_UNIFORM it is degree << scale). */
+int gh_generate_edges(int kind, int scale, int degree, int64_t seed, int weighted, int64_t* m, int32_t* u,
+                      int32_t* v, int32_t* w);
+
 /* Builds a graph from an edge list (u[i] -> v[i], optional weights w[i]).
  * Replaces BuilderBase::MakeGraphFromEL + SquishGraph (src/builder.h:317-341, 172-186). */
 int gh_build_edges(const int32_t* u, const int32_t* v, const int32_t* w, int64_t m,
