@@ -61,7 +61,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build")
 
 
 def log(msg):
@@ -70,6 +70,34 @@ def log(msg):
 
 EXTRA_PROFILES = []  # per-kernel reports of device graphs freed inside a timed region
 PROFILING = [False]  # inside a Timed region
+
+
+def build_report(g, edges, last, per, host_build_s, no_cpu):
+    """The device CSR build's line: device time (edge list already in HBM)
+    against the compulsory bytes -- the edge list read once (8 B per edge),
+    the CSR written once (8 B per offset, 4 B per entry) -- and a check that
+    one more build equals the host builder's graph byte for byte."""
+    import numpy as np
+    from gapbs_amd import gpu
+    m = len(edges[0])
+    dev_s = last["device_ms"] / 1e3
+    alg = 8.0 * m + 8.0 * (g.num_nodes + 1) + 4.0 * int(g.out_offsets[-1])
+    db, _ = gpu.build_graph(*edges)
+    off, dst = db.copy_out()
+    db.free()
+    same = bool(np.array_equal(off, g.out_offsets) and np.array_equal(dst, g.out_neighbors))
+    assert same, "device build differs from the host builder"
+    out = {"device_seconds": round(dev_s, 6), "edges": m, "matches_host_build": same,
+           "roofline": {"bound": "hbm", "kernel": "gg_graph_build (device time)",
+                        "achieved": round(alg / dev_s / 1e9, 2), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(alg / dev_s / 8e12, 5), "traffic": None, "alg_bytes_per_launch": alg},
+           "note": "seconds include copying the edge list from host memory; device_seconds start in HBM"}
+    if not no_cpu and host_build_s is not None:
+        out["cpu"] = {"value": round(host_build_s, 3), "unit": "s", "kind": "port",
+                      "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                      "sample": "the whole edge list through the host builder (OpenMP restatement of "
+                                "BuilderBase::MakeGraphFromEL + SquishCSR, src/builder.h:300-341)"}
+    return out
 
 
 class Timed:
@@ -247,7 +275,7 @@ def main():
         sys.exit(f"unknown kernels {sorted(unknown)}")
 
     from gapbs_amd import gpu
-    from gapbs_amd.graph import Graph
+    from gapbs_amd.graph import KRONECKER, Graph
 
     workload = f"kronecker s{args.scale} ef16 seed {args.seed}"
     if args.layout == "degree":
@@ -255,7 +283,16 @@ def main():
     ordered = args.layout == "degree"
     order_ms = {}
     t0 = time.time()
-    g = Graph.kronecker(args.scale, 16, seed=args.seed)
+    build_edges, host_build_s = None, None
+    if "build" in kernels:
+        # The generator's edge list, kept for the device build; the host
+        # builds the same graph from it (timed: the build's CPU baseline).
+        build_edges = Graph.generate_edges(KRONECKER, args.scale, 16, args.seed)[:2]
+        t1 = time.time()
+        g = Graph.from_edges(*build_edges)
+        host_build_s = time.time() - t1
+    else:
+        g = Graph.kronecker(args.scale, 16, seed=args.seed)
     log(f"{g.stats_line()} built in {time.time() - t0:.1f}s")
     roots = [int(r) for r in g.pick_sources(args.roots)]
     dg = gpu.DeviceGraph(g, order_by_degree=ordered)
@@ -314,6 +351,10 @@ def main():
                 EXTRA_PROFILES.append(dr.profile_report())
             dr.free()
             return {**st, "relabel_ms": rst["device_ms"], "triangles": c}
+        elif k == "build":
+            db, st = gpu.build_graph(*build_edges)
+            db.free()
+            return st
         return None
 
     # Per-graph tuning (gg_autotune) before any timing; its own time is
@@ -372,6 +413,9 @@ def main():
             r["probes"] = last["edges_examined"]
             if "relabel_ms" in last:
                 r["relabel_ms_last"] = round(last["relabel_ms"], 3)
+        if k == "build" and last is not None:
+            r.update(build_report(g, build_edges, last, per, host_build_s, args.no_cpu))
+            build_edges = None
         results[k] = r
         log(f"{k}: {r}")
     if "bfs_batch" in results and "bfs" in results:

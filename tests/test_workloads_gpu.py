@@ -23,9 +23,11 @@ skips them:
     printed); BC from 16 roots, FP32 scores with FP64 path counts, against
     the FP64 Brandes; SSSP distances certified exact
     (oracle.distance_certificate) at two bucket widths.
-  * K26: the batched BFS of 64 roots equals 64 single-root BFS runs, with
-    four roots certified and one root's parents verified; the batched BC of
-    64 roots equals the single-root BC over the same roots bit for bit.
+  * K26: the device CSR build (gg_graph_build) of the generator's edge list
+    equals the host builder's graph byte for byte; the batched BFS of 64
+    roots equals 64 single-root BFS runs, with four roots certified and one
+    root's parents verified; the batched BC of 64 roots equals the
+    single-root BC over the same roots bit for bit.
 """
 
 import os
@@ -34,6 +36,7 @@ import numpy as np
 import pytest
 
 from gapbs_amd import gpu
+from gapbs_amd import graph as hg
 from gapbs_amd.graph import Graph
 from oracle import oracle
 
@@ -165,13 +168,23 @@ class TestUniform25:
 class TestKronecker26:
     @pytest.fixture(scope="class")
     def k26(self):
-        say("[k26] building kronecker 26 seed 42")
-        g = Graph.kronecker(26, seed=42)
+        say("[k26] generating kronecker 26 seed 42 and building on the host")
+        u, v, _ = Graph.generate_edges(hg.KRONECKER, 26, 16, 42)
+        g = Graph.from_edges(u, v)
         say("[k26] uploading")
-        w = {"g": g, "dg": gpu.DeviceGraph(g, order_by_degree=True),
+        w = {"g": g, "edges": (u, v), "dg": gpu.DeviceGraph(g, order_by_degree=True),
              "roots": [int(r) for r in g.pick_sources(64)]}
         yield w
         w["dg"].free()
+
+    def test_device_build_matches_host(self, k26):
+        say("[k26] device build of the 1.07 G-edge list")
+        db, st = gpu.build_graph(*k26["edges"])
+        say(f"[k26] device build {st['device_ms']:.1f} ms")
+        off, dst = db.copy_out()
+        db.free()
+        assert np.array_equal(off, k26["g"].out_offsets)
+        assert np.array_equal(dst, k26["g"].out_neighbors)
 
     def test_batched_bfs_64_roots(self, k26):
         g, dg, roots = k26["g"], k26["dg"], k26["roots"]
