@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         L.gh_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(vp)]
         L.gh_build_edges.argtypes = [_i32p, _i32p, _i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]
+        L.gh_permutation.argtypes = [C.c_int64, C.c_int64, _i32p]
         L.gh_generate_edges.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_int64),
                                         _i32p, _i32p, _i32p]
         L.gh_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp)]
@@ -119,6 +120,13 @@ class Graph:
     @classmethod
     def grid(cls, side: int, seed: int = 7, weighted=True):
         return cls._make(lib().gh_generate, GRID, side, 0, seed, int(weighted))
+
+    @staticmethod
+    def permutation(n: int, seed: int = K_RAND_SEED) -> np.ndarray:
+        """Generator::PermuteIDs' vertex permutation (gh_permutation)."""
+        perm = np.empty(n, dtype=np.int32)
+        _check(lib().gh_permutation(n, seed, _ptr(perm, C.c_int32)))
+        return perm
 
     @staticmethod
     def generate_edges(kind: int, scale: int, degree: int = 16, seed: int = K_RAND_SEED, weighted=False):

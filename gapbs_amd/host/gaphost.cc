@@ -87,6 +87,16 @@ int gh_generate(int kind, int scale, int degree, int64_t seed, int weighted, gh_
   });
 }
 
+int gh_permutation(int64_t n, int64_t seed, int32_t* perm) {
+  return Guard([&] {
+    if (n < 0 || n > (int64_t(1) << 31) || (n > 0 && !perm)) throw Error(-2, "bad permutation arguments");
+    for (int64_t i = 0; i < n; i++) perm[i] = static_cast<int32_t>(i);
+    std::mt19937 rng(static_cast<std::mt19937::result_type>(seed));
+    std::shuffle(perm, perm + n, rng);
+    return 0;
+  });
+}
+
 int gh_generate_edges(int kind, int scale, int degree, int64_t seed, int weighted, int64_t* m, int32_t* u,
                       int32_t* v, int32_t* w) {
   return Guard([&] {

@@ -188,6 +188,17 @@ int gg_relabel_by_degree(gg_graph* g, gg_graph** out, gg_stats* stats);
  * stats->device_ms covers the build after the edge list reached HBM. */
 int gg_graph_build(const int32_t* u, const int32_t* v, const int32_t* w, int64_t m, int symmetrize,
                    int weighted, gg_graph** out, int64_t* num_nodes, gg_stats* stats);
+/* Generates and builds on the device the symmetrised graph gh_generate
+ * builds: kind 0 Kronecker (R-MAT), 1 uniform; m = degree << scale edges;
+ * weighted: weights uniform in [1, 255].  Replaces Generator::GenerateEL and
+ * InsertWeights (src/generator.h:105-187) -- each 2^18-edge block's mt19937
+ * stream seeded with seed + block, one wavefront per block -- followed by
+ * gg_graph_build.  perm (Kronecker only, 2^scale entries) is the vertex
+ * permutation that gh_permutation returns: Generator::PermuteIDs
+ * (generator.h:93-103), a sequential std::shuffle left on the host.
+ * stats->device_ms covers generation and build. */
+int gg_graph_generate(int kind, int scale, int degree, int64_t seed, int weighted, const int32_t* perm,
+                      gg_graph** out, int64_t* num_nodes, gg_stats* stats);
 /* Copies a handle's inverse CSR (in-lists; the out-lists for undirected
  * graphs) back to host memory, like gg_graph_copy_out. */
 int gg_graph_copy_in(const gg_graph* g, int64_t* offsets, int32_t* dests);

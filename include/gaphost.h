@@ -31,6 +31,12 @@ enum gh_kind { GH_KRONECKER = 0, GH_UNIFORM = 1, GH_GRID = 2 };
  * kind GH_GRID: `scale` is the grid side, `degree` is ignored. */
 int gh_generate(int kind, int scale, int degree, int64_t seed, int weighted, gh_graph** out);
 
+/* The vertex permutation of Generator::PermuteIDs (src/generator.h:93-103):
+ * perm[i] = i shuffled by std::shuffle with an mt19937 seeded with `seed`.
+ * The Kronecker generator maps every endpoint v to perm[v]; gg_graph_generate
+ * takes this array. */
+int gh_permutation(int64_t n, int64_t seed, int32_t* perm);
+
 /* The generator's edge list before the build: Generator::GenerateEL
  * (src/generator.h:157-168) with an explicit seed, plus InsertWeights
  * (generator.h:170-187) when weighted -- the list gh_generate builds.  On

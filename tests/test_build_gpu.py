@@ -1,4 +1,4 @@
-"""Device CSR build (gg_graph_build) against the host builder.
+"""Device graph build (gg_graph_build, gg_graph_generate) against the host.
 
 The host builder (gh_build_edges) restates BuilderBase::MakeGraphFromEL,
 MakeCSR and SquishCSR (src/builder.h:300-341, 141-186) and is pinned
@@ -9,7 +9,10 @@ self-loops dropped, both directions when symmetrised, out- and in-lists when
 directed.  Cases: generated Kronecker / uniform / grid edge lists (the lists
 gh_generate builds), random directed lists with repeats, self-loops and
 conflicting weights, the empty list, a list of only self-loops, weights
-defaulting to 1, and the error paths.
+defaulting to 1, and the error paths.  Device generation (R-MAT and uniform
+edges, weights, mt19937 streams per 2^18-edge block) must give the graph
+gh_generate gives: Kronecker and uniform, weighted or not, one block or
+several, and a weight stream that hits the bounded draw's rejection.
 """
 
 import ctypes as C
@@ -102,3 +105,42 @@ def test_errors():
     with pytest.raises(gpu.GpuError) as e:
         gpu.build_graph([0, 2**31 - 2], [1, 0], [1, 8], weighted=True)
     assert e.value.code == -95
+
+
+def host_generated(kind, scale, seed, weighted):
+    make = Graph.kronecker if kind == hg.KRONECKER else Graph.uniform
+    return make(scale, 16, seed=seed, weighted=weighted)
+
+
+@pytest.mark.parametrize("kind,scale,seed,weighted", [
+    (hg.KRONECKER, 10, 27491095, False),
+    (hg.KRONECKER, 16, 27491095, True),
+    (hg.KRONECKER, 17, 8675309, False),   # several 2^18-edge blocks, each its own stream
+    (hg.UNIFORM, 12, 1234, True),
+    (hg.UNIFORM, 15, 27491095, False),
+    # mt19937 seeded with 30376 draws 0xFFFFFFFF at position 36855 of its
+    # stream: the bounded weight draw rejects it, so every later weight of
+    # the block (scale 14 x 16 = one 2^18-edge block) comes one draw later.
+    (hg.KRONECKER, 14, 30376, True),
+])
+def test_device_generation_matches_host(kind, scale, seed, weighted):
+    print(f"[build] generate kind {kind} scale {scale} seed {seed} weighted {weighted}", flush=True)
+    h = host_generated(kind, scale, seed, weighted)
+    dg, st = gpu.generate_graph(kind, scale, 16, seed, weighted)
+    assert dg.num_nodes == h.num_nodes
+    off, dst = dg.copy_out()
+    assert np.array_equal(off, h.out_offsets)
+    assert np.array_equal(dst, h.out_dests_raw.reshape(-1))
+    assert st["device_ms"] > 0
+
+
+def test_generation_errors():
+    print("[build] generation errors", flush=True)
+    i32p = C.POINTER(C.c_int32)
+    h, n = C.c_void_p(), C.c_int64()
+    for args in ((2, 10, 16), (0, 0, 16), (0, 31, 16), (1, 10, 0)):
+        rc = gpu.lib().gg_graph_generate(args[0], args[1], args[2], 1, 0, C.cast(None, i32p), C.byref(h),
+                                         C.byref(n), None)
+        assert rc in (-22, -95), (args, rc)
+    rc = gpu.lib().gg_graph_generate(0, 10, 16, 1, 0, C.cast(None, i32p), C.byref(h), C.byref(n), None)
+    assert rc == -22  # Kronecker without the permutation
