@@ -61,7 +61,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build", "generate")
 
 
 def log(msg):
@@ -97,6 +97,26 @@ def build_report(g, edges, last, per, host_build_s, no_cpu):
                       "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                       "sample": "the whole edge list through the host builder (OpenMP restatement of "
                                 "BuilderBase::MakeGraphFromEL + SquishCSR, src/builder.h:300-341)"}
+    return out
+
+
+def generate_report(g, args, last, host_gen_s, host_build_s):
+    """The device generation's line: generation + build device time, and a
+    check that one more run equals the host generator's graph byte for byte.
+    Seconds include the host's vertex permutation (a sequential shuffle)."""
+    import numpy as np
+    from gapbs_amd import gpu
+    db, _ = gpu.generate_graph(0, args.scale, 16, args.seed)
+    off, dst = db.copy_out()
+    db.free()
+    same = bool(np.array_equal(off, g.out_offsets) and np.array_equal(dst, g.out_neighbors))
+    assert same, "device generation differs from the host generator"
+    out = {"device_seconds": round(last["device_ms"] / 1e3, 6), "matches_host_graph": same,
+           "note": "seconds include the host permutation and its copy; device_seconds = generation + build"}
+    if not args.no_cpu and host_gen_s is not None:
+        out["cpu"] = {"value": round(host_gen_s + host_build_s, 3), "unit": "s", "kind": "port",
+                      "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                      "sample": "host generator (R-MAT blocks, PermuteIDs, OpenMP) + host builder, whole graph"}
     return out
 
 
@@ -283,14 +303,18 @@ def main():
     ordered = args.layout == "degree"
     order_ms = {}
     t0 = time.time()
-    build_edges, host_build_s = None, None
-    if "build" in kernels:
+    build_edges, host_build_s, host_gen_s = None, None, None
+    if "build" in kernels or "generate" in kernels:
         # The generator's edge list, kept for the device build; the host
-        # builds the same graph from it (timed: the build's CPU baseline).
+        # builds the same graph from it (timed: the CPU baselines of the
+        # device build and, with the generation, of the device generation).
         build_edges = Graph.generate_edges(KRONECKER, args.scale, 16, args.seed)[:2]
+        host_gen_s = time.time() - t0
         t1 = time.time()
         g = Graph.from_edges(*build_edges)
         host_build_s = time.time() - t1
+        if "build" not in kernels:
+            build_edges = None
     else:
         g = Graph.kronecker(args.scale, 16, seed=args.seed)
     log(f"{g.stats_line()} built in {time.time() - t0:.1f}s")
@@ -355,6 +379,10 @@ def main():
             db, st = gpu.build_graph(*build_edges)
             db.free()
             return st
+        elif k == "generate":
+            db, st = gpu.generate_graph(0, args.scale, 16, args.seed)
+            db.free()
+            return st
         return None
 
     # Per-graph tuning (gg_autotune) before any timing; its own time is
@@ -416,6 +444,8 @@ def main():
         if k == "build" and last is not None:
             r.update(build_report(g, build_edges, last, per, host_build_s, args.no_cpu))
             build_edges = None
+        if k == "generate" and last is not None:
+            r.update(generate_report(g, args, last, host_gen_s, host_build_s))
         results[k] = r
         log(f"{k}: {r}")
     if "bfs_batch" in results and "bfs" in results:

@@ -24,7 +24,8 @@ skips them:
     the FP64 Brandes; SSSP distances certified exact
     (oracle.distance_certificate) at two bucket widths.
   * K26: the device CSR build (gg_graph_build) of the generator's edge list
-    equals the host builder's graph byte for byte; the batched BFS of 64
+    and the device generation (gg_graph_generate) both equal the host's
+    graph byte for byte; the batched BFS of 64
     roots equals 64 single-root BFS runs, with four roots certified and one
     root's parents verified; the batched BC of 64 roots equals the
     single-root BC over the same roots bit for bit.
@@ -181,6 +182,15 @@ class TestKronecker26:
         say("[k26] device build of the 1.07 G-edge list")
         db, st = gpu.build_graph(*k26["edges"])
         say(f"[k26] device build {st['device_ms']:.1f} ms")
+        off, dst = db.copy_out()
+        db.free()
+        assert np.array_equal(off, k26["g"].out_offsets)
+        assert np.array_equal(dst, k26["g"].out_neighbors)
+
+    def test_device_generation_matches_host(self, k26):
+        say("[k26] device generation of kronecker 26 seed 42")
+        db, st = gpu.generate_graph(hg.KRONECKER, 26, 16, 42)
+        say(f"[k26] device generation + build {st['device_ms']:.1f} ms")
         off, dst = db.copy_out()
         db.free()
         assert np.array_equal(off, k26["g"].out_offsets)
