@@ -150,6 +150,11 @@ namespace gg {
 
 // True once gg_init has run in this process.
 bool GpuInitialised();
+// Checks device lists (offsets non-decreasing, destinations in [0, n); width
+// 2 = {v, w} pairs) and raises *max_degree to their largest degree; throws
+// GG_EINVAL otherwise.
+void ValidateLists(const int64_t* off, const int32_t* dst, int64_t entries, int width, int64_t n,
+                   int64_t* max_degree);
 // A new undirected, unweighted handle on `device` with uninitialised device
 // arrays for n vertices and `entries` destinations (filled by the caller).
 gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries);

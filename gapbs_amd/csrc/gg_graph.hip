@@ -227,6 +227,11 @@ void ValidateSide(const int64_t* off, const int32_t* dst, int64_t entries, int w
 
 bool GpuInitialised() { return g_initialised; }
 
+void ValidateLists(const int64_t* off, const int32_t* dst, int64_t entries, int width, int64_t n,
+                   int64_t* max_degree) {
+  ValidateSide(off, dst, entries, width, n, max_degree);
+}
+
 gg_graph* NewDeviceGraph(int device, int64_t n, int64_t entries) {
   GG_REQUIRE(n >= 0 && n < (int64_t(1) << 31), GG_EINVAL, "num_nodes out of the int32 id range");
   gg_graph* g = new gg_graph;

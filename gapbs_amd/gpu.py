@@ -76,7 +76,7 @@ EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
-    "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate",
+    "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate", "gg_graph_read",
     "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
         L.gg_relabel_by_degree.argtypes = [vp, C.POINTER(vp), C.POINTER(Stats)]
         L.gg_graph_copy_out.argtypes = [vp, _i64p, _i32p]
         L.gg_graph_copy_in.argtypes = [vp, _i64p, _i32p]
+        L.gg_graph_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp), _i64p, C.POINTER(Stats)]
         L.gg_graph_generate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, _i32p, C.POINTER(vp),
                                         _i64p, C.POINTER(Stats)]
         L.gg_graph_build.argtypes = [_i32p, _i32p, _i32p, C.c_int64, C.c_int, C.c_int, C.POINTER(vp),
@@ -506,6 +507,20 @@ def build_graph(u, v, w=None, symmetrize: bool = True, weighted: bool = False,
     dg = DeviceGraph._adopt(h, n.value)
     dg.weighted = bool(weighted)
     dg.directed = not symmetrize
+    return dg, st.as_dict()
+
+
+def read_graph(path: str, device: int = 0) -> tuple[DeviceGraph, dict]:
+    """gg_graph_read: a .sg / .wsg file streamed straight into HBM (weighted
+    when the suffix is .wsg).  Returns (handle, stats)."""
+    init(device)
+    weighted = path.endswith(".wsg")
+    h = C.c_void_p()
+    n = C.c_int64()
+    st = Stats()
+    _check(lib().gg_graph_read(path.encode(), int(weighted), C.byref(h), C.byref(n), C.byref(st)))
+    dg = DeviceGraph._adopt(h, n.value)
+    dg.weighted = weighted
     return dg, st.as_dict()
 
 

@@ -199,6 +199,14 @@ int gg_graph_build(const int32_t* u, const int32_t* v, const int32_t* w, int64_t
  * stats->device_ms covers generation and build. */
 int gg_graph_generate(int kind, int scale, int degree, int64_t seed, int weighted, const int32_t* perm,
                       gg_graph** out, int64_t* num_nodes, gg_stats* stats);
+/* Reads a .sg (weighted = 0) or .wsg (weighted = 1) file straight into a
+ * new handle, streaming it through two pinned staging buffers so that file
+ * reads overlap the copies to HBM; the host never holds the graph.
+ * Replaces Reader::ReadSerializedGraph (src/reader.h:254-307) followed by
+ * gg_graph_upload.  The suffix rules and messages are the reference's
+ * (reader.h:254-266); an unreadable, short or inconsistent file is
+ * GG_EINVAL.  *num_nodes (nullable) receives n. */
+int gg_graph_read(const char* path, int weighted, gg_graph** out, int64_t* num_nodes, gg_stats* stats);
 /* Copies a handle's inverse CSR (in-lists; the out-lists for undirected
  * graphs) back to host memory, like gg_graph_copy_out. */
 int gg_graph_copy_in(const gg_graph* g, int64_t* offsets, int32_t* dests);
