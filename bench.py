@@ -61,7 +61,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build", "generate")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build", "generate", "load")
 
 
 def log(msg):
@@ -97,6 +97,33 @@ def build_report(g, edges, last, per, host_build_s, no_cpu):
                       "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                       "sample": "the whole edge list through the host builder (OpenMP restatement of "
                                 "BuilderBase::MakeGraphFromEL + SquishCSR, src/builder.h:300-341)"}
+    return out
+
+
+def load_report(g, path, per, no_cpu):
+    """The .sg load line: file bytes per second into HBM (page-cached file,
+    read through pinned staging), a byte-for-byte check against the host
+    graph, and the host path (gh_read, then gg_graph_upload) as the CPU
+    baseline."""
+    import numpy as np
+    from gapbs_amd import gpu
+    from gapbs_amd.graph import Graph
+    size = os.path.getsize(path)
+    db, _ = gpu.read_graph(path)
+    off, dst = db.copy_out()
+    db.free()
+    same = bool(np.array_equal(off, g.out_offsets) and np.array_equal(dst, g.out_neighbors))
+    assert same, "device load differs from the host graph"
+    out = {"file_bytes": size, "gb_per_s": round(size / per / 1e9, 2), "matches_host_graph": same}
+    if not no_cpu:
+        t0 = time.time()
+        hg = Graph.read(path)
+        t1 = time.time()
+        gpu.DeviceGraph(hg).free()
+        t2 = time.time()
+        out["cpu"] = {"value": round(t2 - t0, 3), "unit": "s", "kind": "port", "read_s": round(t1 - t0, 3),
+                      "upload_s": round(t2 - t1, 3), "cores": 1,
+                      "sample": "host reader (gh_read, Reader::ReadSerializedGraph restated) then gg_graph_upload"}
     return out
 
 
@@ -383,6 +410,10 @@ def main():
             db, st = gpu.generate_graph(0, args.scale, 16, args.seed)
             db.free()
             return st
+        elif k == "load":
+            db, st = gpu.read_graph(sg_path[0])
+            db.free()
+            return st
         return None
 
     # Per-graph tuning (gg_autotune) before any timing; its own time is
@@ -408,6 +439,17 @@ def main():
             alpha, beta = tuned.get("bfs_alpha", alpha), tuned.get("bfs_beta", beta)
             batch_alpha, delta = tuned.get("batch_alpha", batch_alpha), tuned.get("sssp_delta", delta)
             log(f"autotune: {tuned}")
+
+    # The graph as a .sg file for the load line (written once, untimed; the
+    # reads come from the page cache).
+    sg_path = [None]
+    if "load" in kernels:
+        import tempfile
+        sg_dir = tempfile.mkdtemp(dir=os.environ.get("TMPDIR", "/tmp"))
+        sg_path[0] = os.path.join(sg_dir, "graph.sg")
+        t0 = time.time()
+        g.write(sg_path[0])
+        log(f"wrote {os.path.getsize(sg_path[0]) / 1e9:.2f} GB .sg in {time.time() - t0:.1f}s")
 
     traffic = load_traffic(workload)
     results = {}
@@ -446,6 +488,10 @@ def main():
             build_edges = None
         if k == "generate" and last is not None:
             r.update(generate_report(g, args, last, host_gen_s, host_build_s))
+        if k == "load":
+            r.update(load_report(g, sg_path[0], per, args.no_cpu))
+            os.remove(sg_path[0])
+            os.rmdir(os.path.dirname(sg_path[0]))
         results[k] = r
         log(f"{k}: {r}")
     if "bfs_batch" in results and "bfs" in results:

@@ -8,25 +8,33 @@
 //   and, for directed graphs, the inverse offsets and dests in the same form.
 // The file streams through two pinned staging buffers (hipHostMalloc):
 // while the copy engine moves one buffer to its place in HBM
-// (hipMemcpyAsync on the handle's stream), the host reads the next chunk of
-// the file into the other; an event per buffer says when it may be refilled.
-// The host never holds more than the two buffers.  The lists are then
+// (hipMemcpyAsync on the handle's stream), kReaders host threads read the
+// next chunk of the file into the other with pread, a quarter each; an
+// event per buffer says when it may be refilled.  The host never holds more
+// than the two buffers.  The lists are then
 // checked on the device as an upload's are (offsets from 0 to entries and
 // non-decreasing, destinations in range).
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <cstdio>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "gg_common.h"
 
 namespace gg {
 namespace {
 
-constexpr size_t kStage = size_t(64) << 20;
+constexpr size_t kStage = size_t(256) << 20;
+constexpr int kReaders = 4;
 
 struct Staging {
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
-  FILE* f = nullptr;
+  int fd = -1;
+  int64_t pos = 0;  // file offset of the next byte to stream
   ~Staging() {
     for (int i = 0; i < 2; i++) {
       if (ev[i]) {
@@ -35,20 +43,45 @@ struct Staging {
       }
       if (buf[i]) (void)hipHostFree(buf[i]);
     }
-    if (f) std::fclose(f);
+    if (fd >= 0) close(fd);
   }
 };
 
-// bytes from the file into device memory dev, chunk by chunk.
+// Reads exactly `bytes` at file offset `at` into p (false on a short read).
+bool ReadAt(int fd, char* p, size_t bytes, int64_t at) {
+  while (bytes) {
+    const ssize_t r = pread(fd, p, bytes, static_cast<off_t>(at));
+    if (r <= 0) return false;
+    p += r;
+    bytes -= static_cast<size_t>(r);
+    at += r;
+  }
+  return true;
+}
+
+// The next `bytes` of the file into device memory dev, chunk by chunk.
 void StreamInto(Staging& s, hipStream_t st, void* dev, size_t bytes, int* next, const std::string& path) {
   size_t done = 0;
   while (done < bytes) {
     const int b = *next;
     const size_t chunk = bytes - done < kStage ? bytes - done : kStage;
     GG_HIP(hipEventSynchronize(s.ev[b]));  // its previous copy has finished
-    GG_REQUIRE(std::fread(s.buf[b], 1, chunk, s.f) == chunk, GG_EINVAL, "Short read from " + path);
+    const size_t part = (chunk + kReaders - 1) / kReaders;
+    bool ok[kReaders];
+    std::vector<std::thread> readers;
+    for (int t = 0; t < kReaders; t++) {
+      const size_t lo = t * part, len = lo < chunk ? (chunk - lo < part ? chunk - lo : part) : 0;
+      ok[t] = true;
+      if (len)
+        readers.emplace_back([&, t, lo, len] {
+          ok[t] = ReadAt(s.fd, static_cast<char*>(s.buf[b]) + lo, len, s.pos + static_cast<int64_t>(lo));
+        });
+    }
+    for (auto& r : readers) r.join();
+    for (int t = 0; t < kReaders; t++) GG_REQUIRE(ok[t], GG_EINVAL, "Short read from " + path);
     GG_HIP(hipMemcpyAsync(static_cast<char*>(dev) + done, s.buf[b], chunk, hipMemcpyHostToDevice, st));
     GG_HIP(hipEventRecord(s.ev[b], st));
+    s.pos += static_cast<int64_t>(chunk);
     done += chunk;
     *next = b ^ 1;
   }
@@ -75,13 +108,15 @@ extern "C" int gg_graph_read(const char* path, int weighted, gg_graph** out, int
     GG_REQUIRE(weighted || suffix != ".wsg", GG_EINVAL, ".wsg only allowed for weighted graphs");
     GG_REQUIRE(!weighted || suffix != ".sg", GG_EINVAL, ".sg not allowed for weighted graphs");
     Staging s;
-    s.f = std::fopen(path, "rb");
-    GG_REQUIRE(s.f != nullptr, GG_EINVAL, "Couldn't open file " + p);
+    s.fd = open(path, O_RDONLY);
+    GG_REQUIRE(s.fd >= 0, GG_EINVAL, "Couldn't open file " + p);
     uint8_t directed = 0;
     int64_t entries = 0, n = 0;
-    GG_REQUIRE(std::fread(&directed, 1, 1, s.f) == 1 && std::fread(&entries, 8, 1, s.f) == 1 &&
-                   std::fread(&n, 8, 1, s.f) == 1,
+    GG_REQUIRE(ReadAt(s.fd, reinterpret_cast<char*>(&directed), 1, 0) &&
+                   ReadAt(s.fd, reinterpret_cast<char*>(&entries), 8, 1) &&
+                   ReadAt(s.fd, reinterpret_cast<char*>(&n), 8, 9),
                GG_EINVAL, "Short read from " + p);
+    s.pos = 17;
     GG_REQUIRE(n >= 0 && n < (int64_t(1) << 31) && entries >= 0, GG_EINVAL, "Corrupt header in " + p);
     const int width = weighted ? 2 : 1;
     const size_t off_bytes = sizeof(int64_t) * (n + 1), dst_bytes = sizeof(int32_t) * width * entries;
