@@ -61,7 +61,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "bc", "tc", "build", "generate", "load")
+KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "cc_sv", "bc", "tc", "build", "generate",
+           "load")
 
 
 def log(msg):
@@ -384,6 +385,8 @@ def main():
             return gpu.pagerank(dg, 20, 1e-4, fp64=True, want_scores=False).stats
         elif k == "cc":
             gpu.cc(dg, want_comp=False)
+        elif k == "cc_sv":
+            return gpu.cc_sv(dg, want_comp=False).stats
         elif k == "bc":
             return gpu.bc(dg, bc_roots, want_scores=False).stats
         elif k == "bc_batch":
@@ -477,7 +480,7 @@ def main():
             r["gteps"] = round(sum(m_per_root[:len(bc_roots)]) / per / 1e9, 3)
         if k == "bc_batch":
             r["gteps"] = round(m_total / per / 1e9, 3)
-        if k in ("pr", "pr_fp64") and last:
+        if k in ("pr", "pr_fp64", "cc_sv") and last:
             r["iterations"] = last["iterations"]
         if k == "tc" and last is not None:
             r["probes"] = last["edges_examined"]

@@ -1,4 +1,5 @@
-// Afforest connected components on gfx950.
+// Afforest connected components on gfx950, and Shiloach-Vishkin
+// (gg_cc_sv, src/cc_sv.cc:49-83) as a second variant.
 //
 // Reference: Afforest, Link, Compress, SampleFrequentElement
 // (src/cc.cc:41-151).  Kept:
@@ -119,6 +120,62 @@ __global__ __launch_bounds__(kBS) void CcFinalLinkChunks(const int32_t* __restri
   ExpandChunks(chunks, *nchunks, visit);
 }
 
+// Shiloach-Vishkin's hook (src/cc_sv.cc:58-71): over edge (u, v), the higher
+// of the two labels, while it is still a root, takes the lower one -- the
+// min-max swap that makes lower labels win whatever the edge's direction.
+// A plain store, as the reference's; `hooked` records that this thread
+// changed a label.
+struct HookVisit {
+  const int32_t* dst;
+  int32_t* comp;
+  bool* hooked;
+  __device__ __forceinline__ void operator()(int32_t u, int32_t, int64_t e, bool valid) {
+    if (!valid) return;
+    const int32_t cu = Load(&comp[u]), cv = Load(&comp[dst[e]]);
+    if (cu == cv) return;
+    const int32_t high = cu > cv ? cu : cv;
+    const int32_t low = cu + (cv - high);
+    if (Load(&comp[high]) == high) {
+      comp[high] = low;
+      *hooked = true;
+    }
+  }
+};
+
+// One wavefront store per wavefront that hooked.
+__device__ __forceinline__ void NoteChange(bool hooked, int* change) {
+  if (__ballot(hooked) && LaneId() == 0) *change = 1;
+}
+
+__global__ __launch_bounds__(kBS) void SvHook(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
+                                              int32_t* comp, int64_t n, EdgeChunk* chunks, int32_t* nchunks,
+                                              int* change) {
+  __shared__ ExpandShared sh;
+  bool hooked = false;
+  HookVisit visit{dst, comp, &hooked};
+  for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < n; tile += int64_t(gridDim.x) * kBS) {
+    const int64_t u = tile + threadIdx.x;
+    const int64_t beg = u < n ? off[u] : 0, end = u < n ? off[u + 1] : 0;
+    ExpandTile(sh, static_cast<int32_t>(u), 0, beg, end, visit, chunks, nchunks, kChunkEdges);
+  }
+  NoteChange(hooked, change);
+}
+
+__global__ __launch_bounds__(kBS) void SvHookChunks(const int32_t* __restrict__ dst, int32_t* comp,
+                                                    const EdgeChunk* chunks, const int32_t* nchunks, int* change) {
+  bool hooked = false;
+  HookVisit visit{dst, comp, &hooked};
+  ExpandChunks(chunks, *nchunks, visit);
+  NoteChange(hooked, change);
+}
+
+__global__ void SvClear(int* change, int32_t* nchunks) {
+  if (threadIdx.x == 0) {
+    *change = 0;
+    *nchunks = 0;
+  }
+}
+
 // The reference's sample positions: default-seeded mt19937 and
 // uniform_int_distribution<int32_t>(0, n - 1), 1024 draws (cc.cc:75-80).
 std::vector<int32_t> SamplePositions(int64_t n) {
@@ -220,6 +277,58 @@ extern "C" int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_sta
       stats->total_ms = wt.ms();
       stats->iterations = neighbor_rounds + 1;
       stats->edges_examined = static_cast<int64_t>(ex);
+      stats->launches = launches;
+    }
+  });
+}
+
+extern "C" int gg_cc_sv(gg_graph* g, int32_t* comp_out, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "CC needs an unweighted graph upload");
+    GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
+    GG_HIP(hipSetDevice(g->device));
+    const int64_t n = g->n;
+    hipStream_t st = g->stream;
+    const dim3 blk(kBS), grid_n(GridFor(n, kBS, g->cu_count * 32));
+    const double N = static_cast<double>(n);
+    int32_t* comp = g->Scratch<int32_t>("cc.comp", n);
+    int32_t* nchunks = g->Scratch<int32_t>("cc.nchunks", 2);
+    int* change = g->Scratch<int>("sv.change", 1);
+    const bool use_chunks = g->max_degree >= kChunkEdges;
+    EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("cc.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+    int* h_change = g->Pinned<int>(1);
+    int64_t iterations = 0, launches = 0;
+    BeginTiming(g);
+    Run(g, "cc_init", 4 * N, CcInit, grid_n, blk, comp, n);
+    launches++;
+    // cc_sv.cc:55-80: hook over every out-edge, compress, until no hook.
+    for (;;) {
+      iterations++;
+      Run(g, "sv_clear", 0, SvClear, dim3(1), dim3(64), change, nchunks);
+      Run(g, "sv_hook", 16 * N + 4.0 * g->out_entries, SvHook, dim3(GridFor(n, kBS, g->cu_count * 8)), blk,
+          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), comp, n,
+          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks, change);
+      if (use_chunks)
+        Run(g, "sv_hook_chunks", 0, SvHookChunks, dim3(g->cu_count * 4), blk,
+            static_cast<const int32_t*>(g->out_dst), comp, static_cast<const EdgeChunk*>(chunks),
+            static_cast<const int32_t*>(nchunks), change);
+      Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, comp, n);
+      launches += use_chunks ? 4 : 3;
+      GG_HIP(hipMemcpyAsync(h_change, change, sizeof(int), hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      if (*h_change == 0) break;
+      GG_REQUIRE(iterations < n + 2, GG_EHIP, "Shiloach-Vishkin did not converge");
+    }
+    GG_HIP(hipGetLastError());
+    FinishTiming(g, stats, wt);
+    if (comp_out) CopyOutComponents(g, comp, comp_out);
+    if (stats) {
+      stats->total_ms = wt.ms();
+      stats->iterations = iterations;
+      stats->edges_examined = iterations * g->out_entries;
       stats->launches = launches;
     }
   });

@@ -76,7 +76,7 @@ EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
-    "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate", "gg_graph_read",
+    "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate", "gg_graph_read", "gg_cc_sv",
     "gg_autotune", "gg_tuning_get", "gg_tuning_set",
     "gg_profile_enable", "gg_profile_reset", "gg_profile_get", "gg_last_error",
 ]
@@ -107,6 +107,7 @@ def lib() -> C.CDLL:
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
+        L.gg_cc_sv.argtypes = [vp, _i32p, C.POINTER(Stats)]
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_bc_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_tc.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(Stats)]
@@ -388,6 +389,19 @@ def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True) -> CCR
     _check(lib().gg_cc(dg.handle, neighbor_rounds, _p(comp, C.c_int32) if want_comp else None,
                        C.byref(st)))
     return CCResult(comp, st.as_dict())
+
+
+def cc_sv(dg: DeviceGraph, want_comp: bool = True) -> CCResult:
+    """gg_cc_sv: Shiloach-Vishkin labels (smallest vertex id per component)."""
+    comp = np.empty(dg.num_nodes, dtype=np.int32) if want_comp else None
+    st = Stats()
+    _check(lib().gg_cc_sv(dg.handle, _p(comp, C.c_int32) if want_comp else None, C.byref(st)))
+    return CCResult(comp, st.as_dict())
+
+
+def ShiloachVishkin(dg: DeviceGraph) -> np.ndarray:
+    """Component labels (src/cc_sv.cc:49-83)."""
+    return cc_sv(dg).comp
 
 
 def Afforest(dg: DeviceGraph, neighbor_rounds: int = 2) -> np.ndarray:

@@ -131,6 +131,14 @@ int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* sco
  * vertex id in the component, exactly the reference's labels. */
 int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
 
+/* Shiloach-Vishkin connected components, the reference's second CC binary.
+ * Replaces ShiloachVishkin(const Graph&) (src/cc_sv.cc:49-83): hook every
+ * out-edge's higher root under the lower label (min-max swap, so directed
+ * graphs get their weak components), compress, repeat until no hook.  The
+ * labels equal gg_cc's (the smallest vertex id of each component);
+ * stats->iterations is the reference's iteration count. */
+int gg_cc_sv(gg_graph* g, int32_t* comp, gg_stats* stats);
+
 /* Brandes betweenness centrality from roots[0 .. k).
  * Replaces Brandes(const Graph&, SourcePicker<Graph>&, NodeID num_iters,
  * bool logging) (src/bc.cc:95-150): the caller passes the roots the

@@ -11,7 +11,7 @@ import pytest
 from conftest import ROOT
 
 BIN = os.path.join(ROOT, "gapbs_amd", "bin")
-KERNELS = ["bc", "bfs", "cc", "pr", "sssp", "tc"]
+KERNELS = ["bc", "bfs", "cc", "cc_sv", "pr", "sssp", "tc"]
 
 
 def run(args, timeout=300):
@@ -60,7 +60,7 @@ def test_verify_g10(kernel):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kernel,extra", [("bfs", ["-n4"]), ("sssp", ["-n4", "-d2"]),
-                                          ("pr", ["-n2", "-p", "fp64"]), ("cc", ["-n2"]),
+                                          ("pr", ["-n2", "-p", "fp64"]), ("cc", ["-n2"]), ("cc_sv", ["-n2"]),
                                           ("bc", ["-n2", "-i4"]), ("bc", ["-n1", "-i4", "-p", "kahan"]),
                                           ("tc", ["-n1"])])
 def test_verify_g14(kernel, extra):

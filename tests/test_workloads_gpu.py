@@ -9,7 +9,8 @@ oracle.
   * SSSP (weights uniform in [1, 255]): distances equal Dijkstra
     (src/sssp.cc:156-181) at three bucket widths.
   * PageRank: FP32 within 1e-6 of the FP64 oracle (src/pr_spmv.cc:35-61).
-  * CC: labels equal the smallest id per component.
+  * CC: Afforest and Shiloach-Vishkin labels equal the smallest id per
+    component.
   * BC from 4 roots: bit-exact with the float Brandes (src/bc.cc:169-220)
     on the relabelled graph, and FP64 within 1e-7 of the FP64 oracle.
   * TC: equals OrderedCount (src/tc.cc:52-71) on the relabelled graph.
@@ -97,8 +98,10 @@ class TestKronecker22:
         assert g.verify_pr(r32.scores, 1e-4)
 
     def test_cc(self, k22):
-        say("[k22] cc")
-        assert np.array_equal(gpu.cc(k22["dg"]).comp, oracle.cc_min_labels(k22["g"]))
+        say("[k22] cc (Afforest and Shiloach-Vishkin)")
+        want = oracle.cc_min_labels(k22["g"])
+        assert np.array_equal(gpu.cc(k22["dg"]).comp, want)
+        assert np.array_equal(gpu.cc_sv(k22["dg"]).comp, want)
 
     def test_bc(self, k22):
         g, dg, roots = k22["g"], k22["dg"], k22["roots"]
