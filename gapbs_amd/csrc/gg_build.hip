@@ -21,7 +21,8 @@
 // Every step is a streaming pass over HBM.  The temporaries -- the edge list
 // in HBM (12 bytes per edge with weights) and, per list entry before the
 // squish (2 per edge when symmetrised), two 8-byte keys and a 4-byte flag --
-// are freed before the call returns.
+// are freed before the call returns; the lists keep room for every
+// unsquished entry (the squish only drops duplicates and self-loops).
 #include "gg_common.h"
 #include "gg_sort.h"
 
@@ -178,43 +179,46 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Temporaries of one side's build, allocated before the timed region.
+struct SideBufs {
+  DevBuf keys, tmp, flags, total;
+  explicit SideBufs(int64_t cnt)
+      : keys(sizeof(uint64_t) * (cnt > 0 ? cnt : 1)),
+        tmp(sizeof(uint64_t) * (cnt > 0 ? cnt : 1)),
+        flags(sizeof(uint32_t) * (cnt > 0 ? cnt : 1)),
+        total(sizeof(uint32_t)) {}
+};
+
 // One side of the CSR (out-lists, in-lists, or both directions of a
-// symmetrised graph) from the edge list on the device: fills *off (n + 1,
-// allocated by the caller) and returns the entries and their array.
+// symmetrised graph) from the edge list on the device: fills off (n + 1)
+// and dst (room for every unsquished entry) and returns the entries.
 int64_t BuildSide(gg_graph* g, const int32_t* du, const int32_t* dv, const int32_t* dw, int64_t m, int which,
-                  KeyShape s, int width, int64_t* off, int32_t** dst) {
+                  KeyShape s, int width, SideBufs& b, int64_t* off, int32_t* dst) {
   const int64_t n = g->n;
   const int64_t cnt = which == 2 ? 2 * m : m;
   const dim3 blk(kBS);
-  DevBuf keys(sizeof(uint64_t) * (cnt > 0 ? cnt : 1)), tmp(sizeof(uint64_t) * (cnt > 0 ? cnt : 1));
-  DevBuf flags(sizeof(uint32_t) * (cnt > 0 ? cnt : 1)), total(sizeof(uint32_t));
   const dim3 grid_m(GridFor(m, kBS, g->cu_count * 16)), grid_c(GridFor(cnt, kBS, g->cu_count * 16));
   if (m > 0)
     Run(g, "build_keys", (which == 2 ? 20.0 : 12.0) * m, BdKeys, grid_m, blk, du, dv, dw, m, s, which,
-        keys.as<uint64_t>());
-  uint64_t* sorted = RadixSort(g, "bd.sort", keys.as<uint64_t>(), tmp.as<uint64_t>(), cnt, 0,
+        b.keys.as<uint64_t>());
+  uint64_t* sorted = RadixSort(g, "bd.sort", b.keys.as<uint64_t>(), b.tmp.as<uint64_t>(), cnt, 0,
                                2 * s.id_bits + s.w_bits);
   uint32_t entries32 = 0;
-  GG_HIP(hipMemsetAsync(total.p, 0, sizeof(uint32_t), g->stream));
-  if (cnt > 0) {
-    Run(g, "build_mark", 12.0 * cnt, BdMark, grid_c, blk, static_cast<const uint64_t*>(sorted), cnt, s,
-        flags.as<uint32_t>());
-    DeviceScan<uint32_t>(g, "bd.flags", flags.as<uint32_t>(), cnt, total.as<uint32_t>());
-  }
-  GG_HIP(hipMemcpyAsync(&entries32, total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
-  GG_HIP(hipStreamSynchronize(g->stream));
-  const int64_t entries = entries32;
-  *dst = static_cast<int32_t*>(DeviceAlloc(sizeof(int32_t) * width * entries));
+  GG_HIP(hipMemsetAsync(b.total.p, 0, sizeof(uint32_t), g->stream));
   unsigned long long* deg = reinterpret_cast<unsigned long long*>(off);
   Run(g, "build_zero", 8.0 * (n + 1), BdZero, dim3(GridFor(n + 1, kBS, g->cu_count * 4)), blk, deg, n + 1);
-  if (cnt > 0)
-    Run(g, "build_emit", 12.0 * cnt + 4.0 * width * entries, BdEmit, grid_c, blk,
-        static_cast<const uint64_t*>(sorted), static_cast<const uint32_t*>(flags.as<uint32_t>()), cnt, s, width,
-        *dst, deg);
+  if (cnt > 0) {
+    Run(g, "build_mark", 12.0 * cnt, BdMark, grid_c, blk, static_cast<const uint64_t*>(sorted), cnt, s,
+        b.flags.as<uint32_t>());
+    DeviceScan<uint32_t>(g, "bd.flags", b.flags.as<uint32_t>(), cnt, b.total.as<uint32_t>());
+    Run(g, "build_emit", 12.0 * cnt, BdEmit, grid_c, blk, static_cast<const uint64_t*>(sorted),
+        static_cast<const uint32_t*>(b.flags.as<uint32_t>()), cnt, s, width, dst, deg);
+  }
   DeviceScan<int64_t>(g, "bd.offsets", off, n + 1, static_cast<int64_t*>(nullptr));
-  return entries;
+  GG_HIP(hipMemcpyAsync(&entries32, b.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  return entries32;
 }
-
 
 // ---- generation (src/generator.h) --------------------------------------
 //
@@ -407,21 +411,31 @@ gg_graph* BuildFromDevice(int device, const int32_t* du, const int32_t* dv, cons
   GG_REQUIRE(2 * s.id_bits + s.w_bits <= 64, GG_ENOTSUP,
              "vertex ids and weight range too wide to sort as one 64-bit key");
 
+  // Everything is allocated before the timed region: the lists with room
+  // for every unsquished entry (the squish drops only duplicates and
+  // self-loops), the keys, flags and scan space.
+  const int64_t cnt = symmetrize ? 2 * m : m;
   gg_graph* g = NewDeviceGraph(device, n, 0);
   try {
     g->weighted = weighted;
     DeviceFree(g->out_dst, 0);
     g->out_dst = g->in_dst = nullptr;
-    BeginTiming(g);
-    const int32_t* pw = use_w ? dw : nullptr;
-    g->out_entries = BuildSide(g, du, dv, pw, m, symmetrize ? 2 : 0, s, width, g->out_off, &g->out_dst);
-    g->graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * width * g->out_entries;
+    g->out_dst = static_cast<int32_t*>(DeviceAlloc(sizeof(int32_t) * width * cnt));
     if (!symmetrize) {
       // Directed from here on, so that a failure frees the in-lists too.
       g->in_off = static_cast<int64_t*>(DeviceAlloc(sizeof(int64_t) * (n + 1)));
-      g->in_dst = nullptr;
       g->directed = true;
-      g->in_entries = BuildSide(g, du, dv, pw, m, 1, s, width, g->in_off, &g->in_dst);
+      g->in_dst = static_cast<int32_t*>(DeviceAlloc(sizeof(int32_t) * width * cnt));
+    }
+    SideBufs bufs(cnt);
+    const int64_t tiles = (cnt + kRsTile - 1) / kRsTile;
+    (void)g->Scratch<uint32_t>("bd.sort.hist", tiles * 256);
+    BeginTiming(g);
+    const int32_t* pw = use_w ? dw : nullptr;
+    g->out_entries = BuildSide(g, du, dv, pw, m, symmetrize ? 2 : 0, s, width, bufs, g->out_off, g->out_dst);
+    g->graph_bytes = sizeof(int64_t) * (n + 1) + sizeof(int32_t) * width * g->out_entries;
+    if (!symmetrize) {
+      g->in_entries = BuildSide(g, du, dv, pw, m, 1, s, width, bufs, g->in_off, g->in_dst);
       g->graph_bytes += sizeof(int64_t) * (n + 1) + sizeof(int32_t) * width * g->in_entries;
     } else {
       g->in_off = g->out_off;
@@ -439,6 +453,19 @@ gg_graph* BuildFromDevice(int device, const int32_t* du, const int32_t* dv, cons
     GG_HIP(hipMemcpyAsync(&hmx, mx.p, sizeof(hmx), hipMemcpyDeviceToHost, g->stream));
     FinishTiming(g, stats, wt);
     g->max_degree = static_cast<int64_t>(hmx);
+    // Outside the timed region: each list array down to its exact size.
+    auto shrink = [&](int32_t** p, int64_t entries) {
+      if (entries == cnt) return;
+      int32_t* q = static_cast<int32_t*>(DeviceAlloc(sizeof(int32_t) * width * entries));
+      if (entries) GG_HIP(hipMemcpy(q, *p, sizeof(int32_t) * width * entries, hipMemcpyDeviceToDevice));
+      DeviceFree(*p, sizeof(int32_t) * width * cnt);
+      *p = q;
+    };
+    shrink(&g->out_dst, g->out_entries);
+    if (g->directed)
+      shrink(&g->in_dst, g->in_entries);
+    else
+      g->in_dst = g->out_dst;
     g->ws.Release();  // the sort's and scans' workspaces
   } catch (...) {
     gg_graph_free(g);
