@@ -37,6 +37,12 @@ constexpr int kQueueCap = 1024;
 constexpr int32_t kDistInf = INT32_MAX / 2;
 constexpr int kStripes = 64;
 constexpr int64_t kWindow = 8;  // buckets per mid window
+// Grid caps of the per-bucket kernels, in blocks per CU.  Bucket frontiers
+// and piles are often a few hundred thousand entries; many small blocks keep
+// more relaxations and row reads in flight than a few blocks looping over
+// tiles.  At Kronecker 26 (4 roots, delta 8) 64 took the step from 703 to
+// 294 ms (8 and 4: 703 ms; 16: 501 ms; 32: 328 ms; 128: no better).
+constexpr unsigned kBucketBlocksPerCu = 64;
 enum { kRelaxed, kImproved, kNumSums };
 
 struct SsspCounters {
@@ -362,7 +368,7 @@ extern "C" int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist
     auto pile_min = [&](const int32_t* pa, int64_t na, const int32_t* pb, int64_t nb) {
       const int64_t total = na + nb;
       if (total == 0) return static_cast<int64_t>(kDistInf);
-      Run(g, "sssp_pile_min", 8.0 * total, SsspPileMin, dim3(GridFor(total, kBS, g->cu_count * 8)), blk, pa,
+      Run(g, "sssp_pile_min", 8.0 * total, SsspPileMin, dim3(GridFor(total, kBS, g->cu_count * kBucketBlocksPerCu)), blk, pa,
           static_cast<int32_t>(na), pb, static_cast<int32_t>(nb), static_cast<const int32_t*>(dist),
           clamp(bucket_end), dc);
       launches++;
@@ -377,13 +383,13 @@ extern "C" int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist
         const int32_t end32 = clamp(bucket_end);
         const int64_t chunk_min = fsize <= 4096 ? kExpandBS : kChunkEdges;
         const int idx = Run(g, "sssp_relax", 12.0 * fsize, SsspRelax,
-                            dim3(GridFor(fsize, kBS, g->cu_count * 8)), blk, g->out_off, wdst, dist,
+                            dim3(GridFor(fsize, kBS, g->cu_count * kBucketBlocksPerCu)), blk, g->out_off, wdst, dist,
                             stamp, freshstamp, static_cast<const int32_t*>(fq),
                             static_cast<int32_t>(fsize), end32, near_tag, fresh_tag, nq, fresh, dc,
                             use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), chunk_min);
         launches++;
         if (use_chunks) {
-          Run(g, "sssp_relax_chunks", 0, SsspRelaxChunks, dim3(g->cu_count * 4), blk, wdst, dist,
+          Run(g, "sssp_relax_chunks", 0, SsspRelaxChunks, dim3(g->cu_count * kBucketBlocksPerCu), blk, wdst, dist,
               stamp, freshstamp, end32, near_tag, fresh_tag, nq, fresh, dc,
               static_cast<const EdgeChunk*>(chunks));
           launches++;
@@ -418,7 +424,7 @@ extern "C" int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist
       int32_t* fout = in_window ? far + far_size : far2;
       const int64_t total = na + fresh_size;
       Run(g, in_window ? "sssp_mid_split" : "sssp_far_split", 12.0 * total, SsspSplit,
-          dim3(GridFor(total, kBS, g->cu_count * 8)), blk, pa, static_cast<int32_t>(na),
+          dim3(GridFor(total, kBS, g->cu_count * kBucketBlocksPerCu)), blk, pa, static_cast<int32_t>(na),
           static_cast<const int32_t*>(fresh), static_cast<int32_t>(fresh_size), static_cast<const int32_t*>(dist),
           stamp, midstamp, farstamp, clamp(bucket_end), clamp(next_end), clamp(window_end), f_tag, mid_tag, keep_far,
           fq, mid2, fout, dc);
