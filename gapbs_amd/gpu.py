@@ -191,7 +191,8 @@ class DeviceGraph:
 
     @classmethod
     def _adopt(cls, handle: C.c_void_p, num_nodes: int) -> "DeviceGraph":
-        """Wraps a handle the library created (undirected, unweighted)."""
+        """Wraps a handle the library created (undirected and unweighted unless
+        the caller says otherwise)."""
         self = cls.__new__(cls)
         self.host = None
         self.ordered = False
@@ -535,6 +536,8 @@ def read_graph(path: str, device: int = 0) -> tuple[DeviceGraph, dict]:
     _check(lib().gg_graph_read(path.encode(), int(weighted), C.byref(h), C.byref(n), C.byref(st)))
     dg = DeviceGraph._adopt(h, n.value)
     dg.weighted = weighted
+    with open(path, "rb") as f:  # the header's first byte: directed
+        dg.directed = f.read(1) == b"\x01"
     return dg, st.as_dict()
 
 
