@@ -160,3 +160,25 @@ def test_gpu_library_fails_loudly_without_device():
         assert e.value.code in (-19, -5)
     else:
         pytest.skip("a GPU driver is present")
+
+
+@pytest.mark.parametrize("kind,scale,weighted", [(0, 10, False), (0, 10, True), (1, 10, True), (2, 32, True)])
+def test_generated_edge_list_rebuilds_the_generated_graph(kind, scale, weighted):
+    # gh_generate_edges hands out the list gh_generate builds (the device
+    # builder's input): building it on the host gives the same bytes.
+    from gapbs_amd.graph import Graph
+    u, v, w = Graph.generate_edges(kind, scale, 16, 8675309, weighted=weighted)
+    make = {0: Graph.kronecker, 1: Graph.uniform}.get(kind)
+    want = make(scale, 16, seed=8675309, weighted=weighted) if make else Graph.grid(scale, seed=8675309,
+                                                                                    weighted=weighted)
+    got = Graph.from_edges(u, v, w)
+    assert np.array_equal(got.out_offsets, want.out_offsets)
+    assert np.array_equal(got.out_dests_raw, want.out_dests_raw)
+
+
+def test_permutation_is_a_seeded_permutation():
+    from gapbs_amd.graph import Graph
+    p = Graph.permutation(1 << 12, 42)
+    assert np.array_equal(np.sort(p), np.arange(1 << 12))
+    assert np.array_equal(p, Graph.permutation(1 << 12, 42))
+    assert not np.array_equal(p, Graph.permutation(1 << 12, 43))
