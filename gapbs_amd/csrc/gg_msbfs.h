@@ -58,7 +58,7 @@ constexpr int kUnroll = 4;             // gathers in flight per lane
 // Light tier: list positions scanned per round (a vertex leaves once it
 // has every bit it can still gain).
 constexpr int kRounds = 3;
-__constant__ const int kRoundEnd[kRounds] = {16, 64, static_cast<int>(kHeavyEdges)};
+__constant__ const int kRoundEnd[kRounds] = {8, 64, static_cast<int>(kHeavyEdges)};  // {16, 64}: 2% slower
 
 struct MsCounters {
   int32_t active_tail;   // top-down: next level's active list
