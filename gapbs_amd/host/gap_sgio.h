@@ -2,8 +2,8 @@
 //
 // Reference: Reader::ReadSerializedGraph (src/reader.h:254-307) and
 // WriterBase::WriteSerializedGraph (src/writer.h:39-69).  The byte layout is
-// kept identical so that graphs move between this build and the reference
-// binaries (`-f file.sg`):
+// kept identical so that graphs move between these drivers and the
+// reference binaries (`-f file.sg`):
 //   bool directed (1 byte) | int64 entries | int64 nodes |
 //   int64 offsets[nodes+1] | dest[entries]            (dest: int32 or {int32,int32})
 //   and, for directed graphs, the inverse offsets and dests in the same form.

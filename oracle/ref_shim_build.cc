@@ -1,7 +1,7 @@
 // Test infrastructure only: exposes the reference's graph builder
 // (BuilderBase::MakeGraph, src/builder.h:343-364) and SourcePicker
-// (src/benchmark.h:46-68) so tests can compare this build's host generator
-// and root sequence with the reference's on the same flags.
+// (src/benchmark.h:46-68) so tests can compare libgaphost's generator and
+// root sequence with the reference's on the same flags.
 #include "benchmark.h"
 #include "builder.h"
 #include "command_line.h"
