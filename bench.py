@@ -62,7 +62,7 @@ sys.path.insert(0, ROOT)
 METRIC = "BFS/SSSP/BC GTEPS, PR/CC/TC seconds, Kronecker s26 ef16; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("bfs_batch", "bfs", "bc_batch", "sssp", "pr", "pr_fp64", "cc", "cc_sv", "bc", "tc", "build", "generate",
-           "load")
+           "load", "packed")
 
 
 def log(msg):
@@ -99,6 +99,20 @@ def build_report(g, edges, last, per, host_build_s, no_cpu):
                       "sample": "the whole edge list through the host builder (OpenMP restatement of "
                                 "BuilderBase::MakeGraphFromEL + SquishCSR, src/builder.h:300-341)"}
     return out
+
+
+def packed_report(packed, last, per, steps, warmup, run_kernel):
+    """The packed line: traversed edges per second over all 2048 roots, and
+    the same roots graph by graph (one batched call per graph)."""
+    m = last["reached_edges"] // 2
+    for _ in range(warmup):
+        run_kernel("packed_per_graph")
+    t = time.perf_counter()
+    for _ in range(steps):
+        run_kernel("packed_per_graph")
+    per_graph = (time.perf_counter() - t) / steps
+    return {"gteps": round(m / per / 1e9, 3), "graphs": len(packed[3]), "roots": int(packed[2].size),
+            "per_graph_seconds": round(per_graph, 6), "speedup_vs_per_graph": round(per_graph / per, 3)}
 
 
 def load_report(g, path, per, no_cpu):
@@ -417,6 +431,11 @@ def main():
             db, st = gpu.read_graph(sg_path[0])
             db.free()
             return st
+        elif k == "packed":
+            return gpu.bfs_packed(packed[0], packed[1], packed[2], batch_alpha, want_depths=False).stats
+        elif k == "packed_per_graph":
+            for i, (dgi, r) in enumerate(zip(packed[3], packed[2])):
+                gpu.bfs_batch(dgi, r - packed[1][i], batch_alpha, want_depths=False)
         return None
 
     # Per-graph tuning (gg_autotune) before any timing; its own time is
@@ -453,6 +472,21 @@ def main():
         t0 = time.time()
         g.write(sg_path[0])
         log(f"wrote {os.path.getsize(sg_path[0]) / 1e9:.2f} GB .sg in {time.time() - t0:.1f}s")
+
+    # The packed workload: 256 Kronecker graphs of scale 10..16 (edge factor
+    # 16, generator seed = graph index), 8 roots each, in one traversal; the
+    # same roots graph by graph (one batched call per graph) for comparison.
+    packed = None
+    if "packed" in kernels:
+        import numpy as np
+        t0 = time.time()
+        scales = np.random.default_rng(0).integers(10, 17, 256)
+        pgraphs = [Graph.kronecker(int(sc), 16, seed=i) for i, sc in enumerate(scales)]
+        union, bounds = Graph.pack(pgraphs)
+        proots = np.stack([pg.pick_sources(8) + bounds[i] for i, pg in enumerate(pgraphs)])
+        packed = (gpu.DeviceGraph(union), bounds, proots, [gpu.DeviceGraph(pg) for pg in pgraphs])
+        log(f"packed workload: {union.num_nodes} vertices, {union.num_edges()} edges, built in "
+            f"{time.time() - t0:.1f}s")
 
     traffic = load_traffic(workload)
     results = {}
@@ -491,6 +525,8 @@ def main():
             build_edges = None
         if k == "generate" and last is not None:
             r.update(generate_report(g, args, last, host_gen_s, host_build_s))
+        if k == "packed" and last is not None:
+            r.update(packed_report(packed, last, per, args.steps, args.warmup, run_kernel))
         if k == "load":
             r.update(load_report(g, sg_path[0], per, args.no_cpu))
             os.remove(sg_path[0])
