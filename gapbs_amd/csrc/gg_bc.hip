@@ -554,6 +554,9 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   int64_t launches = 0, levels_total = 0, reached_total = 0, examined_total = 0;
   unsigned long long mail[kMailValues];
 
+  // Dependencies are summed in each vertex's uploaded list order (the
+  // reference's successor order), also on a degree-ordered handle.
+  const int32_t* fold_dst = UploadOrderLists(g);
   BeginTiming(g);
   Run(g, "bc_zero", N * sizeof(T) * 2, BcZero<T>, grid_n, blk, score, comp, n);
   launches++;
@@ -653,7 +656,7 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
       const int32_t qb = bounds[d], qn = bounds[d + 1] - bounds[d];
       Run(g, "bc_delta", (16.0 + 2 * sizeof(T) + 8.0) * qn, BcDelta<T, kKahan>,
           dim3(GridFor(qn, kBS, g->cu_count * 16)), blk, static_cast<const int64_t*>(g->out_off),
-          static_cast<const int32_t*>(g->out_dst), static_cast<const int32_t*>(depth),
+          fold_dst, static_cast<const int32_t*>(depth),
           static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(queue), qb, qn, d,
           wave_list, dc);
       launches++;
@@ -667,14 +670,14 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
           GG_HIP(hipEventRecord(g->ev_fork, st));
           GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
           RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count), dim3(kBcHubBS),
-                static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+                static_cast<const int64_t*>(g->out_off), fold_dst,
                 static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
                 static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
           GG_HIP(hipEventRecord(g->ev_join, g->side));
           launches++;
         }
         Run(g, "bc_delta_wave", 0, BcDeltaWave<T, kKahan>, dim3(GridFor(qn, kBS / kWave, g->cu_count * 8)),
-            blk, static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+            blk, static_cast<const int64_t*>(g->out_off), fold_dst,
             static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
             static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
         if (hubs) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));

@@ -20,8 +20,8 @@
 // stale root would never finish; coherent loads keep the reference's
 // progress guarantee.  The final link expands vertices over the grid with
 // gg_expand.h.
+#include <algorithm>
 #include <random>
-#include <unordered_map>
 
 #include "gg_common.h"
 #include "gg_expand.h"
@@ -178,12 +178,104 @@ __global__ void SvClear(int* change, int32_t* nchunks) {
 
 // The reference's sample positions: default-seeded mt19937 and
 // uniform_int_distribution<int32_t>(0, n - 1), 1024 draws (cc.cc:75-80).
-std::vector<int32_t> SamplePositions(int64_t n) {
+void SamplePositions(int64_t n, int32_t* idx) {
   std::mt19937 gen;
   std::uniform_int_distribution<int32_t> dist(0, static_cast<int32_t>(n - 1));
-  std::vector<int32_t> idx(kSamples);
   for (int i = 0; i < kSamples; i++) idx[i] = dist(gen);
-  return idx;
+}
+
+// Clears the examined counters and the chunk count (a kernel, not a memset,
+// so the sequence holds no memset node if a caller captures it).
+__global__ void CcClear(unsigned long long* examined, int32_t* nchunks) {
+  if (threadIdx.x < 64) examined[threadIdx.x] = 0;
+  if (threadIdx.x == 0) nchunks[0] = 0;
+}
+
+__global__ void CcClearChunks(int32_t* nchunks) {
+  if (threadIdx.x == 0) nchunks[0] = 0;
+}
+
+// The most frequent label among the samples; ties go to the smallest label
+// (the choice only decides which component skips the final link, never the
+// labels).  Sorts the samples in place.
+int32_t MostFrequent(int32_t* s, int k) {
+  std::sort(s, s + k);
+  int32_t best_label = s[0];
+  int best = 0;
+  for (int i = 0; i < k;) {
+    int j = i;
+    while (j < k && s[j] == s[i]) j++;
+    if (j - i > best) {
+      best = j - i;
+      best_label = s[i];
+    }
+    i = j;
+  }
+  return best_label;
+}
+
+// Device buffers of one Afforest run.
+struct CcBuffers {
+  int32_t* comp;                 // n
+  int32_t* sample_idx;           // kSamples, filled by the caller
+  int32_t* sample_val;           // kSamples
+  unsigned long long* examined;  // 64 striped counters
+  int32_t* nchunks;              // 1
+  EdgeChunk* chunks;             // MaxChunks(n, entries), or null when no vertex reaches kChunkEdges
+};
+
+// Afforest (cc.cc:95-151) on g->stream into b.comp, inside the caller's
+// timed region.  h_samples (host, kSamples) receives the sampled labels.
+// Returns the kernel launches issued.
+int64_t RunAfforest(gg_graph* g, int neighbor_rounds, const CcBuffers& b, int32_t* h_samples) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  const dim3 blk(kBS);
+  const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
+  const double N = static_cast<double>(n);
+  int64_t launches = 0;
+  hipLaunchKernelGGL(CcClear, dim3(1), dim3(64), 0, st, b.examined, b.nchunks);
+  Run(g, "cc_init", 4 * N, CcInit, grid_n, blk, b.comp, n);
+  launches += 2;
+  for (int r = 0; r < neighbor_rounds; r++) {
+    // offsets 16 B (both ends), the r-th neighbour, comp read/write.
+    Run(g, "cc_link_round", 16 * N + 4 * N + 8 * N, CcLinkRound, grid_n, blk,
+        static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), b.comp, n, r);
+    Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, b.comp, n);
+    launches += 2;
+  }
+  CcGather<<<dim3(kSamples / kBS), blk, 0, st>>>(b.comp, b.sample_idx, b.sample_val, kSamples);
+  GG_HIP(hipMemcpyAsync(h_samples, b.sample_val, sizeof(int32_t) * kSamples, hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  launches++;
+  const int32_t skip = MostFrequent(h_samples, kSamples);
+  auto final_link = [&](const int64_t* off, const int32_t* dst, int start) {
+    hipLaunchKernelGGL(CcClearChunks, dim3(1), dim3(64), 0, st, b.nchunks);
+    Run(g, "cc_final_link", 8 * N + 4.0 * (off == g->out_off ? g->out_entries : g->in_entries), CcFinalLink,
+        dim3(GridFor(n, kBS, g->cu_count * 8)), blk, off, dst, b.comp, n, skip, start, b.chunks, b.nchunks,
+        b.examined);
+    launches += 2;
+    if (b.chunks) {
+      Run(g, "cc_final_link_chunks", 0, CcFinalLinkChunks, dim3(g->cu_count * 4), blk, dst, b.comp,
+          static_cast<const EdgeChunk*>(b.chunks), static_cast<const int32_t*>(b.nchunks));
+      launches++;
+    }
+  };
+  final_link(g->out_off, g->out_dst, neighbor_rounds);
+  if (g->directed) final_link(g->in_off, g->in_dst, 0);
+  Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, b.comp, n);
+  launches++;
+  GG_HIP(hipGetLastError());
+  return launches;
+}
+
+int64_t CcEntries(const gg_graph* g) { return g->out_entries > g->in_entries ? g->out_entries : g->in_entries; }
+
+void CcCheck(const gg_graph* g, int neighbor_rounds) {
+  GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
+  GG_REQUIRE(!g->weighted, GG_ENOTSUP, "CC needs an unweighted graph upload");
+  GG_REQUIRE(neighbor_rounds >= 0, GG_EINVAL, "neighbor_rounds must be >= 0");
+  GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
 }
 
 }  // namespace
@@ -195,83 +287,31 @@ extern "C" int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_sta
   return Guard([&] {
     WallTimer wt;
     StatsReset(stats);
-    GG_REQUIRE(g != nullptr, GG_EINVAL, "null graph");
-    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "CC needs an unweighted graph upload");
-    GG_REQUIRE(neighbor_rounds >= 0, GG_EINVAL, "neighbor_rounds must be >= 0");
-    GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
+    CcCheck(g, neighbor_rounds);
     GG_HIP(hipSetDevice(g->device));
     const int64_t n = g->n;
-    hipStream_t st = g->stream;
-    const dim3 blk(kBS);
-    const dim3 grid_n(GridFor(n, kBS, g->cu_count * 32));
-    const double N = static_cast<double>(n);
-
-    int32_t* comp = g->Scratch<int32_t>("cc.comp", n);
-    int32_t* sample_idx = g->Scratch<int32_t>("cc.sample_idx", kSamples);
-    int32_t* sample_val = g->Scratch<int32_t>("cc.sample_val", kSamples);
-    unsigned long long* examined = g->Scratch<unsigned long long>("cc.examined", 64);
-    int32_t* nchunks = g->Scratch<int32_t>("cc.nchunks", 2);
-    const bool use_chunks = g->max_degree >= kChunkEdges;
-    EdgeChunk* chunks =
-        use_chunks ? g->Scratch<EdgeChunk>("cc.chunks", MaxChunks(n, g->out_entries > g->in_entries
-                                                                         ? g->out_entries
-                                                                         : g->in_entries))
-                   : nullptr;
+    CcBuffers b;
+    b.comp = g->Scratch<int32_t>("cc.comp", n);
+    b.sample_idx = g->Scratch<int32_t>("cc.sample_idx", kSamples);
+    b.sample_val = g->Scratch<int32_t>("cc.sample_val", kSamples);
+    b.examined = g->Scratch<unsigned long long>("cc.examined", 64);
+    b.nchunks = g->Scratch<int32_t>("cc.nchunks", 2);
+    b.chunks = g->max_degree >= kChunkEdges ? g->Scratch<EdgeChunk>("cc.chunks", MaxChunks(n, CcEntries(g)))
+                                            : nullptr;
     int32_t* h_samples = g->Pinned<int32_t>(kSamples);
     if (g->cc_samples_n != n) {  // sample positions depend only on n
-      const std::vector<int32_t> idx = SamplePositions(n);
-      GG_HIP(hipMemcpy(sample_idx, idx.data(), sizeof(int32_t) * kSamples, hipMemcpyHostToDevice));
+      int32_t idx[kSamples];
+      SamplePositions(n, idx);
+      GG_HIP(hipMemcpy(b.sample_idx, idx, sizeof(int32_t) * kSamples, hipMemcpyHostToDevice));
       g->cc_samples_n = n;
     }
-    int64_t launches = 0;
-
     BeginTiming(g);
-    GG_HIP(hipMemsetAsync(examined, 0, 64 * sizeof(unsigned long long), st));
-    Run(g, "cc_init", 4 * N, CcInit, grid_n, blk, comp, n);
-    launches++;
-    for (int r = 0; r < neighbor_rounds; r++) {
-      // offsets 16 B (both ends), the r-th neighbour, comp read/write.
-      Run(g, "cc_link_round", 16 * N + 4 * N + 8 * N, CcLinkRound, grid_n, blk,
-          static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst), comp, n, r);
-      Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, comp, n);
-      launches += 2;
-    }
-    // Most frequent label among the samples (ties: smallest label; the
-    // choice only decides which component skips the final link).
-    CcGather<<<dim3(kSamples / kBS), blk, 0, st>>>(comp, sample_idx, sample_val, kSamples);
-    GG_HIP(hipMemcpyAsync(h_samples, sample_val, sizeof(int32_t) * kSamples, hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-    std::unordered_map<int32_t, int> counts(64);
-    for (int i = 0; i < kSamples; i++) counts[h_samples[i]]++;
-    int32_t skip = 0;
-    int best = -1;
-    for (const auto& kv : counts)
-      if (kv.second > best || (kv.second == best && kv.first < skip)) {
-        best = kv.second;
-        skip = kv.first;
-      }
-    auto final_link = [&](const int64_t* off, const int32_t* dst, int start) {
-      GG_HIP(hipMemsetAsync(nchunks, 0, sizeof(int32_t), st));
-      Run(g, "cc_final_link", 8 * N + 4.0 * (off == g->out_off ? g->out_entries : g->in_entries),
-          CcFinalLink, dim3(GridFor(n, kBS, g->cu_count * 8)), blk, off, dst, comp, n, skip, start,
-          use_chunks ? chunks : static_cast<EdgeChunk*>(nullptr), nchunks, examined);
-      launches++;
-      if (use_chunks) {
-        Run(g, "cc_final_link_chunks", 0, CcFinalLinkChunks, dim3(g->cu_count * 4), blk, dst, comp,
-            static_cast<const EdgeChunk*>(chunks), static_cast<const int32_t*>(nchunks));
-        launches++;
-      }
-    };
-    final_link(g->out_off, g->out_dst, neighbor_rounds);
-    if (g->directed) final_link(g->in_off, g->in_dst, 0);
-    Run(g, "cc_compress", 8 * N, CcCompress, grid_n, blk, comp, n);
-    launches++;
-    GG_HIP(hipGetLastError());
+    const int64_t launches = RunAfforest(g, neighbor_rounds, b, h_samples);
     FinishTiming(g, stats, wt);
-    if (comp_out) CopyOutComponents(g, comp, comp_out);
+    if (comp_out) CopyOutComponents(g, b.comp, comp_out);
     if (stats) {
       unsigned long long h[64];
-      GG_HIP(hipMemcpy(h, examined, sizeof(h), hipMemcpyDeviceToHost));
+      GG_HIP(hipMemcpy(h, b.examined, sizeof(h), hipMemcpyDeviceToHost));
       unsigned long long ex = 0;
       for (unsigned long long x : h) ex += x;
       stats->total_ms = wt.ms();

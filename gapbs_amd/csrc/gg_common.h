@@ -75,6 +75,8 @@ struct Workspace {
   // smaller.  Contents are undefined.
   void* Get(const std::string& name, size_t bytes);
   void Release();
+  // Frees the buffers whose names start with `prefix`.
+  void Drop(const std::string& prefix);
   size_t Bytes() const;
 };
 
@@ -99,6 +101,10 @@ struct gg_graph {
   // old_id[device id] its original id.  Both null: ids as uploaded.
   int32_t* new_id = nullptr;
   int32_t* old_id = nullptr;
+  // Degree-ordered unweighted handle: the lists in uploaded order (each row
+  // sorted by uploaded id, entries in device ids), built on first use by
+  // UploadOrderLists.  Null otherwise.
+  int32_t* fold_dst = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
   gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}};  // gg_autotune results
@@ -112,6 +118,8 @@ struct gg_graph {
   hipGraphExec_t ms_exec = nullptr;      // batched traversal: two captured levels, replayed
   std::vector<uintptr_t> ms_exec_key;    // the buffers and arguments captured in it
   int ms_exec_kernels = 0;               // kernels per replay
+  hipGraphExec_t sb_exec = nullptr;      // batched SSSP: captured rounds, replayed
+  std::vector<uintptr_t> sb_exec_key;    // the buffers and arguments captured in it
   gg::Workspace ws;
   void* pinned = nullptr;  // small pinned host block for counter read-back
   size_t pinned_bytes = 0;
@@ -173,6 +181,15 @@ void CopyOutVertexF32(gg_graph* g, const float* src, float* host);
 // Component labels in device order, each the smallest device id of its
 // component -> host, each the smallest original id of its component.
 void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host);
+// The same with the caller's two device work arrays of n int32 (unused, and
+// may be null, when the handle is not degree-ordered).
+void CopyOutComponentsWith(gg_graph* g, const int32_t* comp, int32_t* host, int32_t* least, int32_t* tmp);
+// The out-lists in uploaded order: g->out_dst unless the handle is
+// degree-ordered and unweighted (its rows are sorted by device id), then
+// g->fold_dst, built here on first use (device ids, each row sorted by
+// uploaded id).  Kernels whose result depends on the order in which a row
+// is summed read these.  Synchronous on first use; call outside timing.
+const int32_t* UploadOrderLists(gg_graph* g);
 
 // Direction-optimizing BFS (gg_bfs.hip) from device id `source` on
 // g->stream, inside the caller's timed region: parents in the workspace

@@ -68,6 +68,17 @@ void Workspace::Release() {
   slots.clear();
 }
 
+void Workspace::Drop(const std::string& prefix) {
+  for (auto it = slots.begin(); it != slots.end();) {
+    if (it->first.compare(0, prefix.size(), prefix) == 0) {
+      DeviceFree(it->second.p, it->second.bytes);
+      it = slots.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
 size_t Workspace::Bytes() const {
   size_t b = 0;
   for (const auto& kv : slots) b += kv.second.bytes;
@@ -297,6 +308,8 @@ int gg_graph_upload(int64_t num_nodes, const int64_t* out_offsets, const int32_t
     const int64_t e_out = out_offsets[num_nodes] - out_offsets[0];
     GG_REQUIRE(out_offsets[0] == 0 && e_out >= 0, GG_EINVAL, "offsets must start at 0");
     GG_REQUIRE(e_out == 0 || out_dests, GG_EINVAL, "null destinations");
+    GG_REQUIRE(!directed || (in_offsets[0] == 0 && in_offsets[num_nodes] >= 0), GG_EINVAL,
+               "inverse offsets must start at 0");
     const int width = weighted ? 2 : 1;
 
     gg_graph* g = new gg_graph;
@@ -363,6 +376,7 @@ int gg_graph_free(gg_graph* g) {
   return Guard([&] {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->ms_exec) (void)hipGraphExecDestroy(g->ms_exec);
+    if (g->sb_exec) (void)hipGraphExecDestroy(g->sb_exec);
     g->ws.Release();
     const size_t off_bytes = sizeof(int64_t) * (g->n + 1);
     const int width = g->weighted ? 2 : 1;
@@ -372,6 +386,7 @@ int gg_graph_free(gg_graph* g) {
       DeviceFree(g->in_off, off_bytes);
       DeviceFree(g->in_dst, sizeof(int32_t) * width * g->in_entries);
     }
+    DeviceFree(g->fold_dst, sizeof(int32_t) * g->out_entries);
     DeviceFree(g->new_id, sizeof(int32_t) * (g->n > 0 ? g->n : 1));
     DeviceFree(g->old_id, sizeof(int32_t) * (g->n > 0 ? g->n : 1));
     if (g->pinned) (void)hipHostFree(g->pinned);

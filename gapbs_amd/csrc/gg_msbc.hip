@@ -503,6 +503,9 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
   unsigned long long* ctr = g->Scratch<unsigned long long>("mb.ctr", 2 * kMbStripes + 1);
   const double N = static_cast<double>(n);
 
+  // Dependencies are folded in each vertex's uploaded list order (the
+  // reference's successor order), also on a degree-ordered handle.
+  const int32_t* fold_dst = UploadOrderLists(g);
   BeginTiming(g);
   MsTraversal t = MsTraverse(g, droots, nullptr, k, k == 64 ? ~0ull : (1ull << k) - 1, 15, nullptr, true);
   const int levels = static_cast<int>(t.frontiers.size());
@@ -549,7 +552,7 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
       GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
       RunOn(g, g->side, "msbc_delta_hub_wide", 0, MbDeltaHub<T, kHubLarge>, dim3(GridFor(nwide, 1, g->cu_count)),
             dim3((kHubLarge + 1) * kWave), static_cast<const int64_t*>(g->out_off),
-            static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
+            fold_dst, static_cast<const uint64_t*>(t.frontiers[L]), next,
             static_cast<const double*>(sigma), delta, static_cast<const int32_t*>(wide), nwide, ctr + kMbStripes);
       GG_HIP(hipEventRecord(g->ev_join, g->side));
       launches++;
@@ -558,14 +561,14 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
       GG_HIP(hipStreamWaitEvent(g->side2, g->ev_fork, 0));
       RunOn(g, g->side2, "msbc_delta_hub", 0, MbDeltaHub<T, kHubSmall>,
             dim3(GridFor(nhubs, 1, g->cu_count * kHubSmallBlocksPerCu)), dim3((kHubSmall + 1) * kWave),
-            static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
+            static_cast<const int64_t*>(g->out_off), fold_dst,
             static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
             static_cast<const int32_t*>(hubs), nhubs, ctr + kMbStripes);
       GG_HIP(hipEventRecord(g->ev_join2, g->side2));
       launches++;
     }
     Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
-        static_cast<const int32_t*>(g->out_dst), static_cast<const uint64_t*>(t.frontiers[L]), next,
+        fold_dst, static_cast<const uint64_t*>(t.frontiers[L]), next,
         static_cast<const double*>(sigma), delta, n, ctr + kMbStripes);
     launches++;
     if (nwide > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));

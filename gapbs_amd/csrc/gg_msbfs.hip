@@ -79,24 +79,45 @@ extern "C" int gg_bfs_batch(gg_graph* g, const int32_t* roots, int k, int alpha,
 namespace gg {
 namespace {
 
-// Checks that every vertex's neighbours lie in its own part (the lists are
-// sorted, so the first and last entries decide); sets *bad otherwise.
+// Checks that every list entry lies in its vertex's part (every entry, so
+// lists need not be sorted); sets *bad otherwise.  A wavefront per vertex
+// range: each lane finds its vertex's part once, then the wavefront walks
+// the lists of its 64 vertices with coalesced reads.
 __global__ __launch_bounds__(kBS) void PkCheckParts(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                     int64_t n, const int64_t* __restrict__ part, int nparts,
                                                     int32_t* bad) {
-  for (int64_t v = blockIdx.x * int64_t(kBS) + threadIdx.x; v < n; v += int64_t(gridDim.x) * kBS) {
-    const int64_t b = off[v], e = off[v + 1];
-    if (b == e) continue;
-    int lo = 0, hi = nparts;  // part[lo] <= v < part[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (part[mid] <= v)
-        lo = mid;
-      else
-        hi = mid;
+  const int lane = LaneId();
+  const int64_t wave = (blockIdx.x * int64_t(kBS) + threadIdx.x) / kWave;
+  const int64_t waves = int64_t(gridDim.x) * kBS / kWave;
+  bool wrong = false;
+  for (int64_t v0 = wave * kWave; v0 < n; v0 += waves * kWave) {
+    const int64_t v = v0 + lane;
+    int64_t lo_id = 0, hi_id = 0, b = 0, e = 0;
+    if (v < n) {
+      int lo = 0, hi = nparts;  // part[lo] <= v < part[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (part[mid] <= v)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      lo_id = part[lo];
+      hi_id = part[lo + 1];
+      b = off[v];
+      e = off[v + 1];
     }
-    if (dst[b] < part[lo] || dst[e - 1] >= part[lo + 1]) atomicOr(bad, 1);
+    const int last = static_cast<int>(n - v0 < kWave ? n - v0 : kWave);
+    for (int j = 0; j < last; j++) {
+      const int64_t jb = __shfl(b, j), je = __shfl(e, j);
+      const int64_t jlo = __shfl(lo_id, j), jhi = __shfl(hi_id, j);
+      for (int64_t k = jb + lane; k < je; k += kWave) {
+        const int32_t d = dst[k];
+        wrong |= d < jlo || d >= jhi;
+      }
+    }
   }
+  if (__ballot(wrong) && lane == 0) atomicOr(bad, 1);
 }
 
 // depth16[v][j] -> out[R * part[i] + j * |part i| + (v - part[i])] for the

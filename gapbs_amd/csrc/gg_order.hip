@@ -128,14 +128,13 @@ void CopyOutVertexI32(gg_graph* g, const int32_t* src, int32_t* host, bool ids) 
 
 void CopyOutVertexF32(gg_graph* g, const float* src, float* host) { CopyOutVertex<float>(g, src, host, false); }
 
-void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host) {
+void CopyOutComponentsWith(gg_graph* g, const int32_t* comp, int32_t* host, int32_t* least, int32_t* tmp) {
+  const int64_t n = g->n;
   if (!g->new_id) {
-    CopyOutVertex<int32_t>(g, comp, host, false);
+    GG_HIP(hipMemcpyAsync(host, comp, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g->stream));
+    GG_HIP(hipStreamSynchronize(g->stream));
     return;
   }
-  const int64_t n = g->n;
-  int32_t* least = g->Scratch<int32_t>("order.least", n);
-  int32_t* tmp = g->Scratch<int32_t>("order.out", n);
   const dim3 grid(GridFor(n, kBS, g->cu_count * 16)), blk(kBS);
   hipLaunchKernelGGL(OrFill, grid, blk, 0, g->stream, least, n, INT32_MAX);
   hipLaunchKernelGGL(OrComponentMin, grid, blk, 0, g->stream, comp, static_cast<const int32_t*>(g->old_id), n,
@@ -145,6 +144,15 @@ void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host) {
   GG_HIP(hipGetLastError());
   GG_HIP(hipMemcpyAsync(host, tmp, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g->stream));
   GG_HIP(hipStreamSynchronize(g->stream));
+}
+
+void CopyOutComponents(gg_graph* g, const int32_t* comp, int32_t* host) {
+  if (!g->new_id) {
+    CopyOutVertex<int32_t>(g, comp, host, false);
+    return;
+  }
+  CopyOutComponentsWith(g, comp, host, g->Scratch<int32_t>("order.least", g->n),
+                        g->Scratch<int32_t>("order.out", g->n));
 }
 
 }  // namespace gg

@@ -264,7 +264,41 @@ void SortRows(gg_graph* g, const int64_t* off, int32_t* dst) {
   GG_HIP(hipGetLastError());
 }
 
+// out[i] = map[in[i]] over m entries.
+__global__ __launch_bounds__(kBS) void RlMapEntries(const int32_t* in, int64_t m, const int32_t* __restrict__ map,
+                                                    int32_t* out) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < m; i += int64_t(gridDim.x) * kBS) out[i] = map[in[i]];
+}
+
 }  // namespace
+
+const int32_t* UploadOrderLists(gg_graph* g) {
+  if (!g->new_id || g->weighted) return g->out_dst;
+  if (g->fold_dst) return g->fold_dst;
+  const int64_t m = g->out_entries;
+  int32_t* f = static_cast<int32_t*>(DeviceAlloc(sizeof(int32_t) * m));
+  try {
+    // Device ids -> uploaded ids, rows sorted (they are in descending length
+    // order, as SortRows needs), uploaded ids -> device ids.
+    const dim3 grid(GridFor(m, kBS, g->cu_count * 16)), blk(kBS);
+    hipLaunchKernelGGL(RlMapEntries, grid, blk, 0, g->stream, static_cast<const int32_t*>(g->out_dst), m,
+                       static_cast<const int32_t*>(g->old_id), f);
+    SortRows(g, g->out_off, f);
+    hipLaunchKernelGGL(RlMapEntries, grid, blk, 0, g->stream, static_cast<const int32_t*>(f), m,
+                       static_cast<const int32_t*>(g->new_id), f);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipStreamSynchronize(g->stream));
+  } catch (...) {
+    (void)hipStreamSynchronize(g->stream);
+    DeviceFree(f, sizeof(int32_t) * m);
+    throw;
+  }
+  g->ws.Drop("rl.");
+  g->fold_dst = f;
+  g->graph_bytes += sizeof(int32_t) * m;
+  return f;
+}
+
 }  // namespace gg
 
 using namespace gg;
@@ -357,6 +391,11 @@ extern "C" int gg_graph_order_by_degree(gg_graph* g, gg_stats* stats) {
       (void)hipGraphExecDestroy(g->ms_exec);
       g->ms_exec = nullptr;
       g->ms_exec_key.clear();
+    }
+    if (g->sb_exec) {
+      (void)hipGraphExecDestroy(g->sb_exec);
+      g->sb_exec = nullptr;
+      g->sb_exec_key.clear();
     }
     g->ws.Release();
     if (stats) stats->total_ms = wt.ms();

@@ -15,6 +15,7 @@
 // checked on the device as an upload's are (offsets from 0 to entries and
 // non-decreasing, destinations in range).
 #include <fcntl.h>
+#include <sys/stat.h>
 #include <unistd.h>
 
 #include <cstdio>
@@ -118,8 +119,17 @@ extern "C" int gg_graph_read(const char* path, int weighted, gg_graph** out, int
                GG_EINVAL, "Short read from " + p);
     s.pos = 17;
     GG_REQUIRE(n >= 0 && n < (int64_t(1) << 31) && entries >= 0, GG_EINVAL, "Corrupt header in " + p);
+    // Entry counts are bounded (2^40 entries is 8 TB of lists) before any
+    // size is computed, so the byte counts below cannot wrap; the file must
+    // then hold exactly the header and the lists it announces.
+    GG_REQUIRE(entries < (int64_t(1) << 40), GG_EINVAL, "Corrupt header in " + p);
     const int width = weighted ? 2 : 1;
     const size_t off_bytes = sizeof(int64_t) * (n + 1), dst_bytes = sizeof(int32_t) * width * entries;
+    struct stat fs;
+    GG_REQUIRE(fstat(s.fd, &fs) == 0, GG_EINVAL, "Couldn't stat file " + p);
+    const uint64_t want = 17 + (directed ? 2 : 1) * (uint64_t(off_bytes) + dst_bytes);
+    GG_REQUIRE(static_cast<uint64_t>(fs.st_size) >= want, GG_EINVAL, "Short read from " + p);
+    GG_REQUIRE(static_cast<uint64_t>(fs.st_size) == want, GG_EINVAL, "Corrupt header in " + p);
     int device = 0;
     GG_HIP(hipGetDevice(&device));
     for (int i = 0; i < 2; i++) {

@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
         L.gg_bfs_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_bfs_packed.argtypes = [vp, _i64p, C.c_int, _i32p, C.c_int, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_sssp.argtypes = [vp, C.c_int32, C.c_int32, _i32p, C.POINTER(Stats)]
+        L.gg_sssp_batch.argtypes = [vp, _i32p, C.c_int, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
@@ -137,6 +138,16 @@ def _check(rc: int) -> int:
     if rc < 0:
         raise GpuError(rc, lib().gg_last_error().decode())
     return rc
+
+
+def _out(shape, dtype, out):
+    """The caller's preallocated result array (reused across calls, so
+    timing loops do not fault in fresh host pages), or a new one."""
+    if out is None:
+        return np.empty(shape, dtype=dtype)
+    if out.shape != tuple(shape) or out.dtype != np.dtype(dtype) or not out.flags.c_contiguous:
+        raise ValueError(f"out must be a C-contiguous {np.dtype(dtype)} array of shape {tuple(shape)}")
+    return out
 
 
 def init(device: int = 0) -> None:
@@ -287,13 +298,13 @@ class BatchBFSResult:
     stats: dict
 
 
-def bfs_batch(dg: DeviceGraph, roots, alpha: int = 15, want_depths: bool = True) -> BatchBFSResult:
+def bfs_batch(dg: DeviceGraph, roots, alpha: int = 15, want_depths: bool = True, out=None) -> BatchBFSResult:
     """gg_bfs_batch: up to 64 roots in one traversal; depths[r] equals the
     single-root BFS depths from roots[r] (the reference's per-trial loop,
     src/benchmark.h:106, over src/bfs.cc:122-179)."""
     r = np.ascontiguousarray(roots, dtype=np.int32)
     n = dg.num_nodes
-    depths = np.empty((len(r), n), dtype=np.int32) if want_depths else None
+    depths = _out((len(r), n), np.int32, out) if want_depths else None
     st = Stats()
     _check(lib().gg_bfs_batch(dg.handle, _p(r, C.c_int32), len(r), alpha,
                               _p(depths, C.c_int32) if want_depths else None, C.byref(st)))
@@ -311,7 +322,11 @@ def bfs_packed(dg: DeviceGraph, part, roots, alpha: int = 15, want_depths: bool 
     ranges part[i]..part[i+1]); roots is [nparts, R] global vertex ids."""
     part = np.ascontiguousarray(part, dtype=np.int64)
     r = np.ascontiguousarray(roots, dtype=np.int32)
+    if r.ndim != 2:
+        raise ValueError("roots must be a [nparts, roots_per_part] array")
     nparts, R = r.shape
+    if part.ndim != 1 or part.size != nparts + 1:
+        raise ValueError(f"part needs nparts + 1 = {nparts + 1} bounds, got {part.size}")
     n = dg.num_nodes
     flat = np.empty(R * n, dtype=np.int32) if want_depths else None
     st = Stats()
@@ -332,13 +347,33 @@ class SSSPResult:
     stats: dict
 
 
-def sssp(dg: DeviceGraph, source: int, delta: int, want_dist: bool = True) -> SSSPResult:
+def sssp(dg: DeviceGraph, source: int, delta: int, want_dist: bool = True, out=None) -> SSSPResult:
     """gg_sssp; with want_dist False the distances stay on the device."""
-    dist = np.empty(dg.num_nodes, dtype=np.int32) if want_dist else None
+    dist = _out((dg.num_nodes,), np.int32, out) if want_dist else None
     st = Stats()
     _check(lib().gg_sssp(dg.handle, source, delta, _p(dist, C.c_int32) if want_dist else None,
                          C.byref(st)))
     return SSSPResult(dist, st.as_dict())
+
+
+@dataclass
+class SSSPBatchResult:
+    dist: np.ndarray | None  # [k, n]
+    stats: dict
+
+
+def sssp_batch(dg: DeviceGraph, roots, delta: int, want_dist: bool = True, out=None) -> SSSPBatchResult:
+    """gg_sssp_batch: up to 64 roots in one delta-stepping traversal;
+    dist[i] equals sssp(dg, roots[i], delta).dist."""
+    r = np.ascontiguousarray(roots, dtype=np.int32)
+    if r.ndim != 1 or not 1 <= r.size <= 64:
+        raise ValueError("1 to 64 roots")
+    k, n = r.size, dg.num_nodes
+    dist = _out((k, n), np.int32, out) if want_dist else None
+    st = Stats()
+    _check(lib().gg_sssp_batch(dg.handle, _p(r, C.c_int32), k, delta,
+                               _p(dist, C.c_int32) if want_dist else None, C.byref(st)))
+    return SSSPBatchResult(dist, st.as_dict())
 
 
 def DeltaStep(dg: DeviceGraph, source: int, delta: int) -> np.ndarray:
@@ -356,10 +391,10 @@ class PRResult:
 
 
 def pagerank(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4, fp64: bool = False,
-             want_scores: bool = True, gauss_seidel: bool = False) -> PRResult:
+             want_scores: bool = True, gauss_seidel: bool = False, out=None) -> PRResult:
     """gg_pagerank: Jacobi with FP32 compensated (default) or FP64 accumulation,
     or the in-place Gauss-Seidel sweep (gauss_seidel=True, FP32)."""
-    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    out = _out((dg.num_nodes,), np.float32, out) if want_scores else None
     st = Stats()
     mode = PR_GS if gauss_seidel else (PR_FP64 if fp64 else PR_FP32)
     _check(lib().gg_pagerank(dg.handle, max_iters, epsilon, mode,
@@ -383,9 +418,9 @@ class CCResult:
     stats: dict
 
 
-def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True) -> CCResult:
+def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True, out=None) -> CCResult:
     """gg_cc: Afforest labels (smallest vertex id per component)."""
-    comp = np.empty(dg.num_nodes, dtype=np.int32) if want_comp else None
+    comp = _out((dg.num_nodes,), np.int32, out) if want_comp else None
     st = Stats()
     _check(lib().gg_cc(dg.handle, neighbor_rounds, _p(comp, C.c_int32) if want_comp else None,
                        C.byref(st)))
@@ -420,13 +455,13 @@ class BCResult:
 
 
 def bc(dg: DeviceGraph, roots, fp64: bool = False, kahan: bool = False,
-       want_scores: bool = True) -> BCResult:
+       want_scores: bool = True, out=None) -> BCResult:
     """gg_bc: max-normalised Brandes scores from the given roots.
 
     Default: the reference's float arithmetic; kahan=True: compensated score
     sums; fp64=True: double dependencies and scores."""
     roots = np.ascontiguousarray(roots, dtype=np.int32)
-    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    out = _out((dg.num_nodes,), np.float32, out) if want_scores else None
     mode = BC_FP64 if fp64 else (BC_FP32_KAHAN if kahan else BC_FP32)
     st = Stats()
     _check(lib().gg_bc(dg.handle, _p(roots, C.c_int32), len(roots), mode,
@@ -434,11 +469,11 @@ def bc(dg: DeviceGraph, roots, fp64: bool = False, kahan: bool = False,
     return BCResult(out, st.as_dict())
 
 
-def bc_batch(dg: DeviceGraph, roots, fp64: bool = False, want_scores: bool = True) -> BCResult:
+def bc_batch(dg: DeviceGraph, roots, fp64: bool = False, want_scores: bool = True, out=None) -> BCResult:
     """gg_bc_batch: up to 64 roots in one traversal; equals gg_bc over the
     same roots (bit for bit with the reference's float arithmetic)."""
     roots = np.ascontiguousarray(roots, dtype=np.int32)
-    out = np.empty(dg.num_nodes, dtype=np.float32) if want_scores else None
+    out = _out((dg.num_nodes,), np.float32, out) if want_scores else None
     st = Stats()
     _check(lib().gg_bc_batch(dg.handle, _p(roots, C.c_int32), len(roots), BC_FP64 if fp64 else BC_FP32,
                              _p(out, C.c_float) if want_scores else None, C.byref(st)))

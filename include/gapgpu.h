@@ -110,6 +110,21 @@ int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const int32_t* r
  * shortest distance, 1073741823 (INT32_MAX / 2) if unreached. */
 int gg_sssp(gg_graph* g, int32_t source, int32_t delta, int32_t* dist, gg_stats* stats);
 
+/* Batched delta-stepping: k (1..64) roots searched in one traversal that
+ * shares the buckets (width delta) and the edge reads between roots; the
+ * bucket loop runs on the device (its state in HBM, rounds replayed from a
+ * captured hipGraph).  Replaces k sequential DeltaStep trials
+ * (src/benchmark.h:106 around src/sssp.cc:87-153); each root's distances
+ * equal its gg_sssp distances bit for bit.  dist[k * n] (nullable: kept on
+ * the device), root-major: dist[r * n + v] is v's distance from roots[r],
+ * 1073741823 if unreached.  Repeated roots are allowed.  Device memory
+ * beyond the graph: 4 G + 40 bytes per vertex, G = the smallest power of two
+ * >= k (plus 4 k per vertex for the copy-out).  stats->reached counts
+ * reached (root, vertex) pairs, stats->reached_edges sums their out-degrees,
+ * stats->iterations counts buckets, stats->td_steps relax launches and
+ * stats->edges_examined (edge, root) relaxations. */
+int gg_sssp_batch(gg_graph* g, const int32_t* roots, int k, int32_t delta, int32_t* dist, gg_stats* stats);
+
 /* Pull PageRank with damping 0.85.
  * Replaces PageRankPull(const Graph&, int max_iters, double epsilon, bool
  * logging) (src/pr_spmv.cc:35-61): Jacobi iterations until the L1 change

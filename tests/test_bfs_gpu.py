@@ -149,6 +149,51 @@ def test_bfs_errors():
     assert e.value.code == -95
 
 
+def _raw_upload(n, off, dst, directed=False, ioff=None, idst=None):
+    import ctypes as C
+    h = C.c_void_p()
+    p64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64)) if a is not None else None  # noqa: E731
+    p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None  # noqa: E731
+    rc = gpu.lib().gg_graph_upload(n, p64(off), p32(dst), 0, int(directed), p64(ioff), p32(idst), C.byref(h))
+    if rc == 0:
+        gpu.lib().gg_graph_free(h)
+    return rc
+
+
+def test_upload_errors():
+    """Malformed CSR arrays are refused on the host or by the device
+    validation pass, never handed to a kernel."""
+    gpu.init(0)
+    off = np.array([0, 1, 2], dtype=np.int64)
+    dst = np.array([1, 0], dtype=np.int32)
+    assert _raw_upload(2, off, dst) == 0
+    assert _raw_upload(2, off, np.array([1, 2], dtype=np.int32)) == -22  # destination == n
+    assert _raw_upload(2, np.array([0, 2, 1], dtype=np.int64), dst) == -22  # decreasing
+    assert _raw_upload(2, np.array([1, 1, 2], dtype=np.int64), dst) == -22  # out side not from 0
+    # A directed upload whose inverse offsets start below 0 (they would index
+    # in_dests before its start) or above it.
+    assert _raw_upload(2, off, dst, True, off, dst) == 0
+    for bad in ([-4, 1, 2], [1, 1, 2]):
+        assert _raw_upload(2, off, dst, True, np.array(bad, dtype=np.int64), dst) == -22, bad
+
+
+def test_memory_cap(monkeypatch):
+    """GAPGPU_MEM_CAP_BYTES caps the library's device allocations: an upload
+    beyond it returns GG_ENOMEM (a host-side error return) and leaves nothing
+    allocated, and uploads work again once the cap is lifted."""
+    gpu.init(0)
+    g = Graph.kronecker(10)
+    monkeypatch.setenv("GAPGPU_MEM_CAP_BYTES", "4096")
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.DeviceGraph(g)
+    assert e.value.code == -12
+    assert "GAPGPU_MEM_CAP_BYTES" in str(e.value)
+    monkeypatch.delenv("GAPGPU_MEM_CAP_BYTES")
+    dg = gpu.DeviceGraph(g)
+    r = int(g.pick_sources(1)[0])
+    assert np.array_equal(gpu.bfs(dg, r).depth, oracle.bfs_depth(g, r))
+
+
 def test_bfs_parent_only_matches_depth_run():
     g = cached_graph("g16", lambda: Graph.kronecker(16))
     dg = dev("g16", lambda: Graph.kronecker(16))

@@ -12,8 +12,9 @@ point still takes and returns uploaded ids.  Checked here:
     (src/bfs.cc:204-241) in uploaded ids;
   * PageRank stays within 1e-6 of the oracle's FP64 scores;
   * BC scores, single-root and batched, are bit-exact with the oracle's
-    float Brandes (src/bc.cc:169-220) run on the relabelled host graph and
-    mapped back, and within 1e-5 of the uploaded-order oracle;
+    float Brandes (src/bc.cc:95-150) on the uploaded graph: the dependency
+    folds read the lists in uploaded order, so the sums are the reference's
+    in its own successor order;
   * packed traversal, directed graphs and a second ordering are rejected.
 """
 
@@ -110,16 +111,19 @@ def test_cc_many_components():
     assert np.array_equal(gpu.cc(dg).comp, oracle.cc_min_labels(g))
 
 
-def test_bc_bit_exact_on_relabelled_graph():
-    print("[order] bc g12", flush=True)
-    g, dg = ordered("g12", lambda: Graph.kronecker(12))
+@pytest.mark.parametrize("tag,make", [("g12", lambda: Graph.kronecker(12)), ("u12", lambda: Graph.uniform(12)),
+                                      ("g15", lambda: Graph.kronecker(15, seed=9))])
+def test_bc_bit_exact_in_uploaded_order(tag, make):
+    print(f"[order] bc {tag}", flush=True)
+    g, dg = ordered(tag, make)
     roots = g.pick_sources(8)
-    new_id = dg.vertex_order()
-    h = g.relabel_by_degree()
-    want = oracle.brandes(h, new_id[roots])[new_id]
+    want = oracle.brandes(g, roots)
     assert np.array_equal(gpu.bc(dg, roots).scores, want)
     assert np.array_equal(gpu.bc_batch(dg, roots).scores, want)
-    assert np.abs(want - oracle.brandes(g, roots)).max() < 1e-5
+    # The same handle's FP64 paths against the FP64 oracle.
+    w64 = oracle.brandes(g, roots, fp64=True)
+    assert np.abs(gpu.bc(dg, roots, fp64=True).scores - w64).max() < 1e-6
+    assert np.abs(gpu.bc_batch(dg, roots, fp64=True).scores - w64).max() < 1e-6
 
 
 def test_rejects_directed_and_repeat():

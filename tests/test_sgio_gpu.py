@@ -71,6 +71,22 @@ def test_errors(tmp_path):
         with pytest.raises(gpu.GpuError) as e:
             gpu.read_graph(str(tmp_path / name))
         assert e.value.code == -22, name
+    # A crafted 33-byte file whose header announces 2^62 entries (one
+    # vertex, offsets {0, 2^62}): the byte count would wrap to 0, so the
+    # header is refused on the host before any allocation or kernel.
+    import struct
+    for suffix in (".sg", ".wsg"):
+        for entries in (1 << 62, 1 << 61, 1 << 39):
+            blob = struct.pack("<?qq", False, entries, 1) + struct.pack("<qq", 0, entries)
+            path = tmp_path / f"wrap{entries}{suffix}"
+            path.write_bytes(blob)
+            with pytest.raises(gpu.GpuError) as e:
+                gpu.read_graph(str(path))
+            assert e.value.code == -22, (entries, suffix)
+    (tmp_path / "trailing.sg").write_bytes(data + b"\0" * 8)  # longer than its header says
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.read_graph(str(tmp_path / "trailing.sg"))
+    assert e.value.code == -22
     with pytest.raises(gpu.GpuError) as e:
         gpu.read_graph(str(tmp_path / "missing.sg"))
     assert e.value.code == -22
