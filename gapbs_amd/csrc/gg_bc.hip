@@ -149,7 +149,10 @@ __device__ __forceinline__ void EmitReduceChunks(int32_t u, int64_t beg, int64_t
 __global__ __launch_bounds__(kBS) void BcSigma(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
                                                const int32_t* __restrict__ depth, double* __restrict__ sigma,
                                                const int32_t* __restrict__ queue, int32_t qb, int32_t qn,
-                                               int32_t prev_depth, EdgeChunk* chunks, BcCounters* c) {
+                                               int32_t prev_depth, EdgeChunk* chunks, BcCounters* c,
+                                               unsigned long long* work) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  unsigned long long ent = 0, mat = 0;
   for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < qn; i += int64_t(gridDim.x) * kBS) {
     const int32_t v = queue[qb + i];
     const int64_t b = in_off[v], e = in_off[v + 1];
@@ -161,28 +164,40 @@ __global__ __launch_bounds__(kBS) void BcSigma(const int64_t* __restrict__ in_of
     double s = 0.0;
     for (int64_t k = b; k < e; k++) {
       const int32_t u = in_dst[k];
-      if (depth[u] == prev_depth) s += sigma[u];
+      if (depth[u] == prev_depth) {
+        s += sigma[u];
+        mat++;
+      }
     }
+    ent += static_cast<unsigned long long>(e - b);
     sigma[v] = s;
   }
+  WorkAdd(work, BlockSum<kBS>(ent, s_red), BlockSum<kBS>(mat, s_red));
 }
 
 __global__ __launch_bounds__(kBS) void BcSigmaChunks(const int32_t* __restrict__ in_dst,
                                                      const int32_t* __restrict__ depth, double* sigma,
                                                      int32_t prev_depth, const EdgeChunk* chunks,
-                                                     const BcCounters* c) {
+                                                     const BcCounters* c, unsigned long long* work) {
   __shared__ double s_red[kBS / kWave];
+  __shared__ unsigned long long s_cnt[kBS / kWave];
   const int32_t nc = c->nred;
+  unsigned long long ent = 0, mat = 0;
   for (int32_t ci = blockIdx.x; ci < nc; ci += gridDim.x) {
     const EdgeChunk ch = chunks[ci];
     double s = 0.0;
     for (int64_t k = ch.beg + threadIdx.x; k < ch.end; k += kBS) {
       const int32_t u = in_dst[k];
-      if (depth[u] == prev_depth) s += sigma[u];
+      ent++;
+      if (depth[u] == prev_depth) {
+        s += sigma[u];
+        mat++;
+      }
     }
     s = BlockSum<kBS>(s, s_red);
     if (threadIdx.x == 0 && s != 0.0) atomicAdd(&sigma[ch.u], s);
   }
+  WorkAdd(work, BlockSum<kBS>(ent, s_cnt), BlockSum<kBS>(mat, s_cnt));
 }
 
 // delta for one level, the reference's way: a double term added into a
@@ -258,7 +273,10 @@ __global__ __launch_bounds__(kBS) void BcDelta(const int64_t* __restrict__ off, 
                                                const double* __restrict__ sigma, T* __restrict__ delta,
                                                T* __restrict__ score, T* __restrict__ comp,
                                                const int32_t* __restrict__ queue, int32_t qb, int32_t qn,
-                                               int32_t level, int32_t* __restrict__ wave_list, BcCounters* c) {
+                                               int32_t level, int32_t* __restrict__ wave_list, BcCounters* c,
+                                               unsigned long long* work) {
+  __shared__ unsigned long long s_red[kBS / kWave];
+  unsigned long long ent = 0, mat = 0;
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < qn; base += int64_t(gridDim.x) * kBS) {
     const int64_t i = base + threadIdx.x;
     bool big = false;
@@ -272,8 +290,12 @@ __global__ __launch_bounds__(kBS) void BcDelta(const int64_t* __restrict__ off, 
         typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
         for (int64_t k = b; k < e; k++) {
           const int32_t v = dst[k];
-          if (depth[v] == level + 1) acc.Add(DeltaTerm(su, sigma[v], delta[v]));
+          if (depth[v] == level + 1) {
+            acc.Add(DeltaTerm(su, sigma[v], delta[v]));
+            mat++;
+          }
         }
+        ent += static_cast<unsigned long long>(e - b);
         const T d = acc.Value();
         delta[u] = d;
         AddScore<kKahan>(score, comp, u, d);
@@ -281,6 +303,7 @@ __global__ __launch_bounds__(kBS) void BcDelta(const int64_t* __restrict__ off, 
     }
     WaveAppend(big, u, wave_list, &c->nwave);
   }
+  WorkAdd(work, BlockSum<kBS>(ent, s_red), BlockSum<kBS>(mat, s_red));
 }
 
 // One wavefront per listed vertex: 64 successor terms computed at once,
@@ -291,8 +314,10 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
                                                    const double* __restrict__ sigma, T* __restrict__ delta,
                                                    T* __restrict__ score, T* __restrict__ comp,
                                                    const int32_t* __restrict__ wave_list, int32_t level,
-                                                   const BcCounters* c) {
+                                                   const BcCounters* c, unsigned long long* work) {
+  __shared__ unsigned long long s_red[kBS / kWave];
   const int lane = LaneId();
+  unsigned long long ent = 0, mat = 0;  // lane 0's
   const int32_t nw = c->nwave;
   const int64_t wave = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kBS / kWave);
@@ -314,6 +339,7 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
         }
       }
       uint64_t m = __ballot(has);
+      if (lane == 0) mat += static_cast<unsigned long long>(__popcll(m));
       while (m) {
         const int j = __ffsll(static_cast<unsigned long long>(m)) - 1;
         acc.Add(__shfl(term, j));
@@ -321,11 +347,13 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
       }
     }
     if (lane == 0) {
+      ent += static_cast<unsigned long long>(e - b);
       const T d = acc.Value();
       delta[u] = d;
       AddScore<kKahan>(score, comp, u, d);
     }
   }
+  WorkAdd(work, BlockSum<kBS>(ent, s_red), BlockSum<kBS>(mat, s_red));
 }
 
 // Listed vertices with at least kBcHubEdges successors, one block each:
@@ -342,10 +370,12 @@ __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict
                                                        const double* __restrict__ sigma, T* __restrict__ delta,
                                                        T* __restrict__ score, T* __restrict__ comp,
                                                        const int32_t* __restrict__ wave_list, int32_t level,
-                                                       const BcCounters* c) {
+                                                       const BcCounters* c, unsigned long long* work) {
   __shared__ double terms[2][kBcHubProducers][kWave];
   __shared__ int counts[2][kBcHubProducers];
+  __shared__ unsigned long long s_red[kBcHubBS / kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
+  unsigned long long ent = 0, mat = 0;  // lane 0 of each producer's
   const int32_t nw = c->nwave;
   constexpr int64_t kRound = int64_t(kBcHubProducers) * kWave;
   for (int32_t i = blockIdx.x; i < nw; i += gridDim.x) {
@@ -369,8 +399,13 @@ __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict
             }
           }
           const uint64_t m = __ballot(has);
+          const uint64_t in = __ballot(kk < e);
           if (has) terms[k & 1][wid][__popcll(LanesBelow(m))] = term;
-          if (lane == 0) counts[k & 1][wid] = __popcll(m);
+          if (lane == 0) {
+            counts[k & 1][wid] = __popcll(m);
+            mat += static_cast<unsigned long long>(__popcll(m));
+            ent += static_cast<unsigned long long>(__popcll(in));
+          }
         }
       } else if (k >= 1) {
         const int h = (k - 1) & 1;
@@ -395,6 +430,7 @@ __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict
       AddScore<kKahan>(score, comp, u, d);
     }
   }
+  WorkAdd(work, BlockSum<kBcHubBS>(ent, s_red), BlockSum<kBcHubBS>(mat, s_red));
 }
 
 
@@ -540,6 +576,15 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   float* out32 = g->Scratch<float>("bc.out32", n);
   unsigned long long* max_bits = g->Scratch<unsigned long long>("bc.max", 1);
   BcCounters* dc = g->Scratch<BcCounters>("bc.ctr", 1);
+  // Work counters (entries read, matched in-neighbours / successors) per
+  // kernel kind, for the profiled byte credits.
+  enum { kWSigma, kWSigmaChunks, kWDelta, kWDeltaWave, kWDeltaHub, kWKinds };
+  unsigned long long* work = g->Scratch<unsigned long long>("bc.work", kWKinds * kWorkCounters);
+  int rec[kWKinds] = {-1, -1, -1, -1, -1};
+  auto note = [&](int kind, int idx) {
+    if (rec[kind] < 0) rec[kind] = idx;
+  };
+  auto wk = [&](int kind) { return work + kind * kWorkCounters; };
   int32_t* lvl_counts = g->Scratch<int32_t>("bc.level_counts", kBucketLevels);
   int32_t* lvl_cursor = g->Scratch<int32_t>("bc.level_cursor", kBucketLevels);
   std::vector<int32_t> h_counts(kBucketLevels);
@@ -559,7 +604,8 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   const int32_t* fold_dst = UploadOrderLists(g);
   BeginTiming(g);
   Run(g, "bc_zero", N * sizeof(T) * 2, BcZero<T>, grid_n, blk, score, comp, n);
-  launches++;
+  Run(g, "bc_zero", 0, WorkClear, dim3(1), dim3(256), work, kWKinds * kWorkCounters);
+  launches += 2;
   std::vector<int32_t> bounds;
   for (int r = 0; r < k; r++) {
     Run(g, "bc_reset_source", 4 * N, BcResetSource, grid_n, blk, depth, n, roots[r], sigma, queue, dc);
@@ -591,15 +637,16 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
       launches += 2;
       for (int32_t d = 1; d <= last; d++) {
         const int32_t nb = bounds[d], nn = bounds[d + 1] - bounds[d];
-        Run(g, "bc_sigma", 20.0 * nn, BcSigma, dim3(GridFor(nn, kBS, g->cu_count * 16)), blk,
-            static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
-            static_cast<const int32_t*>(depth), sigma, static_cast<const int32_t*>(queue), nb, nn, d - 1, rchunks,
-            dc);
+        note(kWSigma, Run(g, "bc_sigma", 20.0 * nn, BcSigma, dim3(GridFor(nn, kBS, g->cu_count * 16)), blk,
+                          static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
+                          static_cast<const int32_t*>(depth), sigma, static_cast<const int32_t*>(queue), nb, nn,
+                          d - 1, rchunks, dc, wk(kWSigma)));
         launches++;
         if (hubs) {
-          Run(g, "bc_sigma_chunks", 0, BcSigmaChunks, grid_c, blk, static_cast<const int32_t*>(g->in_dst),
-              static_cast<const int32_t*>(depth), sigma, d - 1, static_cast<const EdgeChunk*>(rchunks),
-              static_cast<const BcCounters*>(dc));
+          note(kWSigmaChunks,
+               Run(g, "bc_sigma_chunks", 0, BcSigmaChunks, grid_c, blk, static_cast<const int32_t*>(g->in_dst),
+                   static_cast<const int32_t*>(depth), sigma, d - 1, static_cast<const EdgeChunk*>(rchunks),
+                   static_cast<const BcCounters*>(dc), wk(kWSigmaChunks)));
           Run(g, "bc_clear_reduce", 0, BcClearReduce, dim3(1), blk, dc);
           launches += 2;
         }
@@ -635,15 +682,16 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
       if (tail == bounds.back()) break;
       const int32_t nb = bounds.back(), nn = tail - nb;
       bounds.push_back(tail);
-      Run(g, "bc_sigma", 20.0 * nn, BcSigma, dim3(GridFor(nn, kBS, g->cu_count * 16)), blk,
-          static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
-          static_cast<const int32_t*>(depth), sigma, static_cast<const int32_t*>(queue), nb, nn, level,
-          rchunks, dc);
+      note(kWSigma, Run(g, "bc_sigma", 20.0 * nn, BcSigma, dim3(GridFor(nn, kBS, g->cu_count * 16)), blk,
+                        static_cast<const int64_t*>(g->in_off), static_cast<const int32_t*>(g->in_dst),
+                        static_cast<const int32_t*>(depth), sigma, static_cast<const int32_t*>(queue), nb, nn,
+                        level, rchunks, dc, wk(kWSigma)));
       launches++;
       if (hubs) {
-        Run(g, "bc_sigma_chunks", 0, BcSigmaChunks, grid_c, blk, static_cast<const int32_t*>(g->in_dst),
-            static_cast<const int32_t*>(depth), sigma, level, static_cast<const EdgeChunk*>(rchunks),
-            static_cast<const BcCounters*>(dc));
+        note(kWSigmaChunks,
+             Run(g, "bc_sigma_chunks", 0, BcSigmaChunks, grid_c, blk, static_cast<const int32_t*>(g->in_dst),
+                 static_cast<const int32_t*>(depth), sigma, level, static_cast<const EdgeChunk*>(rchunks),
+                 static_cast<const BcCounters*>(dc), wk(kWSigmaChunks)));
         Run(g, "bc_clear_reduce", 0, BcClearReduce, dim3(1), blk, dc);
         launches += 2;
       }
@@ -654,11 +702,11 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
     // Backward: deepest level first; delta of level d reads level d + 1.
     for (int32_t d = level; d >= 0; d--) {
       const int32_t qb = bounds[d], qn = bounds[d + 1] - bounds[d];
-      Run(g, "bc_delta", (16.0 + 2 * sizeof(T) + 8.0) * qn, BcDelta<T, kKahan>,
-          dim3(GridFor(qn, kBS, g->cu_count * 16)), blk, static_cast<const int64_t*>(g->out_off),
-          fold_dst, static_cast<const int32_t*>(depth),
-          static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(queue), qb, qn, d,
-          wave_list, dc);
+      note(kWDelta, Run(g, "bc_delta", (16.0 + 2 * sizeof(T) + 8.0) * qn, BcDelta<T, kKahan>,
+                        dim3(GridFor(qn, kBS, g->cu_count * 16)), blk, static_cast<const int64_t*>(g->out_off),
+                        fold_dst, static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta,
+                        score, comp, static_cast<const int32_t*>(queue), qb, qn, d, wave_list, dc,
+                        wk(kWDelta)));
       launches++;
       if (g->max_degree >= kWaveEdges) {
         // Hubs fold on the side stream beside the wavefront kernel (disjoint
@@ -669,17 +717,19 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
           EnsureSideStream(g);
           GG_HIP(hipEventRecord(g->ev_fork, st));
           GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
-          RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count), dim3(kBcHubBS),
-                static_cast<const int64_t*>(g->out_off), fold_dst,
-                static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
-                static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
+          note(kWDeltaHub,
+               RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count), dim3(kBcHubBS),
+                     static_cast<const int64_t*>(g->out_off), fold_dst, static_cast<const int32_t*>(depth),
+                     static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(wave_list),
+                     d, static_cast<const BcCounters*>(dc), wk(kWDeltaHub)));
           GG_HIP(hipEventRecord(g->ev_join, g->side));
           launches++;
         }
-        Run(g, "bc_delta_wave", 0, BcDeltaWave<T, kKahan>, dim3(GridFor(qn, kBS / kWave, g->cu_count * 8)),
-            blk, static_cast<const int64_t*>(g->out_off), fold_dst,
-            static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta, score, comp,
-            static_cast<const int32_t*>(wave_list), d, static_cast<const BcCounters*>(dc));
+        note(kWDeltaWave,
+             Run(g, "bc_delta_wave", 0, BcDeltaWave<T, kKahan>, dim3(GridFor(qn, kBS / kWave, g->cu_count * 8)),
+                 blk, static_cast<const int64_t*>(g->out_off), fold_dst, static_cast<const int32_t*>(depth),
+                 static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(wave_list), d,
+                 static_cast<const BcCounters*>(dc), wk(kWDeltaWave)));
         if (hubs) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
         Run(g, "bc_clear_reduce", 0, BcClearReduce, dim3(1), blk, dc);
         launches += 2;
@@ -693,6 +743,16 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
       static_cast<const T*>(comp), n, static_cast<const unsigned long long*>(max_bits), out32);
   launches += 2;
   GG_HIP(hipGetLastError());
+  if (g->profile) {
+    // Bytes beyond the per-vertex parts credited at launch: per list entry
+    // its id and the neighbour's depth (8 B); per matched in-neighbour its
+    // sigma (8 B); per successor its sigma and delta (8 + sizeof(T) B).
+    const std::vector<unsigned long long> w = ReadWork(g, work, kWKinds);
+    for (int kind = 0; kind < kWKinds; kind++) {
+      const double per_match = kind <= kWSigmaChunks ? 8.0 : 8.0 + sizeof(T);
+      CreditBytes(g, rec[kind], 8.0 * w[2 * kind] + per_match * w[2 * kind + 1]);
+    }
+  }
   FinishTiming(g, stats, wt);
   if (scores_out) CopyOutVertexF32(g, out32, scores_out);
   if (stats) {

@@ -268,6 +268,20 @@ inline unsigned long long MailNext(gg_graph* g) { return ++g->mail_seq; }
 // Waits until the mailbox holds `seq`; copies values[0 .. k) out.
 void MailWait(gg_graph* g, unsigned long long seq, unsigned long long* values, int k);
 
+// Exact work of a kernel kind, counted for the byte credits of a profiled
+// run: kWorkStripes striped counters of list entries read, then as many of
+// matched entries (the entries whose neighbour the kernel went on to read).
+constexpr int kWorkStripes = 64;
+constexpr int kWorkCounters = 2 * kWorkStripes;
+
+namespace {
+__global__ __attribute__((unused)) void WorkClear(unsigned long long* work, int count) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) work[i] = 0;
+}
+}  // namespace
+// Sums the counters of `kinds` consecutive kinds: {entries, matched} each.
+std::vector<unsigned long long> ReadWork(gg_graph* g, const unsigned long long* work, int kinds);
+
 inline unsigned GridFor(int64_t items, int block, unsigned cap = 1u << 20) {
   int64_t b = (items + block - 1) / block;
   if (b < 1) b = 1;
@@ -332,6 +346,16 @@ __device__ __forceinline__ int BlockExclusiveScan(int v, int* total, int* sh) {
   __syncthreads();
   *total = tot;
   return wave_base + inc - v;
+}
+
+// Adds a block's totals to its stripe; call from every thread with values
+// valid in thread 0 (e.g. after BlockSum).
+__device__ __forceinline__ void WorkAdd(unsigned long long* work, unsigned long long entries,
+                                        unsigned long long matched) {
+  if (threadIdx.x != 0) return;
+  const int s = blockIdx.x & (kWorkStripes - 1);
+  if (entries) atomicAdd(&work[s], entries);
+  if (matched) atomicAdd(&work[kWorkStripes + s], matched);
 }
 
 // Writes values to the host mailbox, then the sequence number, with system

@@ -151,6 +151,21 @@ int ProfileSlot(gg_graph* g, const char* name, double bytes, hipEvent_t* a, hipE
   return static_cast<int>(g->pending.size()) - 1;
 }
 
+std::vector<unsigned long long> ReadWork(gg_graph* g, const unsigned long long* work, int kinds) {
+  std::vector<unsigned long long> h(static_cast<size_t>(kinds) * kWorkCounters);
+  GG_HIP(hipMemcpyAsync(h.data(), work, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, g->stream));
+  GG_HIP(hipStreamSynchronize(g->stream));
+  if (g->side) GG_HIP(hipStreamSynchronize(g->side));
+  if (g->side2) GG_HIP(hipStreamSynchronize(g->side2));
+  std::vector<unsigned long long> out(2 * kinds, 0);
+  for (int k = 0; k < kinds; k++)
+    for (int s = 0; s < kWorkStripes; s++) {
+      out[2 * k] += h[k * kWorkCounters + s];
+      out[2 * k + 1] += h[k * kWorkCounters + kWorkStripes + s];
+    }
+  return out;
+}
+
 void CreditBytes(gg_graph* g, int idx, double bytes) {
   if (idx >= 0) g->pending[idx].bytes += bytes;
 }

@@ -34,7 +34,6 @@ namespace gg {
 namespace {
 
 constexpr int kMbBS = 256;
-constexpr int kMbStripes = 64;
 constexpr int kSigmaLoads = 8;   // sigma rows in flight per wavefront step
 constexpr int kDeltaLoads = 8;   // (sigma, delta) row pairs in flight per wavefront step
 // Dependency folds of vertices with at least kHubFold successors run in
@@ -80,12 +79,12 @@ __device__ __forceinline__ void PopBits(uint64_t& any, int (&j)[B]) {
 __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
                                                  const uint64_t* __restrict__ cur, const uint64_t* __restrict__ prev,
                                                  double* __restrict__ sigma, int64_t n,
-                                                 unsigned long long* examined) {
+                                                 unsigned long long* work) {
   __shared__ unsigned long long s_red[kMbBS / kWave];
   const int lane = LaneId();
   const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
-  unsigned long long ex = 0;
+  unsigned long long ex = 0, pairs = 0;
   for (int64_t base = wave; base < n; base += nwaves * kWave) {
     const int64_t vl = base + lane * nwaves;
     uint64_t fl = 0;
@@ -119,6 +118,7 @@ __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_
           ul = in_dst[k];
           ml = prev[ul] & fv;
         }
+        pairs += static_cast<unsigned long long>(__popcll(ml));
         uint64_t any = __ballot(ml != 0);
         while (any) {
           int j[kSigmaLoads];
@@ -142,7 +142,8 @@ __global__ __launch_bounds__(kMbBS) void MbSigma(const int64_t* __restrict__ in_
     }
   }
   ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
-  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+  pairs = BlockSum<kMbBS>(pairs, s_red);
+  WorkAdd(work, ex, pairs);
 }
 
 // Sigma of heavy vertices: one wavefront per chunk of in-edges, partial
@@ -151,12 +152,12 @@ __global__ __launch_bounds__(kMbBS) void MbSigmaHeavy(const int32_t* __restrict_
                                                       const uint64_t* __restrict__ cur,
                                                       const uint64_t* __restrict__ prev, double* sigma,
                                                       const EdgeChunk* __restrict__ chunks, int32_t nch,
-                                                      unsigned long long* examined) {
+                                                      unsigned long long* work) {
   __shared__ unsigned long long s_red[kMbBS / kWave];
   const int lane = LaneId();
   const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
-  unsigned long long ex = 0;
+  unsigned long long ex = 0, pairs = 0;
   for (int64_t it = wave; it < nch; it += nwaves) {
     const EdgeChunk ch = chunks[it];
     const uint64_t fv = cur[ch.u];
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(kMbBS) void MbSigmaHeavy(const int32_t* __restrict_
         ul = in_dst[k];
         ml = prev[ul] & fv;
       }
+      pairs += static_cast<unsigned long long>(__popcll(ml));
       uint64_t any = __ballot(ml != 0);
       while (any) {
         int j[kSigmaLoads];
@@ -192,7 +194,8 @@ __global__ __launch_bounds__(kMbBS) void MbSigmaHeavy(const int32_t* __restrict_
     if (acc != 0.0) unsafeAtomicAdd(&sigma[int64_t(ch.u) * 64 + lane], acc);
   }
   ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
-  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+  pairs = BlockSum<kMbBS>(pairs, s_red);
+  WorkAdd(work, ex, pairs);
 }
 
 // The reference's dependency term and accumulation (float totals take each
@@ -217,7 +220,7 @@ __device__ __forceinline__ double Fold(double acc, double t) { return __dadd_rn(
 template <class T>
 __device__ __forceinline__ int64_t DeltaVertex(const int32_t* __restrict__ dst, const uint64_t* __restrict__ next,
                                                const double* __restrict__ sigma, T* __restrict__ delta, int64_t u,
-                                               uint64_t fu, int64_t b, int64_t e) {
+                                               uint64_t fu, int64_t b, int64_t e, unsigned long long& pairs) {
   const int lane = LaneId();
   const bool mine = (fu >> lane) & 1;
   const double su = mine ? sigma[u * 64 + lane] : 1.0;
@@ -231,6 +234,7 @@ __device__ __forceinline__ int64_t DeltaVertex(const int32_t* __restrict__ dst, 
   for (int64_t cb = b; cb < e; cb += kWave) {
     const int32_t vl = vn;
     const uint64_t ml = mn;
+    pairs += static_cast<unsigned long long>(__popcll(ml));
     const int64_t kn = cb + kWave + lane;
     vn = 0;
     mn = 0;
@@ -279,12 +283,12 @@ template <class T>
 __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                  const uint64_t* __restrict__ cur, const uint64_t* __restrict__ next,
                                                  const double* __restrict__ sigma, T* __restrict__ delta, int64_t n,
-                                                 unsigned long long* examined) {
+                                                 unsigned long long* work) {
   __shared__ unsigned long long s_red[kMbBS / kWave];
   const int lane = LaneId();
   const int64_t wave = (int64_t(blockIdx.x) * kMbBS + threadIdx.x) / kWave;
   const int64_t nwaves = int64_t(gridDim.x) * (kMbBS / kWave);
-  unsigned long long ex = 0;
+  unsigned long long ex = 0, pairs = 0;
   for (int64_t base = wave; base < n; base += nwaves * kWave) {
     const int64_t ul = base + lane * nwaves;
     uint64_t fl = 0;
@@ -303,11 +307,13 @@ __global__ __launch_bounds__(kMbBS) void MbDelta(const int64_t* __restrict__ off
       const int i = __builtin_ctzll(act);
       act &= act - 1;
       ex += static_cast<unsigned long long>(
-          DeltaVertex<T>(dst, next, sigma, delta, base + i * nwaves, __shfl(fl, i), __shfl(bl, i), __shfl(el, i)));
+          DeltaVertex<T>(dst, next, sigma, delta, base + i * nwaves, __shfl(fl, i), __shfl(bl, i), __shfl(el, i),
+                         pairs));
     }
   }
   ex = BlockSum<kMbBS>(lane == 0 ? ex : 0ull, s_red);
-  if (threadIdx.x == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+  pairs = BlockSum<kMbBS>(pairs, s_red);
+  WorkAdd(work, ex, pairs);
 }
 
 // Dependencies of the hubs (>= kHubFold successors) at level L, one block
@@ -330,11 +336,11 @@ __global__ __launch_bounds__((P + 1) * kWave, P <= kHubSmall ? 7 : 1) void MbDel
                                                      const uint64_t* __restrict__ next,
                                                      const double* __restrict__ sigma, T* __restrict__ delta,
                                                      const int32_t* __restrict__ hubs, int32_t nhubs,
-                                                     unsigned long long* examined) {
+                                                     unsigned long long* work) {
   constexpr int kHubRound = P * kHubEdges;
   __shared__ double terms[2][kHubRound * kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
-  unsigned long long ex = 0;
+  unsigned long long ex = 0, pairs = 0;
   for (int32_t h = blockIdx.x; h < nhubs; h += gridDim.x) {
     const int64_t u = hubs[h];
     const uint64_t fu = cur[u];
@@ -375,6 +381,7 @@ __global__ __launch_bounds__((P + 1) * kWave, P <= kHubSmall ? 7 : 1) void MbDel
           dv[j] = delta[row];
           on |= static_cast<uint32_t>(o) << j;
         }
+        pairs += static_cast<unsigned long long>(__popc(on));
         double* out = terms[k & 1] + wid * kHubEdges * kWave;
 #pragma unroll
         for (int j = 0; j < kHubEdges; j++) {
@@ -417,7 +424,12 @@ __global__ __launch_bounds__((P + 1) * kWave, P <= kHubSmall ? 7 : 1) void MbDel
       if (lane == 0) ex += static_cast<unsigned long long>(e - b);
     }
   }
-  if (wid == P && lane == 0 && ex) atomicAdd(&examined[blockIdx.x & (kMbStripes - 1)], ex);
+  const int s = blockIdx.x & (kWorkStripes - 1);
+  if (wid == P && lane == 0 && ex) atomicAdd(&work[s], ex);
+  if (wid < P) {
+    pairs = WaveSum(pairs);
+    if (lane == 0 && pairs) atomicAdd(&work[kWorkStripes + s], pairs);
+  }
 }
 
 // Vertices with at least kHubFold out-edges, split at kHubWide (once per
@@ -499,8 +511,17 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
   T* delta = g->Scratch<T>(tag, n * 64);
   T* score = g->Scratch<T>(sizeof(T) == 4 ? "mb.score32" : "mb.score64", n);
   float* out = g->Scratch<float>("mb.out", n);
-  // [0, 64): sigma examined, [64, 128): delta examined, [128]: max bits.
-  unsigned long long* ctr = g->Scratch<unsigned long long>("mb.ctr", 2 * kMbStripes + 1);
+  // Work counters per kernel kind (list entries read, matched (entry, root)
+  // pairs), then the max-score bits.
+  enum { kWSigma, kWSigmaHeavy, kWDelta, kWHub, kWHubWide, kWKinds };
+  constexpr int kCtr = kWKinds * kWorkCounters + 1;
+  unsigned long long* ctr = g->Scratch<unsigned long long>("mb.ctr", kCtr);
+  unsigned long long* max_bits = ctr + kWKinds * kWorkCounters;
+  auto wk = [&](int kind) { return ctr + kind * kWorkCounters; };
+  int rec[kWKinds] = {-1, -1, -1, -1, -1};
+  auto note = [&](int kind, int idx) {
+    if (rec[kind] < 0) rec[kind] = idx;
+  };
   const double N = static_cast<double>(n);
 
   // Dependencies are folded in each vertex's uploaded list order (the
@@ -510,18 +531,20 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
   MsTraversal t = MsTraverse(g, droots, nullptr, k, k == 64 ? ~0ull : (1ull << k) - 1, 15, nullptr, true);
   const int levels = static_cast<int>(t.frontiers.size());
   int64_t launches = t.launches;
-  Run(g, "msbc_clear", 0, MbClear, dim3(1), blk, ctr, 2 * kMbStripes + 1);
+  Run(g, "msbc_clear", 0, MbClear, dim3(1), blk, ctr, kCtr);
   Run(g, "msbc_root_sigma", 0, MbRootSigma, dim3(1), dim3(64), droots, k, sigma);
   launches += 2;
   const int32_t nhchunks = static_cast<int32_t>(g->ms_heavy_chunks);
   const EdgeChunk* hchunks = g->Scratch<EdgeChunk>("ms.hchunks", 1);
   for (int L = 1; L < levels; L++) {
-    Run(g, "msbc_sigma", 16.0 * N, MbSigma, grid_w, blk, static_cast<const int64_t*>(g->in_off),
-        static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
-        static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, n, ctr);
-    Run(g, "msbc_sigma_heavy", 0, MbSigmaHeavy, dim3(GridFor(int64_t(nhchunks) * kWave, kMbBS, g->cu_count * 16)),
-        blk, static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
-        static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, hchunks, nhchunks, ctr);
+    note(kWSigma, Run(g, "msbc_sigma", 16.0 * N, MbSigma, grid_w, blk, static_cast<const int64_t*>(g->in_off),
+                      static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
+                      static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, n, wk(kWSigma)));
+    note(kWSigmaHeavy,
+         Run(g, "msbc_sigma_heavy", 0, MbSigmaHeavy,
+             dim3(GridFor(int64_t(nhchunks) * kWave, kMbBS, g->cu_count * 16)), blk,
+             static_cast<const int32_t*>(g->in_dst), static_cast<const uint64_t*>(t.frontiers[L]),
+             static_cast<const uint64_t*>(t.frontiers[L - 1]), sigma, hchunks, nhchunks, wk(kWSigmaHeavy)));
     launches += 2;
   }
   // Hubs (their folds split across a block, MbDeltaHub): listed once per
@@ -550,42 +573,53 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
     if (nhubs + nwide > 0) GG_HIP(hipEventRecord(g->ev_fork, st));
     if (nwide > 0) {
       GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
-      RunOn(g, g->side, "msbc_delta_hub_wide", 0, MbDeltaHub<T, kHubLarge>, dim3(GridFor(nwide, 1, g->cu_count)),
-            dim3((kHubLarge + 1) * kWave), static_cast<const int64_t*>(g->out_off),
-            fold_dst, static_cast<const uint64_t*>(t.frontiers[L]), next,
-            static_cast<const double*>(sigma), delta, static_cast<const int32_t*>(wide), nwide, ctr + kMbStripes);
+      note(kWHubWide, RunOn(g, g->side, "msbc_delta_hub_wide", 0, MbDeltaHub<T, kHubLarge>,
+                            dim3(GridFor(nwide, 1, g->cu_count)), dim3((kHubLarge + 1) * kWave),
+                            static_cast<const int64_t*>(g->out_off), fold_dst,
+                            static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma),
+                            delta, static_cast<const int32_t*>(wide), nwide, wk(kWHubWide)));
       GG_HIP(hipEventRecord(g->ev_join, g->side));
       launches++;
     }
     if (nhubs > 0) {
       GG_HIP(hipStreamWaitEvent(g->side2, g->ev_fork, 0));
-      RunOn(g, g->side2, "msbc_delta_hub", 0, MbDeltaHub<T, kHubSmall>,
-            dim3(GridFor(nhubs, 1, g->cu_count * kHubSmallBlocksPerCu)), dim3((kHubSmall + 1) * kWave),
-            static_cast<const int64_t*>(g->out_off), fold_dst,
-            static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
-            static_cast<const int32_t*>(hubs), nhubs, ctr + kMbStripes);
+      note(kWHub, RunOn(g, g->side2, "msbc_delta_hub", 0, MbDeltaHub<T, kHubSmall>,
+                        dim3(GridFor(nhubs, 1, g->cu_count * kHubSmallBlocksPerCu)), dim3((kHubSmall + 1) * kWave),
+                        static_cast<const int64_t*>(g->out_off), fold_dst,
+                        static_cast<const uint64_t*>(t.frontiers[L]), next, static_cast<const double*>(sigma), delta,
+                        static_cast<const int32_t*>(hubs), nhubs, wk(kWHub)));
       GG_HIP(hipEventRecord(g->ev_join2, g->side2));
       launches++;
     }
-    Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
-        fold_dst, static_cast<const uint64_t*>(t.frontiers[L]), next,
-        static_cast<const double*>(sigma), delta, n, ctr + kMbStripes);
+    note(kWDelta, Run(g, "msbc_delta", 16.0 * N, MbDelta<T>, grid_w, blk, static_cast<const int64_t*>(g->out_off),
+                      fold_dst, static_cast<const uint64_t*>(t.frontiers[L]), next,
+                      static_cast<const double*>(sigma), delta, n, wk(kWDelta)));
     launches++;
     if (nwide > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join, 0));
     if (nhubs > 0) GG_HIP(hipStreamWaitEvent(st, g->ev_join2, 0));
   }
   Run(g, "msbc_scores", 8.0 * N + sizeof(T) * N, MbScores<T>, grid_n, blk, static_cast<const uint64_t*>(t.seen),
       static_cast<const T*>(delta), n, score);
-  Run(g, "msbc_max", sizeof(T) * N, MbMax<T>, grid_n, blk, static_cast<const T*>(score), n, ctr + 2 * kMbStripes);
+  Run(g, "msbc_max", sizeof(T) * N, MbMax<T>, grid_n, blk, static_cast<const T*>(score), n, max_bits);
   Run(g, "msbc_normalize", (sizeof(T) + 4.0) * N, MbNormalize<T>, grid_n, blk, static_cast<const T*>(score), n,
-      static_cast<const unsigned long long*>(ctr + 2 * kMbStripes), out);
+      static_cast<const unsigned long long*>(max_bits), out);
   launches += 3;
   GG_HIP(hipGetLastError());
+  if (g->profile) {
+    // Bytes beyond the 16 B per vertex credited at launch: per list entry
+    // its id and the neighbour's level mask (12 B); per matched (entry,
+    // root) pair the neighbour's sigma (8 B) when pulling sigma, its sigma
+    // and delta (8 + sizeof(T) B) when folding dependencies.
+    const std::vector<unsigned long long> w = ReadWork(g, ctr, kWKinds);
+    for (int kind = 0; kind < kWKinds; kind++) {
+      const double per_pair = kind <= kWSigmaHeavy ? 8.0 : 8.0 + sizeof(T);
+      CreditBytes(g, rec[kind], 12.0 * w[2 * kind] + per_pair * w[2 * kind + 1]);
+    }
+  }
   FinishTiming(g, stats, wt);
   if (scores) CopyOutVertexF32(g, out, scores);
   if (stats) {
-    unsigned long long h[2 * kMbStripes];
-    GG_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    const std::vector<unsigned long long> w = ReadWork(g, ctr, kWKinds);
     unsigned long long* sums = g->Scratch<unsigned long long>("ms.reach", 2);
     MsZero2<<<1, 64, 0, st>>>(sums);
     MsReach<<<grid_n, dim3(kBS), 0, st>>>(t.seen, g->out_off, n, sums);
@@ -593,7 +627,7 @@ void RunBatchBrandes(gg_graph* g, const int32_t* droots, int k, float* scores, g
     GG_HIP(hipMemcpyAsync(reach, sums, sizeof(reach), hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
     unsigned long long ex = 0;
-    for (int i = 0; i < 2 * kMbStripes; i++) ex += h[i];
+    for (int kind = 0; kind < kWKinds; kind++) ex += w[2 * kind];
     stats->total_ms = wt.ms();
     stats->iterations = levels;
     stats->td_steps = t.td;

@@ -296,9 +296,10 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
     Run(g, "tc_items", 20.0 * N + 16.0 * nitems, TcItemFill, dim3(spans), blk,
         static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(fdeg), n,
         static_cast<const unsigned long long*>(span_items), items);
-    // Per item: its header and partner headers; per probe one 4-byte list
-    // entry (credited below from the probe counter).
-    const int idx = Run(g, "tc_count", 16.0 * nitems, TcCount, dim3(GridFor(nitems, 1, g->cu_count * kCountBlocksPerCu)), blk,
+    // Compulsory bytes: one pass over the CSR, 8 (n + 1) + 4 m.  The merge
+    // work (probes, stats->edges_examined) is reported apart: its re-reads
+    // of lists and LDS probes are not compulsory traffic.
+    Run(g, "tc_count", 8.0 * (N + 1) + 4.0 * static_cast<double>(g->out_entries), TcCount, dim3(GridFor(nitems, 1, g->cu_count * kCountBlocksPerCu)), blk,
                         static_cast<const int64_t*>(g->out_off), static_cast<const int32_t*>(g->out_dst),
                         static_cast<const int32_t*>(fdeg), static_cast<const TcItem*>(items), nitems, counts);
     launches += 2;
@@ -310,9 +311,6 @@ extern "C" int gg_tc(gg_graph* g, uint64_t* count, gg_stats* stats) {
       total += h[2 * i];
       probes += h[2 * i + 1];
     }
-    // The partners' headers (12 B) and the staged entries (4 B) of each
-    // (vertex, partner) once per side: 2 x 16 B per forward edge.
-    CreditBytes(g, idx, 4.0 * static_cast<double>(probes) + 16.0 * static_cast<double>(g->out_entries));
     FinishTiming(g, stats, wt);
     *count = total;
     if (stats) {

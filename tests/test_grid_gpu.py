@@ -1,9 +1,10 @@
 """The grid workload at full size: a 4096 x 4096 grid (16,777,216
 vertices), 4-neighbour edges with 5% deleted and 1% random shortcuts added
 (the shortcuts bring the BFS depth down to about 56 levels),
-weights uniform in [1, 1000], seed 7 (Graph.grid).  BFS and SSSP from
-SourcePicker roots, single-root and batched; every checked root against the
-serial oracles (bfs_depth restating src/bfs.cc:204-215, Dijkstra restating
+weights uniform in [1, 1000], seed 7 (Graph.grid).  BFS and SSSP from the
+32 SourcePicker roots, single-root and batched: every root certified exact
+(oracle.distance_certificate, linear in the edges), some against the serial
+oracles (bfs_depth restating src/bfs.cc:204-215, Dijkstra restating
 src/sssp.cc:156-181), and SSSP distances identical across bucket widths.
 Widths below 64 are covered on the 96 x 96 grid in test_sssp_gpu.py: here
 they would take hundreds of thousands of buckets.
@@ -39,6 +40,10 @@ def test_grid_shape():
 
 
 def test_grid_bfs_single_and_batched():
+    """All 32 roots: batched depths certified exact (oracle.distance_certificate:
+    depth 0 only at the root, no edge spans two levels, every other reached
+    vertex has a neighbour one level up), equal to the single-root depths,
+    and four roots against the serial BFS of the oracle."""
     wg, g, roots = grid()
     dg = gpu.DeviceGraph(g)
     print("[grid] batched BFS of 32 roots", flush=True)
@@ -47,11 +52,38 @@ def test_grid_bfs_single_and_batched():
     for i in (0, 1, 2, 3):
         want = oracle.bfs_depth(g, roots[i])
         assert np.array_equal(batch.depths[i], want), f"root {roots[i]}"
-    for i in (4, 5, 31):
+    for i in range(32):
+        assert oracle.distance_certificate(g, roots[i], batch.depths[i]) == 0, f"root {roots[i]}"
         single = gpu.bfs(dg, roots[i])
         assert np.array_equal(batch.depths[i], single.depth), f"root {roots[i]}"
-        assert g.verify_bfs(roots[i], single.parent)
+        if i in (4, 5, 31):
+            assert g.verify_bfs(roots[i], single.parent)
     assert batch.stats["reached"] == sum(int((batch.depths[i] >= 0).sum()) for i in range(32))
+
+
+def test_grid_sssp_all_roots_single_and_batched():
+    """All 32 roots, single-root and batched, each root's distances certified
+    exact by the shortest-path certificate (positive weights, dist[root] = 0,
+    no relaxable edge, a tight in-edge for every other reached vertex), the
+    batch equal to the single-root runs, two roots against Dijkstra."""
+    wg, g, roots = grid()
+    if "dwg" not in _G:
+        _G["dwg"] = gpu.DeviceGraph(wg)
+    dwg = _G["dwg"]
+    for delta in (1000, 4096):
+        print(f"[grid] batched SSSP of 32 roots, delta {delta}", flush=True)
+        batch = gpu.sssp_batch(dwg, roots, delta)
+        print(f"[grid] batched: {batch.stats['iterations']} buckets, {batch.stats['td_steps']} relax rounds, "
+              f"{batch.stats['device_ms']:.1f} ms", flush=True)
+        for i in range(32):
+            assert oracle.distance_certificate(wg, roots[i], batch.dist[i]) == 0, f"root {roots[i]}"
+            if delta == 1000:
+                single = gpu.sssp(dwg, roots[i], delta)
+                assert np.array_equal(batch.dist[i], single.dist), f"root {roots[i]}"
+    if "dist" not in _G:
+        _G["dist"] = {r: oracle.dijkstra(wg, r) for r in roots[:2]}
+    for r in roots[:2]:
+        assert np.array_equal(batch.dist[roots.index(r)], _G["dist"][r])
 
 
 @pytest.mark.parametrize("delta", [64, 1000, 4096])

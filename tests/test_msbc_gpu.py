@@ -136,12 +136,9 @@ def test_batch_bc_hub_folds(ordered):
     g = Graph.from_edges(np.concatenate(u), np.concatenate(v))
     dg = gpu.DeviceGraph(g, order_by_degree=ordered)
     roots = np.concatenate([hubs, rng.choice(np.arange(4, n), 28, replace=False)]).astype(np.int32)
-    if not ordered:
-        r = check(g, dg, roots)
-    else:
-        new_id = dg.vertex_order()
-        want = oracle.brandes(g.relabel_by_degree(), new_id[roots])[new_id]
-        assert np.array_equal(gpu.bc_batch(dg, roots).scores, want)
+    # Degree-ordered or not, the folds read each vertex's list in uploaded
+    # order, so the scores equal the oracle's float Brandes bit for bit.
+    check(g, dg, roots)
     dg.profile(True)
     gpu.bc_batch(dg, roots)
     rep = dg.profile_report()
