@@ -21,10 +21,40 @@
 // progress guarantee.  The final link expands vertices over the grid with
 // gg_expand.h.
 #include <algorithm>
+#include <mutex>
 #include <random>
 
 #include "gg_common.h"
 #include "gg_expand.h"
+
+// Fixed device scratch of gg_cc_noalloc, for callers that cannot allocate
+// device memory.  Each GPU the library runs on holds its own copy, at its
+// own address, from gg_init until the process ends.  In host code the name
+// `scratch` is only a host-side stand-in: kernels get the device address
+// that NoAllocSetup looked up (gg_noalloc_ptr[device]).
+static constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* gg_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+namespace gg {
+
+void NoAllocSetup(int device) {
+  GG_REQUIRE(device >= 0 && device < kMaxDevices, GG_ENOTSUP, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (gg_noalloc_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  GG_REQUIRE(e == hipSuccess && ptr != nullptr, GG_EHIP,
+             std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  gg_noalloc_ptr[device] = ptr;
+}
+
+}  // namespace gg
 
 namespace gg {
 namespace {
@@ -309,6 +339,79 @@ extern "C" int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_sta
     const int64_t launches = RunAfforest(g, neighbor_rounds, b, h_samples);
     FinishTiming(g, stats, wt);
     if (comp_out) CopyOutComponents(g, b.comp, comp_out);
+    if (stats) {
+      unsigned long long h[64];
+      GG_HIP(hipMemcpy(h, b.examined, sizeof(h), hipMemcpyDeviceToHost));
+      unsigned long long ex = 0;
+      for (unsigned long long x : h) ex += x;
+      stats->total_ms = wt.ms();
+      stats->iterations = neighbor_rounds + 1;
+      stats->edges_examined = static_cast<int64_t>(ex);
+      stats->launches = launches;
+    }
+  });
+}
+
+// Allocation-free Afforest: gg_cc's kernels with every device buffer carved
+// out of the fixed `scratch` array of the current GPU (no hipMalloc, no
+// hipFree, no pinned or library allocation).  Layout, each part 256-byte
+// aligned: comp[n], the sample positions and labels, the striped counters,
+// the chunk count, the hub chunk list, and for a degree-ordered handle the
+// two per-vertex arrays of the label mapping.
+extern "C" int gg_cc_noalloc(gg_graph* g, int neighbor_rounds, int32_t* comp_out, gg_stats* stats) {
+  return Guard([&] {
+    WallTimer wt;
+    StatsReset(stats);
+    CcCheck(g, neighbor_rounds);
+    GG_HIP(hipSetDevice(g->device));
+    void* base = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_noalloc_mu);
+      if (g->device >= 0 && g->device < kMaxDevices) base = gg_noalloc_ptr[g->device];
+    }
+    GG_REQUIRE(base != nullptr, GG_ENODEV, "gg_init has not been called for this graph's device");
+    const int64_t n = g->n;
+    const bool use_chunks = g->max_degree >= kChunkEdges;
+    size_t used = 0;
+    auto carve = [&](size_t bytes) {
+      const size_t at = used;
+      used += (bytes + 255) & ~size_t(255);
+      return at;
+    };
+    const size_t at_comp = carve(sizeof(int32_t) * n);
+    const size_t at_idx = carve(sizeof(int32_t) * kSamples);
+    const size_t at_val = carve(sizeof(int32_t) * kSamples);
+    const size_t at_ex = carve(sizeof(unsigned long long) * 64);
+    const size_t at_nc = carve(sizeof(int32_t) * 2);
+    const size_t at_chunks = use_chunks ? carve(sizeof(EdgeChunk) * MaxChunks(n, CcEntries(g))) : 0;
+    const size_t at_least = g->new_id ? carve(sizeof(int32_t) * n) : 0;
+    const size_t at_out = g->new_id ? carve(sizeof(int32_t) * n) : 0;
+    GG_REQUIRE(used <= kNoAllocBytes, GG_EINVAL,
+               "graph needs " + std::to_string(used) + " bytes of scratch; gg_cc_noalloc holds " +
+                   std::to_string(kNoAllocBytes) + " (use gg_cc)");
+    unsigned char* p = static_cast<unsigned char*>(base);
+    CcBuffers b;
+    b.comp = reinterpret_cast<int32_t*>(p + at_comp);
+    b.sample_idx = reinterpret_cast<int32_t*>(p + at_idx);
+    b.sample_val = reinterpret_cast<int32_t*>(p + at_val);
+    b.examined = reinterpret_cast<unsigned long long*>(p + at_ex);
+    b.nchunks = reinterpret_cast<int32_t*>(p + at_nc);
+    b.chunks = use_chunks ? reinterpret_cast<EdgeChunk*>(p + at_chunks) : nullptr;
+    // The scratch keeps whatever the last call on this GPU left, so the
+    // sample positions are written every call (RunAfforest clears the rest).
+    int32_t idx[kSamples], h_samples[kSamples];
+    SamplePositions(n, idx);
+    GG_HIP(hipMemcpyAsync(b.sample_idx, idx, sizeof(idx), hipMemcpyHostToDevice, g->stream));
+    BeginTiming(g);
+    const int64_t launches = RunAfforest(g, neighbor_rounds, b, h_samples);
+    FinishTiming(g, stats, wt);
+    if (comp_out) {
+      if (g->new_id)
+        CopyOutComponentsWith(g, b.comp, comp_out, reinterpret_cast<int32_t*>(p + at_least),
+                              reinterpret_cast<int32_t*>(p + at_out));
+      else
+        CopyOutComponentsWith(g, b.comp, comp_out, nullptr, nullptr);
+    }
     if (stats) {
       unsigned long long h[64];
       GG_HIP(hipMemcpy(h, b.examined, sizeof(h), hipMemcpyDeviceToHost));

@@ -190,6 +190,9 @@ void CopyOutComponentsWith(gg_graph* g, const int32_t* comp, int32_t* host, int3
 // uploaded id).  Kernels whose result depends on the order in which a row
 // is summed read these.  Synchronous on first use; call outside timing.
 const int32_t* UploadOrderLists(gg_graph* g);
+// Looks up, once per device, the device address of gg_cc_noalloc's fixed
+// scratch array on `device` (the current device); called by gg_init.
+void NoAllocSetup(int device);
 
 // Direction-optimizing BFS (gg_bfs.hip) from device id `source` on
 // g->stream, inside the caller's timed region: parents in the workspace

@@ -304,6 +304,7 @@ int gg_init(int device) {
     hipDeviceProp_t prop;
     GG_HIP(hipGetDeviceProperties(&prop, device));
     GG_REQUIRE(prop.warpSize == 64, GG_ENOTSUP, "kernels assume 64-wide wavefronts");
+    NoAllocSetup(device);
     g_initialised = true;
   });
 }

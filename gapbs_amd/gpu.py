@@ -108,6 +108,7 @@ def lib() -> C.CDLL:
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
+        L.gg_cc_noalloc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_cc_sv.argtypes = [vp, _i32p, C.POINTER(Stats)]
         L.gg_bc.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
         L.gg_bc_batch.argtypes = [vp, _i32p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(Stats)]
@@ -424,6 +425,16 @@ def cc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True, out=No
     st = Stats()
     _check(lib().gg_cc(dg.handle, neighbor_rounds, _p(comp, C.c_int32) if want_comp else None,
                        C.byref(st)))
+    return CCResult(comp, st.as_dict())
+
+
+def cc_noalloc(dg: DeviceGraph, neighbor_rounds: int = 2, want_comp: bool = True) -> CCResult:
+    """gg_cc_noalloc: Afforest with every device buffer in the library's
+    fixed scratch array (no device allocation); labels equal gg_cc's."""
+    comp = np.empty(dg.num_nodes, dtype=np.int32) if want_comp else None
+    st = Stats()
+    _check(lib().gg_cc_noalloc(dg.handle, neighbor_rounds, _p(comp, C.c_int32) if want_comp else None,
+                               C.byref(st)))
     return CCResult(comp, st.as_dict())
 
 

@@ -146,6 +146,19 @@ int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* sco
  * vertex id in the component, exactly the reference's labels. */
 int gg_cc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
 
+/* Allocation-free Afforest.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * result as gg_cc (labels bit-identical), but every device buffer comes from
+ * a fixed 64 MiB array that each GPU holds from gg_init until the process
+ * ends: the call makes no hipMalloc, hipFree, pinned or library allocation.
+ * The graph needs 4 n bytes (8 n more when degree-ordered) plus 24 bytes per
+ * 4096-entry chunk of its hubs' lists and a few KB to fit; a larger graph is
+ * GG_EINVAL, with the bytes it needs in gg_last_error() (use gg_cc).  All
+ * calls on one GPU share the array and a call leaves its contents behind:
+ * calls that overlap on one GPU (two host threads, two handles) race. */
+int gg_cc_noalloc(gg_graph* g, int neighbor_rounds, int32_t* comp, gg_stats* stats);
+
 /* Shiloach-Vishkin connected components, the reference's second CC binary.
  * Replaces ShiloachVishkin(const Graph&) (src/cc_sv.cc:49-83): hook every
  * out-edge's higher root under the lower label (min-max swap, so directed
