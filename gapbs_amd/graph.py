@@ -37,7 +37,9 @@ class HostError(RuntimeError):
 def lib() -> C.CDLL:
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libgaphost.so")
+        # GAPBS_CPU_LIB_SUFFIX=.asan loads the AddressSanitizer build
+        # (make -C gapbs_amd sanitize), for the sanitized CPU test run.
+        path = os.path.join(_HERE, "libgaphost" + os.environ.get("GAPBS_CPU_LIB_SUFFIX", "") + ".so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
         L = C.CDLL(path)

@@ -36,7 +36,7 @@ class BFSStep(C.Structure):
 def lib() -> C.CDLL:
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.path.join(_HERE, "liboracle" + os.environ.get("GAPBS_CPU_LIB_SUFFIX", "") + ".so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run `make -C oracle`")
         L = C.CDLL(path)
