@@ -107,9 +107,25 @@ struct gg_graph {
   int32_t* fold_dst = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
-  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}};  // gg_autotune results
+  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}};  // gg_autotune results
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
+  // Jacobi PageRank source-segment layout, one per precision (gg_pr.hip):
+  // the in-entries whose source lies below `hot`, regrouped by source
+  // segment of 2^sb vertices into (destination, sub-list) pairs; the device
+  // arrays live in the workspace ("pr.seg32.*" / "pr.seg64.*").
+  struct PrSegments {
+    bool built = false;
+    int sb = 0;                              // log2 segment width
+    int32_t small_max = 0, medium_max = 0;   // pair bin cut-offs it was built with
+    int ns = 0;                              // segments in use (0: plain pull)
+    int64_t hot = 0;                         // sources below this go by segment
+    int64_t pairs = 0, hot_entries = 0;
+    std::vector<int64_t> first_pair;         // ns + 1: segment s owns pairs [first_pair[s], first_pair[s + 1])
+    std::vector<int64_t> large, medium;      // its pairs in the block and wavefront bins (listed first)
+    std::vector<int64_t> entries;            // its in-entries
+    double build_ms = 0;
+  } pr_seg[2];
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for
   int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
   int64_t ms_heavy_chunks = 0;           // and their in-edge chunks

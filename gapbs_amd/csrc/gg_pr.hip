@@ -10,16 +10,33 @@
 //
 // GPU design: vertices are binned once per graph by in-degree -- below 32
 // one thread per vertex, below 1024 one wavefront, else one block -- and an
-// iteration is one gather launch per bin.  Each gather writes the vertex's
-// new score and, for the next iteration, its new contribution into a second
-// buffer, so the reference's separate contrib pass is fused away.  Two
-// precisions:
+// iteration is one gather launch over the three bins.  Each gather writes the
+// vertex's new score and, for the next iteration, its new contribution into a
+// second buffer, so the reference's separate contrib pass is fused away.
+//
+// Source segments (Jacobi modes).  A pull gather reads contrib[v] for every
+// in-entry at random: on a large graph most of those reads miss L2 and move a
+// whole cache line for 4 or 8 useful bytes.  With sorted in-lists, the
+// entries of u whose source lies in [s * 2^sb, (s + 1) * 2^sb) form one
+// contiguous run, a (destination, sub-list) pair of segment s.  Built once
+// per graph and precision: the pairs of the leading segments that are dense
+// enough (at least kSegMinEntriesPerPair entries per pair), grouped by
+// segment, with their sub-lists copied into one segment-major array.  An
+// iteration then runs one launch per segment, which reads only that
+// segment's 2^sb contributions (2 MB by default, so they stay in every
+// XCD's L2) and adds each pair's partial sum into the destination's
+// accumulator, then the usual three-bin gather over the rest of every list
+// (sources >= hot), which starts from the accumulator and finishes the
+// vertex.  On a degree-ordered Kronecker graph the hubs sit at the low ids,
+// so a few segments hold most of the entries (scale 26: 16 segments of 2^19
+// sources hold 95% of them in 1.8 pairs per vertex).  Two precisions:
 //   GG_PR_FP32  float scores and contributions, each vertex's sum kept as a
 //               compensated (Neumaier) pair and merged across lanes with
 //               TwoSum, so the sum carries about twice float's precision;
 //   GG_PR_FP64  double scores, contributions and sums.
 // The convergence residual is always accumulated in double.
 #include "gg_common.h"
+#include "gg_sort.h"
 
 namespace gg {
 namespace {
@@ -56,16 +73,34 @@ struct PlainSum {
   __device__ __forceinline__ double Value() const { return s; }
 };
 
+// Accumulator type of each precision, and its form in memory (the
+// per-vertex partial sums the segment launches build up).
 template <class T>
 struct Acc;
 template <>
 struct Acc<float> {
   using Type = CompSum;
+  using Stored = float2;
 };
 template <>
 struct Acc<double> {
   using Type = PlainSum;
+  using Stored = double;
 };
+
+__device__ __forceinline__ CompSum Unpack(float2 v) { return CompSum{v.x, v.y}; }
+__device__ __forceinline__ PlainSum Unpack(double v) { return PlainSum{v}; }
+__device__ __forceinline__ float2 Pack(const CompSum& a) { return make_float2(a.s, a.c); }
+__device__ __forceinline__ double Pack(const PlainSum& a) { return a.s; }
+
+// Reads vertex u's accumulated partial sum and clears it for the next
+// iteration.
+template <class S>
+__device__ __forceinline__ auto TakeAcc(S* acc, int32_t u) {
+  const S v = acc[u];
+  acc[u] = S{};
+  return Unpack(v);
+}
 
 template <class A>
 __device__ __forceinline__ A ShflXor(const A& a, int o);
@@ -109,23 +144,44 @@ struct PrArgs {
   T base, damp;
   double* residual;  // striped
   int32_t lo, hi;    // vertex ids updated by this launch: [lo, hi)
+  // With source segments: each list is gathered from in_off[u] + tail[u]
+  // (the entries the segment launches did not take) and its sum starts from
+  // acc[u], which is cleared.  Both null: whole lists, sums from zero.
+  const int32_t* tail;
+  typename Acc<T>::Stored* acc;
 };
 
-// Adds contrib[in_dst[k]] for k = k0, k0 + step, ... < e to acc, with
-// four independent gathers in flight per lane.
+// The gather's starting point and initial sum for vertex u.
+template <class T>
+__device__ __forceinline__ int64_t ListBegin(const PrArgs<T>& a, int32_t u) {
+  return a.in_off[u] + (a.tail ? a.tail[u] : 0);
+}
+template <class T>
+__device__ __forceinline__ typename Acc<T>::Type StartSum(const PrArgs<T>& a, int32_t u, bool take) {
+  using A = typename Acc<T>::Type;
+  return a.acc && take ? TakeAcc(a.acc, u) : A::Zero();
+}
+
+// Adds contrib[ids[k]] for k = k0, k0 + step, ... < e to acc, with four
+// independent gathers in flight per lane.
 template <class T, class A>
-__device__ __forceinline__ void GatherStrided(const PrArgs<T>& a, A& acc, int64_t k0, int64_t e, int step) {
+__device__ __forceinline__ void GatherIds(const T* __restrict__ contrib, const int32_t* __restrict__ ids, A& acc,
+                                          int64_t k0, int64_t e, int step) {
   int64_t k = k0;
   for (; k + 3 * step < e; k += 4 * step) {
-    const int32_t v0 = a.in_dst[k], v1 = a.in_dst[k + step], v2 = a.in_dst[k + 2 * step],
-                  v3 = a.in_dst[k + 3 * step];
-    const T x0 = a.contrib_in[v0], x1 = a.contrib_in[v1], x2 = a.contrib_in[v2], x3 = a.contrib_in[v3];
+    const int32_t v0 = ids[k], v1 = ids[k + step], v2 = ids[k + 2 * step], v3 = ids[k + 3 * step];
+    const T x0 = contrib[v0], x1 = contrib[v1], x2 = contrib[v2], x3 = contrib[v3];
     acc.Add(x0);
     acc.Add(x1);
     acc.Add(x2);
     acc.Add(x3);
   }
-  for (; k < e; k += step) acc.Add(a.contrib_in[a.in_dst[k]]);
+  for (; k < e; k += step) acc.Add(contrib[ids[k]]);
+}
+
+template <class T, class A>
+__device__ __forceinline__ void GatherStrided(const PrArgs<T>& a, A& acc, int64_t k0, int64_t e, int step) {
+  GatherIds(a.contrib_in, a.in_dst, acc, k0, e, step);
 }
 
 // Finishes vertex u: new score, next contribution, |change|.
@@ -156,18 +212,9 @@ __device__ __forceinline__ double GatherThreadPart(const PrArgs<T>& a, const int
   for (int64_t i = blk * kBS + threadIdx.x; i < nv; i += nblk * kBS) {
     const int32_t u = verts[i];
     if (u < a.lo || u >= a.hi) continue;
-    const int64_t b = a.in_off[u], e = a.in_off[u + 1];
-    A acc = A::Zero();
-    int64_t k = b;
-    for (; k + 4 <= e; k += 4) {  // four independent gathers in flight
-      const int32_t v0 = a.in_dst[k], v1 = a.in_dst[k + 1], v2 = a.in_dst[k + 2], v3 = a.in_dst[k + 3];
-      const T x0 = a.contrib_in[v0], x1 = a.contrib_in[v1], x2 = a.contrib_in[v2], x3 = a.contrib_in[v3];
-      acc.Add(x0);
-      acc.Add(x1);
-      acc.Add(x2);
-      acc.Add(x3);
-    }
-    for (; k < e; k++) acc.Add(a.contrib_in[a.in_dst[k]]);
+    const int64_t b = ListBegin(a, u), e = a.in_off[u + 1];
+    A acc = StartSum(a, u, true);
+    GatherStrided(a, acc, b, e, 1);
     r += Finish(a, u, static_cast<T>(acc.Value()));
   }
   return r;
@@ -184,8 +231,8 @@ __device__ __forceinline__ double GatherWavePart(const PrArgs<T>& a, const int32
   for (int64_t i = wave; i < nv; i += nwaves) {
     const int32_t u = verts[i];
     if (u < a.lo || u >= a.hi) continue;  // uniform per wavefront
-    const int64_t b = a.in_off[u], e = a.in_off[u + 1];
-    A acc = A::Zero();
+    const int64_t b = ListBegin(a, u), e = a.in_off[u + 1];
+    A acc = StartSum(a, u, lane == 0);
     GatherStrided(a, acc, b + lane, e, kWave);
     acc = WaveMerge(acc);
     if (lane == 0) r += Finish(a, u, static_cast<T>(acc.Value()));
@@ -203,8 +250,8 @@ __device__ __forceinline__ double GatherBlockPart(const PrArgs<T>& a, const int3
   for (int64_t i = blk; i < nv; i += nblk) {
     const int32_t u = verts[i];
     if (u < a.lo || u >= a.hi) continue;  // uniform per block
-    const int64_t b = a.in_off[u], e = a.in_off[u + 1];
-    A acc = A::Zero();
+    const int64_t b = ListBegin(a, u), e = a.in_off[u + 1];
+    A acc = StartSum(a, u, threadIdx.x == 0);
     GatherStrided(a, acc, b + threadIdx.x, e, kBS);
     acc = WaveMerge(acc);
     if (lane == 0) s_acc[wid] = acc;
@@ -239,12 +286,238 @@ __global__ __launch_bounds__(kBS) void PrGather(PrArgs<T> a, const int32_t* __re
 
 template <class T>
 __global__ __launch_bounds__(kBS) void PrInit(const int64_t* __restrict__ out_off, T* __restrict__ score,
-                                              T* __restrict__ contrib, int64_t n, T init) {
+                                              T* __restrict__ contrib, typename Acc<T>::Stored* __restrict__ acc,
+                                              int64_t n, T init) {
   for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBS) {
     score[i] = init;
     contrib[i] = init / static_cast<T>(out_off[i + 1] - out_off[i]);
+    if (acc) acc[i] = typename Acc<T>::Stored{};
   }
 }
+
+// ---------------------------------------------------------------------------
+// Source segments.
+
+// One segment launch: every pair of segment s adds the sum of its sources'
+// contributions to its destination's accumulator.  A destination has at
+// most one pair per segment, so the read-modify-write of acc needs no
+// atomics; launches of different segments are ordered by the stream.
+template <class T>
+struct SegArgs {
+  const T* contrib;
+  const int32_t* src;                // sub-lists, segment-major
+  const int32_t* dst;                // destination of each pair
+  const int64_t* off;                // pair p's sources: src[off[p], off[p + 1])
+  typename Acc<T>::Stored* acc;
+  int64_t p0;                        // the segment's first pair
+  int32_t n_large, n_medium, n_small;  // its pairs per bin, stored in this order
+  int32_t nb_large, nb_medium;       // blocks serving the block and wavefront bins
+};
+
+template <class T>
+__device__ __forceinline__ void AddToAcc(typename Acc<T>::Stored* acc, int32_t v, const typename Acc<T>::Type& part) {
+  auto t = Unpack(acc[v]);
+  t.Merge(part);
+  acc[v] = Pack(t);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBS) void PrSegment(SegArgs<T> a) {
+  using A = typename Acc<T>::Type;
+  __shared__ A s_acc[kBS / kWave];
+  const int64_t b = blockIdx.x;
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  if (b < a.nb_large) {  // one block per pair
+    for (int64_t i = b; i < a.n_large; i += a.nb_large) {
+      const int64_t p = a.p0 + i;
+      A acc = A::Zero();
+      GatherIds(a.contrib, a.src, acc, a.off[p] + threadIdx.x, a.off[p + 1], kBS);
+      acc = WaveMerge(acc);
+      if (lane == 0) s_acc[wid] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        A t = s_acc[0];
+        for (int w = 1; w < kBS / kWave; w++) t.Merge(s_acc[w]);
+        AddToAcc<T>(a.acc, a.dst[p], t);
+      }
+      __syncthreads();
+    }
+  } else if (b < a.nb_large + a.nb_medium) {  // one wavefront per pair
+    const int64_t nw = int64_t(a.nb_medium) * (kBS / kWave);
+    for (int64_t i = (b - a.nb_large) * (kBS / kWave) + wid; i < a.n_medium; i += nw) {
+      const int64_t p = a.p0 + a.n_large + i;
+      A acc = A::Zero();
+      GatherIds(a.contrib, a.src, acc, a.off[p] + lane, a.off[p + 1], kWave);
+      acc = WaveMerge(acc);
+      if (lane == 0) AddToAcc<T>(a.acc, a.dst[p], acc);
+    }
+  } else {  // one thread per pair
+    const int64_t nb = int64_t(gridDim.x) - a.nb_large - a.nb_medium;
+    for (int64_t i = (b - a.nb_large - a.nb_medium) * kBS + threadIdx.x; i < a.n_small; i += nb * kBS) {
+      const int64_t p = a.p0 + a.n_large + a.n_medium + i;
+      A acc = A::Zero();
+      GatherIds(a.contrib, a.src, acc, a.off[p], a.off[p + 1], 1);
+      AddToAcc<T>(a.acc, a.dst[p], acc);
+    }
+  }
+}
+
+// Building the layout (once per graph and precision, outside timing).
+constexpr int kSegMax = 64;                // candidate segments examined
+constexpr int kSegMinEntriesPerPair = 4;   // density a default-width segment needs
+constexpr int kCopyTile = 4096;            // output entries per block of the sub-list copy
+
+// End of the run of sources below x that starts at b (a sorted list ending
+// at e): a short linear probe first, since most runs are short.
+__device__ __forceinline__ int64_t RunEnd(const int32_t* __restrict__ dst, int64_t b, int64_t e, int64_t x) {
+  const int64_t probe_end = b + 8 < e ? b + 8 : e;
+  for (; b < probe_end; b++)
+    if (dst[b] >= x) return b;
+  while (b < e) {
+    const int64_t m = b + ((e - b) >> 1);
+    if (dst[m] < x)
+      b = m + 1;
+    else
+      e = m;
+  }
+  return b;
+}
+
+// Flags a list that is not sorted, or too long for a 32-bit tail offset
+// (the layout then stays off).  One wavefront per vertex.
+__global__ __launch_bounds__(kBS) void PlCheck(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                               int64_t n, int32_t* bad) {
+  const int lane = LaneId();
+  const int64_t nw = int64_t(gridDim.x) * (kBS / kWave);
+  for (int64_t v = (int64_t(blockIdx.x) * kBS + threadIdx.x) / kWave; v < n; v += nw) {
+    const int64_t b = in_off[v], e = in_off[v + 1];
+    bool wrong = e - b > INT32_MAX;
+    for (int64_t k = b + lane; !wrong && k + 1 < e; k += kWave) wrong = in_dst[k] > in_dst[k + 1];
+    if (__ballot(wrong) != 0 && lane == 0) atomicOr(bad, 1);
+  }
+}
+
+struct PlTotals {
+  unsigned long long pairs[kSegMax][3];  // per segment and bin (block, wavefront, thread)
+  unsigned long long entries[kSegMax];
+  int32_t bad;
+};
+
+// Pass 1: for each tile of kBS vertices, the pairs it holds per segment and
+// bin (stored (segment, bin)-major, tile-minor, so one scan turns them into
+// output positions), and the totals per segment.
+__global__ __launch_bounds__(kBS) void PlCount(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                               int64_t n, int sb, int nseg, int32_t small_max, int32_t medium_max,
+                                               int64_t tiles, uint32_t* __restrict__ counts, PlTotals* tot) {
+  __shared__ uint32_t s_cnt[kSegMax * 3];
+  __shared__ unsigned long long s_ent[kSegMax];
+  for (int i = threadIdx.x; i < kSegMax * 3; i += kBS) s_cnt[i] = 0;
+  for (int i = threadIdx.x; i < kSegMax; i += kBS) s_ent[i] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t v = tile * kBS + threadIdx.x;
+  if (v < n) {
+    const int64_t hot = int64_t(nseg) << sb;
+    const int64_t end = in_off[v + 1];
+    int64_t pos = in_off[v];
+    while (pos < end) {
+      const int64_t x = in_dst[pos];
+      if (x >= hot) break;
+      const int s = static_cast<int>(x >> sb);
+      const int64_t nxt = RunEnd(in_dst, pos + 1, end, int64_t(s + 1) << sb);
+      const int64_t len = nxt - pos;
+      const int bin = len >= medium_max ? 0 : (len >= small_max ? 1 : 2);
+      atomicAdd(&s_cnt[s * 3 + bin], 1u);
+      atomicAdd(&s_ent[s], static_cast<unsigned long long>(len));
+      pos = nxt;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nseg * 3; i += kBS) {
+    counts[int64_t(i) * tiles + tile] = s_cnt[i];
+    if (s_cnt[i]) atomicAdd(&tot->pairs[i / 3][i % 3], static_cast<unsigned long long>(s_cnt[i]));
+  }
+  for (int s = threadIdx.x; s < nseg; s += kBS)
+    if (s_ent[s]) atomicAdd(&tot->entries[s], s_ent[s]);
+}
+
+// Pass 2: the same walk over the first ns segments, writing each pair's
+// destination, length and list position at its place (segment, then bin,
+// then tile, then vertex order), and each vertex's tail offset.
+__global__ __launch_bounds__(kBS) void PlEmit(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                              int64_t n, int sb, int ns, int32_t small_max, int32_t medium_max,
+                                              int64_t tiles, const uint32_t* __restrict__ base,
+                                              int32_t* __restrict__ pair_dst, int64_t* __restrict__ pair_len,
+                                              int64_t* __restrict__ pair_pos, int32_t* __restrict__ tail) {
+  __shared__ int sh[kBS / kWave];
+  const int64_t tile = blockIdx.x;
+  const int64_t v = tile * kBS + threadIdx.x;
+  const int64_t hot = int64_t(ns) << sb;
+  int64_t beg = 0, pos = 0, end = 0;
+  int cur = ns;  // segment of the next run, ns once none is left
+  if (v < n) {
+    beg = pos = in_off[v];
+    end = in_off[v + 1];
+    if (pos < end && in_dst[pos] < hot) cur = static_cast<int>(in_dst[pos] >> sb);
+  }
+  for (int s = 0; s < ns; s++) {
+    const bool has = cur == s;
+    int64_t nxt = pos;
+    int bin = 0;
+    if (has) {
+      nxt = RunEnd(in_dst, pos + 1, end, int64_t(s + 1) << sb);
+      const int64_t len = nxt - pos;
+      bin = len >= medium_max ? 0 : (len >= small_max ? 1 : 2);
+    }
+    int total;  // per-bin ranks in 10-bit fields (at most kBS per field)
+    const int excl = BlockExclusiveScan<kBS>(has ? 1 << (10 * bin) : 0, &total, sh);
+    if (has) {
+      const int64_t p = int64_t(base[int64_t(s * 3 + bin) * tiles + tile]) + ((excl >> (10 * bin)) & 1023);
+      pair_dst[p] = static_cast<int32_t>(v);
+      pair_len[p] = nxt - pos;
+      pair_pos[p] = pos;
+      pos = nxt;
+      cur = pos < end && in_dst[pos] < hot ? static_cast<int>(in_dst[pos] >> sb) : ns;
+    }
+  }
+  if (v < n) tail[v] = static_cast<int32_t>(pos - beg);
+}
+
+// First index in [lo, hi) with a[index] > x.
+__device__ __forceinline__ int64_t UpperBound(const int64_t* __restrict__ a, int64_t lo, int64_t hi, int64_t x) {
+  while (lo < hi) {
+    const int64_t m = lo + ((hi - lo) >> 1);
+    if (a[m] <= x)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  return lo;
+}
+
+// Pass 3: copies every pair's sub-list to its place in the segment-major
+// array, kCopyTile outputs per block (owner pair by binary search).
+__global__ __launch_bounds__(kBS) void PlCopy(const int64_t* __restrict__ pair_off, const int64_t* __restrict__ pair_pos,
+                                              int64_t pairs, const int32_t* __restrict__ in_dst,
+                                              int32_t* __restrict__ src, int64_t total) {
+  __shared__ int64_t s_lo, s_hi;
+  for (int64_t b0 = int64_t(blockIdx.x) * kCopyTile; b0 < total; b0 += int64_t(gridDim.x) * kCopyTile) {
+    const int64_t b1 = b0 + kCopyTile < total ? b0 + kCopyTile : total;
+    if (threadIdx.x == 0) {
+      s_lo = UpperBound(pair_off, 0, pairs + 1, b0) - 1;
+      s_hi = UpperBound(pair_off, 0, pairs + 1, b1 - 1);
+    }
+    __syncthreads();
+    const int64_t lo = s_lo, hi = s_hi;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kBS) {
+      const int64_t p = UpperBound(pair_off, lo, hi, i) - 1;
+      src[i] = in_dst[pair_pos[p] + (i - pair_off[p])];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void PlSetLast(int64_t* pair_off, int64_t pairs, int64_t total) { pair_off[pairs] = total; }
 
 __global__ __launch_bounds__(kBS) void PrResetResidual(double* residual) {
   if (threadIdx.x < kStripes) residual[threadIdx.x] = 0.0;
@@ -331,6 +604,103 @@ PrBins GetBins(gg_graph* g) {
   return b;
 }
 
+// The source-segment layout of precision T, built on first use (and again
+// when the width or the bin cut-offs change); null when the gather runs
+// without segments.  Default width: 2 MB of contributions per segment
+// (2^19 float or 2^18 double sources), used only when the contributions
+// span more than two segments and only for the leading segments with at
+// least kSegMinEntriesPerPair entries per pair; an explicit width
+// (gg_tuning.pr_segment_log2) takes every candidate segment.  Lists must be
+// sorted (they are for builder output and degree-ordered handles).
+template <class T>
+const gg_graph::PrSegments* Segments(gg_graph* g) {
+  const int prec = sizeof(T) == 4 ? 0 : 1;
+  const int32_t knob = g->tune.pr_segment_log2[prec];
+  if (knob < 0) return nullptr;
+  gg_graph::PrSegments& L = g->pr_seg[prec];
+  const int sb = knob > 0 ? knob : (sizeof(T) == 4 ? 19 : 18);
+  if (L.built && L.sb == sb && L.small_max == g->tune.pr_small_max && L.medium_max == g->tune.pr_medium_max)
+    return L.ns > 0 ? &L : nullptr;
+  const std::string tag = sizeof(T) == 4 ? "pr.seg32." : "pr.seg64.";
+  g->ws.Drop(tag);
+  WallTimer wt;
+  L = gg_graph::PrSegments{};
+  L.built = true;
+  L.sb = sb;
+  L.small_max = g->tune.pr_small_max;
+  L.medium_max = g->tune.pr_medium_max;
+  const int64_t n = g->n, width = int64_t(1) << sb;
+  if (knob == 0 && n <= 2 * width) return nullptr;
+  const bool prof = g->profile;
+  g->profile = false;  // the build is not part of any measured call
+  struct Restore {
+    gg_graph* g;
+    bool prof;
+    ~Restore() { g->profile = prof; }
+  } restore{g, prof};
+  hipStream_t st = g->stream;
+  const int nseg = static_cast<int>(std::min<int64_t>(kSegMax, (n + width - 1) / width));
+  const int64_t tiles = (n + kBS - 1) / kBS;
+  PlTotals* dtot = g->Scratch<PlTotals>("pr.segb.tot", 1);
+  uint32_t* counts = g->Scratch<uint32_t>("pr.segb.counts", static_cast<size_t>(nseg) * 3 * tiles);
+  GG_HIP(hipMemsetAsync(dtot, 0, sizeof(PlTotals), st));
+  PlCheck<<<GridFor(n, kBS / kWave, g->cu_count * 16), kBS, 0, st>>>(g->in_off, g->in_dst, n, &dtot->bad);
+  PlCount<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, sb, nseg, L.small_max,
+                                                        L.medium_max, tiles, counts, dtot);
+  GG_HIP(hipGetLastError());
+  PlTotals tot;
+  GG_HIP(hipMemcpyAsync(&tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  int ns = 0;
+  int64_t pairs = 0, hot_entries = 0;
+  if (!tot.bad) {
+    for (; ns < nseg; ns++) {
+      const unsigned long long p = tot.pairs[ns][0] + tot.pairs[ns][1] + tot.pairs[ns][2];
+      if (knob == 0 && tot.entries[ns] < kSegMinEntriesPerPair * p) break;
+      pairs += static_cast<int64_t>(p);
+      hot_entries += static_cast<int64_t>(tot.entries[ns]);
+    }
+  }
+  if (pairs >= (int64_t(1) << 31)) ns = 0;  // pair counts and positions are 32-bit
+  if (ns == 0 || pairs == 0) {
+    g->ws.Drop("pr.segb.");
+    L.ns = 0;
+    return nullptr;
+  }
+  L.ns = ns;
+  L.hot = std::min<int64_t>(int64_t(ns) << sb, n);
+  L.pairs = pairs;
+  L.hot_entries = hot_entries;
+  L.first_pair.assign(ns + 1, 0);
+  L.large.resize(ns);
+  L.medium.resize(ns);
+  L.entries.resize(ns);
+  for (int s = 0; s < ns; s++) {
+    L.large[s] = static_cast<int64_t>(tot.pairs[s][0]);
+    L.medium[s] = static_cast<int64_t>(tot.pairs[s][1]);
+    L.entries[s] = static_cast<int64_t>(tot.entries[s]);
+    L.first_pair[s + 1] = L.first_pair[s] + L.large[s] + L.medium[s] + static_cast<int64_t>(tot.pairs[s][2]);
+  }
+  // Output positions of each (segment, bin, tile) run of pairs.
+  DeviceScan<uint32_t>(g, "pr.segb.scan", counts, int64_t(ns) * 3 * tiles, nullptr);
+  int32_t* pair_dst = g->Scratch<int32_t>((tag + "dst").c_str(), pairs);
+  int64_t* pair_off = g->Scratch<int64_t>((tag + "off").c_str(), pairs + 1);
+  int32_t* tail = g->Scratch<int32_t>((tag + "tail").c_str(), n);
+  int32_t* src = g->Scratch<int32_t>((tag + "src").c_str(), hot_entries);
+  int64_t* pair_pos = g->Scratch<int64_t>("pr.segb.pos", pairs);
+  PlEmit<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, sb, ns, L.small_max, L.medium_max,
+                                                       tiles, counts, pair_dst, pair_off, pair_pos, tail);
+  DeviceScan<int64_t>(g, "pr.segb.scan64", pair_off, pairs, nullptr);
+  PlSetLast<<<1, 1, 0, st>>>(pair_off, pairs, hot_entries);
+  PlCopy<<<GridFor(hot_entries, kCopyTile, g->cu_count * 16), kBS, 0, st>>>(pair_off, pair_pos, pairs, g->in_dst, src,
+                                                                           hot_entries);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(st));
+  g->ws.Drop("pr.segb.");
+  L.build_ms = wt.ms();
+  return &L;
+}
+
 // in_place: one contribution array, each vertex's contribution replaced as
 // soon as its score is (PageRankPullGS, pr.cc:34-61), and each sweep runs as
 // kGsPasses launches over consecutive vertex-id ranges, so later ranges
@@ -341,8 +711,16 @@ PrBins GetBins(gg_graph* g) {
 template <class T>
 void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, float* scores_out, gg_stats* stats,
                  const WallTimer& wt) {
+  using Stored = typename Acc<T>::Stored;
   const int64_t n = g->n;
   const PrBins bins = GetBins(g);
+  const gg_graph::PrSegments* seg = in_place ? nullptr : Segments<T>(g);
+  const std::string seg_tag = sizeof(T) == 4 ? "pr.seg32." : "pr.seg64.";
+  const int32_t* seg_src = seg ? g->Scratch<int32_t>((seg_tag + "src").c_str(), seg->hot_entries) : nullptr;
+  const int32_t* seg_dst = seg ? g->Scratch<int32_t>((seg_tag + "dst").c_str(), seg->pairs) : nullptr;
+  const int64_t* seg_off = seg ? g->Scratch<int64_t>((seg_tag + "off").c_str(), seg->pairs + 1) : nullptr;
+  const int32_t* seg_tail = seg ? g->Scratch<int32_t>((seg_tag + "tail").c_str(), n) : nullptr;
+  Stored* acc = seg ? g->Scratch<Stored>(sizeof(T) == 4 ? "pr.acc32" : "pr.acc64", n) : nullptr;
   const int width = g->tune.pr_blocks_per_cu;  // gather blocks per CU
   T* score = g->Scratch<T>(sizeof(T) == 4 ? "pr.score32" : "pr.score64", n);
   T* ca = g->Scratch<T>(sizeof(T) == 4 ? "pr.contrib32a" : "pr.contrib64a", n);
@@ -362,20 +740,37 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
   BeginTiming(g);
   Run(g, "pr_reset", 0, PrResetResidual, dim3(1), blk, residual);
   Run(g, "pr_init", 8 * (N + 1) + 2 * sizeof(T) * N, PrInit<T>, dim3(GridFor(n, kBS, g->cu_count * 32)),
-      blk, static_cast<const int64_t*>(g->out_off), score, ca, n, init);
+      blk, static_cast<const int64_t*>(g->out_off), score, ca, acc, n, init);
   launches += 2;
   int iters = 0;
   double error = 0.0;
   for (int it = 0; it < max_iters; it++) {
-    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual, 0, 0};
+    PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual, 0, 0,
+                seg_tail, acc};
     const int passes = in_place ? kGsPasses : 1;
     // Algorithmic bytes of one iteration: offsets 8 B and contrib/score
     // traffic per vertex, 4 B per in-neighbour id (each array counted once;
     // the random contribution gathers are not).  A Gauss-Seidel pass is
-    // credited its share.  One launch per pass: the three bins' blocks run
-    // side by side.
+    // credited its share; with segments, each segment launch is credited
+    // its entries' ids and the finishing gather the rest.  One gather launch
+    // per pass: the three bins' blocks run side by side.
     const double per_vertex = 8.0 + vbytes, share = 1.0 / passes;
-    const double bytes = per_vertex * n + 4.0 * (bins.e_small + bins.e_medium + bins.e_large);
+    double bytes = per_vertex * n + 4.0 * (bins.e_small + bins.e_medium + bins.e_large);
+    if (seg) {
+      for (int s = 0; s < seg->ns; s++) {
+        const int64_t p0 = seg->first_pair[s];
+        const int64_t nl = seg->large[s], nm = seg->medium[s], nsm = seg->first_pair[s + 1] - p0 - nl - nm;
+        if (nl + nm + nsm == 0) continue;
+        const int32_t gl = static_cast<int32_t>(nl ? GridFor(nl, 1, g->cu_count * (width + 1) / 2) : 0);
+        const int32_t gm = static_cast<int32_t>(nm ? GridFor(nm, kBS / kWave, g->cu_count * width) : 0);
+        const int32_t gs = static_cast<int32_t>(nsm ? GridFor(nsm, kBS, g->cu_count * width) : 0);
+        SegArgs<T> sa{ca, seg_src, seg_dst, seg_off, acc, p0, static_cast<int32_t>(nl), static_cast<int32_t>(nm),
+                      static_cast<int32_t>(nsm), gl, gm};
+        Run(g, "pr_segment", 4.0 * seg->entries[s], PrSegment<T>, dim3(gl + gm + gs), blk, sa);
+        launches++;
+      }
+      bytes -= 4.0 * seg->hot_entries;
+    }
     const int32_t nb_large =
         static_cast<int32_t>(bins.n_large ? GridFor(bins.n_large, 1, g->cu_count * (width + 1) / 2) : 0);
     const int32_t nb_medium =

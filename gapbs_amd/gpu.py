@@ -61,11 +61,13 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("bfs_alpha", C.c_int32), ("bfs_beta", C.c_int32), ("batch_alpha", C.c_int32),
                 ("sssp_delta", C.c_int32), ("pr_small_max", C.c_int32), ("pr_medium_max", C.c_int32),
-                ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4)]
+                ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4),
+                ("pr_segment_log2", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "best_ms"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("best_ms", "pr_segment_log2")}
         d["best_ms"] = list(self.best_ms)
+        d["pr_segment_log2"] = list(self.pr_segment_log2)
         return d
 
 
@@ -539,6 +541,9 @@ def set_tuning(dg: DeviceGraph, values: dict) -> None:
     _check(lib().gg_tuning_get(dg.handle, C.byref(t)))
     for k, v in values.items():
         if k == "best_ms":
+            continue
+        if k == "pr_segment_log2":
+            t.pr_segment_log2[0], t.pr_segment_log2[1] = v
             continue
         setattr(t, k, v)
     _check(lib().gg_tuning_set(dg.handle, C.byref(t)))

@@ -299,6 +299,11 @@ typedef struct gg_tuning {
   int32_t pr_blocks_per_cu;  /* gather launch width */
   int32_t tuned;             /* GG_TUNE_* bits tuned so far */
   double best_ms[4];         /* device ms of the chosen setting: bfs, batch, sssp, pr */
+  /* Jacobi PageRank source segments, [0] FP32 and [1] FP64: 0 picks the
+   * default width (2 MB of contributions per segment), -1 turns the
+   * segmented gather off (one pull per vertex), 4..30 sets log2 of the
+   * segment width in vertices. */
+  int32_t pr_segment_log2[2];
 } gg_tuning;
 enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };
 int gg_autotune(gg_graph* g, int what, const int32_t* roots, int k, gg_tuning* out);

@@ -108,3 +108,96 @@ def test_pr_gauss_seidel(tag, make):
     fixed, _, _ = oracle.pagerank(g, 200, 1e-12, fp64=True)
     # L1 distance to the fixed point <= last change x d / (1 - d) (L1 contraction).
     assert np.abs(gs.scores.astype(np.float64) - fixed).sum() < 1e-4 * 0.85 / 0.15 + 1e-5
+
+
+def _segmented(dg, sb, eps=1e-4):
+    """Jacobi runs with the given source-segment width (None: the default;
+    -1: segments off), restoring the default afterwards."""
+    gpu.set_tuning(dg, {"pr_segment_log2": [sb, sb] if sb is not None else [0, 0]})
+    try:
+        return (gpu.pagerank(dg, 20, eps), gpu.pagerank(dg, 20, eps, fp64=True))
+    finally:
+        gpu.set_tuning(dg, {"pr_segment_log2": [0, 0]})
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+@pytest.mark.parametrize("sb", [6, 10, 13])
+def test_pr_source_segments_against_fp64_oracle(ordered, sb):
+    # Source segments (the in-entries regrouped by source range, one launch
+    # per segment) forced at small widths on a graph whose contributions fit
+    # L2 anyway, so every part of the layout is exercised: many segments,
+    # block / wavefront / thread pairs, and the tail gather.  The segmented
+    # runs take more launches than the plain pull; both meet the oracle.
+    print(f"[pr] segments sb={sb} ordered={ordered}", flush=True)
+    g = cached_graph("g16", GRAPHS[1][1])
+    dg = gpu.DeviceGraph(g, order_by_degree=ordered)
+    want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+    p32, p64 = _segmented(dg, -1)
+    s32, s64 = _segmented(dg, sb)
+    assert s32.stats["launches"] > p32.stats["launches"] and s64.stats["launches"] > p64.stats["launches"]
+    for r in (s32, s64):
+        assert r.stats["iterations"] == it
+    x32 = s32.scores.astype(np.float64)
+    assert np.abs(x32 - want).max() < 1e-6
+    assert (np.abs(x32 - want) / want).max() < 1e-5
+    assert (np.abs(s64.scores.astype(np.float64) - want) / want).max() < 1e-6
+    assert np.abs(x32 - p32.scores.astype(np.float64)).max() < 1e-7
+    assert g.verify_pr(s32.scores, 1e-4)
+    dg.free()
+
+
+@pytest.mark.parametrize("sb", [6, 11])
+def test_pr_source_segments_directed_hub_and_isolated(sb):
+    # A directed graph with a hub of 6000 in-neighbours (wavefront pairs at
+    # 64-vertex segments, block pairs at 2048), dangling and isolated
+    # vertices; at 64-vertex segments the 64 candidate segments cover only
+    # part of the ids and the tail gather finishes the rest.
+    print(f"[pr] segments directed sb={sb}", flush=True)
+    rng = np.random.default_rng(17)
+    n = 9000
+    u = np.concatenate([rng.integers(0, n - 50, 40000), np.arange(1, 6001), rng.integers(0, 200, 5000)])
+    v = np.concatenate([rng.integers(0, n - 50, 40000), np.zeros(6000, np.int64), rng.integers(0, 40, 5000)])
+    g = Graph.from_edges(u, v, symmetrize=False)
+    dg = gpu.DeviceGraph(g)
+    want, it, _ = oracle.pagerank(g, 20, 0.0, fp64=True)
+    s32, s64 = _segmented(dg, sb, 0.0)
+    p32, _ = _segmented(dg, -1, 0.0)
+    assert s32.stats["launches"] > p32.stats["launches"]
+    assert s32.stats["iterations"] == it == 20
+    assert (np.abs(s32.scores.astype(np.float64) - want) / want).max() < 1e-5
+    np.testing.assert_allclose(s64.scores, want.astype(np.float32), rtol=1e-6, atol=0)
+    dg.free()
+
+
+def test_pr_source_segments_stay_off_for_unsorted_lists():
+    # Segments need sorted in-lists; an upload whose lists are shuffled keeps
+    # the plain pull (same launch count) and the same scores.
+    print("[pr] segments unsorted", flush=True)
+    import ctypes as C
+    g = cached_graph("u12", GRAPHS[2][1])
+    off = np.ascontiguousarray(g.out_offsets, dtype=np.int64)
+    dst = np.array(g.out_neighbors, dtype=np.int32)
+    rng = np.random.default_rng(3)
+    for v in range(g.num_nodes):
+        rng.shuffle(dst[off[v]:off[v + 1]])
+    h = C.c_void_p()
+    rc = gpu.lib().gg_graph_upload(g.num_nodes, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   dst.ctypes.data_as(C.POINTER(C.c_int32)), 0, 0, None, None, C.byref(h))
+    assert rc == 0
+    dg = gpu.DeviceGraph._adopt(h, g.num_nodes)
+    want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+    s32, _ = _segmented(dg, 6)
+    p32, _ = _segmented(dg, -1)
+    assert s32.stats["launches"] == p32.stats["launches"]
+    assert s32.stats["iterations"] == it
+    assert (np.abs(s32.scores.astype(np.float64) - want) / want).max() < 1e-5
+    dg.free()
+
+
+def test_pr_segment_width_is_validated():
+    dg = dev("g10", GRAPHS[0][1])
+    for bad in ([3, 0], [0, 31], [-2, 0]):
+        with pytest.raises(gpu.GpuError) as e:
+            gpu.set_tuning(dg, {"pr_segment_log2": bad})
+        assert e.value.code == -22
+    assert gpu.tuning(dg)["pr_segment_log2"] == [0, 0]
