@@ -107,7 +107,7 @@ struct gg_graph {
   int32_t* fold_dst = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
-  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}};  // gg_autotune results
+  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}, {0, 0}, 0};  // gg_autotune results
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
   // Jacobi PageRank source-segment layout, one per precision (gg_pr.hip):
@@ -116,8 +116,11 @@ struct gg_graph {
   // arrays live in the workspace ("pr.seg32.*" / "pr.seg64.*").
   struct PrSegments {
     bool built = false;
-    int sb = 0;                              // log2 segment width
-    int32_t small_max = 0, medium_max = 0;   // pair bin cut-offs it was built with
+    int sb = 0;                              // log2 width of the L2 segments
+    int sbl = 0;                             // log2 width of the LDS segments (-1: none)
+    int n_lds = 0;                           // LDS segments (the first n_lds of ns)
+    int32_t small_max = 0, medium_max = 0;   // pair bin cut-offs (sub-list lengths)
+    int32_t min_density = 0;                 // and the density rule
     int ns = 0;                              // segments in use (0: plain pull)
     int64_t hot = 0;                         // sources below this go by segment
     int64_t pairs = 0, hot_entries = 0;

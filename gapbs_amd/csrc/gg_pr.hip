@@ -15,21 +15,27 @@
 // second buffer, so the reference's separate contrib pass is fused away.
 //
 // Source segments (Jacobi modes).  A pull gather reads contrib[v] for every
-// in-entry at random: on a large graph most of those reads miss L2 and move a
-// whole cache line for 4 or 8 useful bytes.  With sorted in-lists, the
-// entries of u whose source lies in [s * 2^sb, (s + 1) * 2^sb) form one
-// contiguous run, a (destination, sub-list) pair of segment s.  Built once
-// per graph and precision: the pairs of the leading segments that are dense
-// enough (at least kSegMinEntriesPerPair entries per pair), grouped by
-// segment, with their sub-lists copied into one segment-major array.  An
-// iteration then runs one launch per segment, which reads only that
-// segment's 2^sb contributions (2 MB by default, so they stay in every
-// XCD's L2) and adds each pair's partial sum into the destination's
-// accumulator, then the usual three-bin gather over the rest of every list
-// (sources >= hot), which starts from the accumulator and finishes the
-// vertex.  On a degree-ordered Kronecker graph the hubs sit at the low ids,
-// so a few segments hold most of the entries (scale 26: 16 segments of 2^19
-// sources hold 95% of them in 1.8 pairs per vertex).  Two precisions:
+// in-entry at random.  Measured on the MI355X, random 4-byte gathers run at
+// about 1350 G/s from LDS, 200 G/s from an L2-resident range and 52 G/s from
+// HBM: on a large graph most pull gathers miss L2 and move a whole cache
+// line for 4 or 8 useful bytes.  With sorted in-lists, the entries of u
+// whose source lies in one range of sources (a segment) form one contiguous
+// run, a (destination, sub-list) pair of that segment.  Built once per graph
+// and precision: the pairs of the leading segments that hold a few entries
+// per pair, grouped by segment, with their sub-lists copied into one
+// segment-major array.  A segment covers 2 MB of contributions (2^19 float
+// or 2^18 double sources), so they stay in every XCD's L2 during its launch.
+// An iteration runs one launch per segment, each adding its pairs' partial
+// sums into the destinations' accumulators, then the usual three-bin gather
+// over the rest of every list (the sources beyond the last segment), which
+// starts from the accumulator and finishes the vertex.  On a degree-ordered
+// Kronecker graph the hubs sit at the low ids, so a few segments hold most
+// entries (scale 26: 16 segments hold 95% of them in 1.8 pairs per vertex).
+// Optional LDS segments (gg_tuning.pr_lds_log2) come first and stage 128 KB
+// of contributions in LDS per launch, one 1024-thread block per CU; at
+// Kronecker 26 they measured slower than L2 segments (their per-pair
+// gathers are latency-bound at 16 waves per CU), so they are off by default.
+// Two precisions:
 //   GG_PR_FP32  float scores and contributions, each vertex's sum kept as a
 //               compensated (Neumaier) pair and merged across lanes with
 //               TwoSum, so the sum carries about twice float's precision;
@@ -321,6 +327,11 @@ __device__ __forceinline__ void AddToAcc(typename Acc<T>::Stored* acc, int32_t v
   acc[v] = Pack(t);
 }
 
+// Pair bins of the segment layout, by sub-list length: a block per pair
+// from kPairBlock entries, a wavefront from kPairWave, else a thread.
+constexpr int kPairWave = 32;
+constexpr int kPairBlock = 1024;
+
 template <class T>
 __global__ __launch_bounds__(kBS) void PrSegment(SegArgs<T> a) {
   using A = typename Acc<T>::Type;
@@ -362,10 +373,100 @@ __global__ __launch_bounds__(kBS) void PrSegment(SegArgs<T> a) {
   }
 }
 
+// LDS segments: a block of kLdsBS threads per CU stages the segment's
+// contributions (kLdsBytes) in LDS and gathers from there.
+constexpr int kLdsBS = 1024;
+constexpr int kLdsBytes = 128 * 1024;
+
+template <class T>
+__device__ __forceinline__ void GatherLds(const T* __restrict__ slice, int64_t base, const int32_t* __restrict__ ids,
+                                          typename Acc<T>::Type& acc, int64_t k0, int64_t e, int step) {
+  int64_t k = k0;
+  for (; k + 3 * step < e; k += 4 * step) {
+    const int32_t v0 = ids[k], v1 = ids[k + step], v2 = ids[k + 2 * step], v3 = ids[k + 3 * step];
+    const T x0 = slice[v0 - base], x1 = slice[v1 - base], x2 = slice[v2 - base], x3 = slice[v3 - base];
+    acc.Add(x0);
+    acc.Add(x1);
+    acc.Add(x2);
+    acc.Add(x3);
+  }
+  for (; k < e; k += step) acc.Add(slice[ids[k] - base]);
+}
+
+// One LDS segment launch, one block per CU: the segment's contributions
+// [base, base + count) are staged in LDS, then the block takes its share of
+// the segment's block, wavefront and short pairs in turn.
+template <class T>
+__global__ __launch_bounds__(kLdsBS) void PrSegmentLds(SegArgs<T> a, int64_t base, int32_t count) {
+  using A = typename Acc<T>::Type;
+  __shared__ T slice[kLdsBytes / sizeof(T)];
+  __shared__ A s_acc[kLdsBS / kWave];
+  for (int i = threadIdx.x; i < count; i += kLdsBS) slice[i] = a.contrib[base + i];
+  __syncthreads();
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  for (int64_t i = blockIdx.x; i < a.n_large; i += gridDim.x) {
+    const int64_t p = a.p0 + i;
+    A acc = A::Zero();
+    GatherLds<T>(slice, base, a.src, acc, a.off[p] + threadIdx.x, a.off[p + 1], kLdsBS);
+    acc = WaveMerge(acc);
+    if (lane == 0) s_acc[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      A t = s_acc[0];
+      for (int w = 1; w < kLdsBS / kWave; w++) t.Merge(s_acc[w]);
+      AddToAcc<T>(a.acc, a.dst[p], t);
+    }
+    __syncthreads();
+  }
+  const int64_t nw = int64_t(gridDim.x) * (kLdsBS / kWave);
+  for (int64_t i = int64_t(blockIdx.x) * (kLdsBS / kWave) + wid; i < a.n_medium; i += nw) {
+    const int64_t p = a.p0 + a.n_large + i;
+    A acc = A::Zero();
+    GatherLds<T>(slice, base, a.src, acc, a.off[p] + lane, a.off[p + 1], kWave);
+    acc = WaveMerge(acc);
+    if (lane == 0) AddToAcc<T>(a.acc, a.dst[p], acc);
+  }
+  // Short pairs, two per thread at a time (their loads interleaved).
+  const int64_t q0 = a.p0 + a.n_large + a.n_medium, stride = int64_t(gridDim.x) * kLdsBS;
+  for (int64_t i = int64_t(blockIdx.x) * kLdsBS + threadIdx.x; i < a.n_small; i += 2 * stride) {
+    const int64_t p = q0 + i;
+    const bool two = i + stride < a.n_small;
+    const int64_t pb = two ? p + stride : p;
+    const int64_t b0 = a.off[p], e0 = a.off[p + 1], b1 = a.off[pb], e1 = two ? a.off[pb + 1] : b1;
+    const int32_t d0 = a.dst[p], d1 = a.dst[pb];
+    A acc0 = A::Zero(), acc1 = A::Zero();
+    const int64_t len = e0 - b0 > e1 - b1 ? e0 - b0 : e1 - b1;
+    for (int64_t k = 0; k < len; k++) {
+      const bool h0 = b0 + k < e0, h1 = b1 + k < e1;
+      const int32_t v0 = h0 ? a.src[b0 + k] : 0, v1 = h1 ? a.src[b1 + k] : 0;
+      if (h0) acc0.Add(slice[v0 - base]);
+      if (h1) acc1.Add(slice[v1 - base]);
+    }
+    AddToAcc<T>(a.acc, d0, acc0);
+    if (two) AddToAcc<T>(a.acc, d1, acc1);
+  }
+}
+
 // Building the layout (once per graph and precision, outside timing).
-constexpr int kSegMax = 64;                // candidate segments examined
-constexpr int kSegMinEntriesPerPair = 4;   // density a default-width segment needs
+constexpr int kLdsSegMax = 128;            // candidate LDS segments examined
+constexpr int kL2SegMax = 64;              // candidate L2 segments examined
+constexpr int kSegMax = kLdsSegMax + kL2SegMax;
+constexpr int kSegMinEntriesPerPair = 3;   // density a default-width segment needs (gg_tuning can change it)
 constexpr int kCopyTile = 4096;            // output entries per block of the sub-list copy
+
+// Segment geometry: n_lds segments of 2^sbl sources, then segments of 2^sb2
+// sources up to ns in all.
+struct SegGeom {
+  int sbl, sb2, n_lds, ns;
+  __device__ __forceinline__ int64_t LdsEnd() const { return int64_t(n_lds) << sbl; }
+  __device__ __forceinline__ int64_t Hot() const { return LdsEnd() + (int64_t(ns - n_lds) << sb2); }
+  __device__ __forceinline__ int Of(int64_t x) const {
+    return x < LdsEnd() ? static_cast<int>(x >> sbl) : n_lds + static_cast<int>((x - LdsEnd()) >> sb2);
+  }
+  __device__ __forceinline__ int64_t End(int s) const {
+    return s < n_lds ? int64_t(s + 1) << sbl : LdsEnd() + (int64_t(s + 1 - n_lds) << sb2);
+  }
+};
 
 // End of the run of sources below x that starts at b (a sorted list ending
 // at e): a short linear probe first, since most runs are short.
@@ -407,7 +508,7 @@ struct PlTotals {
 // bin (stored (segment, bin)-major, tile-minor, so one scan turns them into
 // output positions), and the totals per segment.
 __global__ __launch_bounds__(kBS) void PlCount(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
-                                               int64_t n, int sb, int nseg, int32_t small_max, int32_t medium_max,
+                                               int64_t n, SegGeom G, int32_t small_max, int32_t medium_max,
                                                int64_t tiles, uint32_t* __restrict__ counts, PlTotals* tot) {
   __shared__ uint32_t s_cnt[kSegMax * 3];
   __shared__ unsigned long long s_ent[kSegMax];
@@ -417,14 +518,14 @@ __global__ __launch_bounds__(kBS) void PlCount(const int64_t* __restrict__ in_of
   const int64_t tile = blockIdx.x;
   const int64_t v = tile * kBS + threadIdx.x;
   if (v < n) {
-    const int64_t hot = int64_t(nseg) << sb;
+    const int64_t hot = G.Hot();
     const int64_t end = in_off[v + 1];
     int64_t pos = in_off[v];
     while (pos < end) {
       const int64_t x = in_dst[pos];
       if (x >= hot) break;
-      const int s = static_cast<int>(x >> sb);
-      const int64_t nxt = RunEnd(in_dst, pos + 1, end, int64_t(s + 1) << sb);
+      const int s = G.Of(x);
+      const int64_t nxt = RunEnd(in_dst, pos + 1, end, G.End(s));
       const int64_t len = nxt - pos;
       const int bin = len >= medium_max ? 0 : (len >= small_max ? 1 : 2);
       atomicAdd(&s_cnt[s * 3 + bin], 1u);
@@ -433,11 +534,11 @@ __global__ __launch_bounds__(kBS) void PlCount(const int64_t* __restrict__ in_of
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nseg * 3; i += kBS) {
+  for (int i = threadIdx.x; i < G.ns * 3; i += kBS) {
     counts[int64_t(i) * tiles + tile] = s_cnt[i];
     if (s_cnt[i]) atomicAdd(&tot->pairs[i / 3][i % 3], static_cast<unsigned long long>(s_cnt[i]));
   }
-  for (int s = threadIdx.x; s < nseg; s += kBS)
+  for (int s = threadIdx.x; s < G.ns; s += kBS)
     if (s_ent[s]) atomicAdd(&tot->entries[s], s_ent[s]);
 }
 
@@ -445,27 +546,27 @@ __global__ __launch_bounds__(kBS) void PlCount(const int64_t* __restrict__ in_of
 // destination, length and list position at its place (segment, then bin,
 // then tile, then vertex order), and each vertex's tail offset.
 __global__ __launch_bounds__(kBS) void PlEmit(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
-                                              int64_t n, int sb, int ns, int32_t small_max, int32_t medium_max,
+                                              int64_t n, SegGeom G, int32_t small_max, int32_t medium_max,
                                               int64_t tiles, const uint32_t* __restrict__ base,
                                               int32_t* __restrict__ pair_dst, int64_t* __restrict__ pair_len,
                                               int64_t* __restrict__ pair_pos, int32_t* __restrict__ tail) {
   __shared__ int sh[kBS / kWave];
   const int64_t tile = blockIdx.x;
   const int64_t v = tile * kBS + threadIdx.x;
-  const int64_t hot = int64_t(ns) << sb;
+  const int64_t hot = G.Hot();
   int64_t beg = 0, pos = 0, end = 0;
-  int cur = ns;  // segment of the next run, ns once none is left
+  int cur = G.ns;  // segment of the next run, ns once none is left
   if (v < n) {
     beg = pos = in_off[v];
     end = in_off[v + 1];
-    if (pos < end && in_dst[pos] < hot) cur = static_cast<int>(in_dst[pos] >> sb);
+    if (pos < end && in_dst[pos] < hot) cur = G.Of(in_dst[pos]);
   }
-  for (int s = 0; s < ns; s++) {
+  for (int s = 0; s < G.ns; s++) {
     const bool has = cur == s;
     int64_t nxt = pos;
     int bin = 0;
     if (has) {
-      nxt = RunEnd(in_dst, pos + 1, end, int64_t(s + 1) << sb);
+      nxt = RunEnd(in_dst, pos + 1, end, G.End(s));
       const int64_t len = nxt - pos;
       bin = len >= medium_max ? 0 : (len >= small_max ? 1 : 2);
     }
@@ -477,7 +578,7 @@ __global__ __launch_bounds__(kBS) void PlEmit(const int64_t* __restrict__ in_off
       pair_len[p] = nxt - pos;
       pair_pos[p] = pos;
       pos = nxt;
-      cur = pos < end && in_dst[pos] < hot ? static_cast<int>(in_dst[pos] >> sb) : ns;
+      cur = pos < end && in_dst[pos] < hot ? G.Of(in_dst[pos]) : G.ns;
     }
   }
   if (v < n) tail[v] = static_cast<int32_t>(pos - beg);
@@ -605,32 +706,38 @@ PrBins GetBins(gg_graph* g) {
 }
 
 // The source-segment layout of precision T, built on first use (and again
-// when the width or the bin cut-offs change); null when the gather runs
-// without segments.  Default width: 2 MB of contributions per segment
-// (2^19 float or 2^18 double sources), used only when the contributions
-// span more than two segments and only for the leading segments with at
-// least kSegMinEntriesPerPair entries per pair; an explicit width
-// (gg_tuning.pr_segment_log2) takes every candidate segment.  Lists must be
-// sorted (they are for builder output and degree-ordered handles).
+// when a width or the density rule changes); null when the gather runs
+// without segments.  Segments of two kinds, in source order: LDS segments
+// (only when gg_tuning.pr_lds_log2 asks for them), whose launches stage up
+// to kLdsBytes of contributions in LDS, then L2 segments of 2 MB (2^19 float
+// or 2^18 double sources by default), whose contributions stay in every
+// XCD's L2.  By default (graphs of 2^20 vertices or more) the leading L2
+// segments with at least kSegMinEntriesPerPair entries per pair (or the
+// tuned density) are kept; explicit widths take every candidate segment.
+// Lists must be sorted (they are for builder output and degree-ordered
+// handles).
 template <class T>
 const gg_graph::PrSegments* Segments(gg_graph* g) {
   const int prec = sizeof(T) == 4 ? 0 : 1;
-  const int32_t knob = g->tune.pr_segment_log2[prec];
+  const int32_t knob = g->tune.pr_segment_log2[prec], lknob = g->tune.pr_lds_log2[prec];
   if (knob < 0) return nullptr;
   gg_graph::PrSegments& L = g->pr_seg[prec];
-  const int sb = knob > 0 ? knob : (sizeof(T) == 4 ? 19 : 18);
-  if (L.built && L.sb == sb && L.small_max == g->tune.pr_small_max && L.medium_max == g->tune.pr_medium_max)
-    return L.ns > 0 ? &L : nullptr;
+  const int sb2 = knob > 0 ? knob : (sizeof(T) == 4 ? 19 : 18);
+  const int sbl = lknob > 0 ? lknob : -1;
+  const int32_t density = g->tune.pr_seg_min_density > 0 ? g->tune.pr_seg_min_density : kSegMinEntriesPerPair;
+  if (L.built && L.sb == sb2 && L.sbl == sbl && L.min_density == density) return L.ns > 0 ? &L : nullptr;
   const std::string tag = sizeof(T) == 4 ? "pr.seg32." : "pr.seg64.";
   g->ws.Drop(tag);
   WallTimer wt;
   L = gg_graph::PrSegments{};
   L.built = true;
-  L.sb = sb;
-  L.small_max = g->tune.pr_small_max;
-  L.medium_max = g->tune.pr_medium_max;
-  const int64_t n = g->n, width = int64_t(1) << sb;
-  if (knob == 0 && n <= 2 * width) return nullptr;
+  L.sb = sb2;
+  L.sbl = sbl;
+  L.small_max = kPairWave;
+  L.medium_max = kPairBlock;
+  L.min_density = density;
+  const int64_t n = g->n;
+  if (knob == 0 && sbl < 0 && n < (int64_t(1) << 20)) return nullptr;
   const bool prof = g->profile;
   g->profile = false;  // the build is not part of any measured call
   struct Restore {
@@ -639,57 +746,84 @@ const gg_graph::PrSegments* Segments(gg_graph* g) {
     ~Restore() { g->profile = prof; }
   } restore{g, prof};
   hipStream_t st = g->stream;
-  const int nseg = static_cast<int>(std::min<int64_t>(kSegMax, (n + width - 1) / width));
   const int64_t tiles = (n + kBS - 1) / kBS;
   PlTotals* dtot = g->Scratch<PlTotals>("pr.segb.tot", 1);
-  uint32_t* counts = g->Scratch<uint32_t>("pr.segb.counts", static_cast<size_t>(nseg) * 3 * tiles);
+  uint32_t* counts = g->Scratch<uint32_t>("pr.segb.counts", static_cast<size_t>(kSegMax) * 3 * tiles);
   GG_HIP(hipMemsetAsync(dtot, 0, sizeof(PlTotals), st));
   PlCheck<<<GridFor(n, kBS / kWave, g->cu_count * 16), kBS, 0, st>>>(g->in_off, g->in_dst, n, &dtot->bad);
-  PlCount<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, sb, nseg, L.small_max,
-                                                        L.medium_max, tiles, counts, dtot);
-  GG_HIP(hipGetLastError());
   PlTotals tot;
-  GG_HIP(hipMemcpyAsync(&tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, st));
-  GG_HIP(hipStreamSynchronize(st));
-  int ns = 0;
-  int64_t pairs = 0, hot_entries = 0;
-  if (!tot.bad) {
-    for (; ns < nseg; ns++) {
-      const unsigned long long p = tot.pairs[ns][0] + tot.pairs[ns][1] + tot.pairs[ns][2];
-      if (knob == 0 && tot.entries[ns] < kSegMinEntriesPerPair * p) break;
-      pairs += static_cast<int64_t>(p);
-      hot_entries += static_cast<int64_t>(tot.entries[ns]);
+  auto count = [&](const SegGeom& G) {
+    GG_HIP(hipMemsetAsync(dtot->pairs, 0, sizeof(tot.pairs) + sizeof(tot.entries), st));
+    PlCount<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, G, L.small_max, L.medium_max,
+                                                          tiles, counts, dtot);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpyAsync(&tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+  };
+  // Leading segments of [s0, s1) dense enough to keep (all of them with an
+  // explicit width).
+  auto keep = [&](int s0, int s1, bool all) {
+    int s = s0;
+    for (; s < s1; s++) {
+      const unsigned long long p = tot.pairs[s][0] + tot.pairs[s][1] + tot.pairs[s][2];
+      if (p == 0 || (!all && tot.entries[s] < static_cast<unsigned long long>(density) * p)) break;
     }
+    return s;
+  };
+  SegGeom G{sbl < 0 ? 0 : sbl, sb2, 0, 0};
+  if (sbl >= 0) {
+    const int cand = static_cast<int>(std::min<int64_t>(kLdsSegMax, ((n - 1) >> sbl) + 1));
+    G.n_lds = G.ns = cand;
+    count(G);
+    if (tot.bad) {
+      g->ws.Drop("pr.segb.");
+      return nullptr;
+    }
+    G.n_lds = G.ns = keep(0, cand, lknob > 0);
   }
-  if (pairs >= (int64_t(1) << 31)) ns = 0;  // pair counts and positions are 32-bit
-  if (ns == 0 || pairs == 0) {
+  const int64_t rest = n - (int64_t(G.n_lds) << G.sbl);
+  const int cand2 = rest > 0 ? static_cast<int>(std::min<int64_t>(kL2SegMax, ((rest - 1) >> sb2) + 1)) : 0;
+  G.ns = G.n_lds + cand2;
+  if (G.ns > 0) count(G);
+  if (G.ns == 0 || tot.bad) {
     g->ws.Drop("pr.segb.");
-    L.ns = 0;
     return nullptr;
   }
-  L.ns = ns;
-  L.hot = std::min<int64_t>(int64_t(ns) << sb, n);
+  G.ns = keep(G.n_lds, G.n_lds + cand2, knob > 0);
+  int64_t pairs = 0, hot_entries = 0;
+  for (int s = 0; s < G.ns; s++) {
+    pairs += static_cast<int64_t>(tot.pairs[s][0] + tot.pairs[s][1] + tot.pairs[s][2]);
+    hot_entries += static_cast<int64_t>(tot.entries[s]);
+  }
+  if (G.ns == 0 || pairs == 0 || pairs >= (int64_t(1) << 31)) {  // pair counts and positions are 32-bit
+    g->ws.Drop("pr.segb.");
+    return nullptr;
+  }
+  L.ns = G.ns;
+  L.n_lds = G.n_lds;
+  L.hot = std::min<int64_t>((int64_t(G.n_lds) << G.sbl) + (int64_t(G.ns - G.n_lds) << sb2), n);
   L.pairs = pairs;
   L.hot_entries = hot_entries;
-  L.first_pair.assign(ns + 1, 0);
-  L.large.resize(ns);
-  L.medium.resize(ns);
-  L.entries.resize(ns);
-  for (int s = 0; s < ns; s++) {
+  L.first_pair.assign(G.ns + 1, 0);
+  L.large.resize(G.ns);
+  L.medium.resize(G.ns);
+  L.entries.resize(G.ns);
+  for (int s = 0; s < G.ns; s++) {
     L.large[s] = static_cast<int64_t>(tot.pairs[s][0]);
     L.medium[s] = static_cast<int64_t>(tot.pairs[s][1]);
     L.entries[s] = static_cast<int64_t>(tot.entries[s]);
     L.first_pair[s + 1] = L.first_pair[s] + L.large[s] + L.medium[s] + static_cast<int64_t>(tot.pairs[s][2]);
   }
-  // Output positions of each (segment, bin, tile) run of pairs.
-  DeviceScan<uint32_t>(g, "pr.segb.scan", counts, int64_t(ns) * 3 * tiles, nullptr);
+  // Output positions of each (segment, bin, tile) run of pairs (the counts
+  // of the kept segments lead the array).
+  DeviceScan<uint32_t>(g, "pr.segb.scan", counts, int64_t(G.ns) * 3 * tiles, nullptr);
   int32_t* pair_dst = g->Scratch<int32_t>((tag + "dst").c_str(), pairs);
   int64_t* pair_off = g->Scratch<int64_t>((tag + "off").c_str(), pairs + 1);
   int32_t* tail = g->Scratch<int32_t>((tag + "tail").c_str(), n);
   int32_t* src = g->Scratch<int32_t>((tag + "src").c_str(), hot_entries);
   int64_t* pair_pos = g->Scratch<int64_t>("pr.segb.pos", pairs);
-  PlEmit<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, sb, ns, L.small_max, L.medium_max,
-                                                       tiles, counts, pair_dst, pair_off, pair_pos, tail);
+  PlEmit<<<static_cast<unsigned>(tiles), kBS, 0, st>>>(g->in_off, g->in_dst, n, G, L.small_max, L.medium_max, tiles,
+                                                       counts, pair_dst, pair_off, pair_pos, tail);
   DeviceScan<int64_t>(g, "pr.segb.scan64", pair_off, pairs, nullptr);
   PlSetLast<<<1, 1, 0, st>>>(pair_off, pairs, hot_entries);
   PlCopy<<<GridFor(hot_entries, kCopyTile, g->cu_count * 16), kBS, 0, st>>>(pair_off, pair_pos, pairs, g->in_dst, src,
@@ -766,7 +900,14 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
         const int32_t gs = static_cast<int32_t>(nsm ? GridFor(nsm, kBS, g->cu_count * width) : 0);
         SegArgs<T> sa{ca, seg_src, seg_dst, seg_off, acc, p0, static_cast<int32_t>(nl), static_cast<int32_t>(nm),
                       static_cast<int32_t>(nsm), gl, gm};
-        Run(g, "pr_segment", 4.0 * seg->entries[s], PrSegment<T>, dim3(gl + gm + gs), blk, sa);
+        if (s < seg->n_lds) {
+          const int64_t lo = int64_t(s) << seg->sbl;
+          const int32_t cnt = static_cast<int32_t>(std::min<int64_t>(int64_t(1) << seg->sbl, n - lo));
+          Run(g, "pr_segment_lds", 4.0 * seg->entries[s], PrSegmentLds<T>, dim3(g->cu_count), dim3(kLdsBS), sa, lo,
+              cnt);
+        } else {
+          Run(g, "pr_segment", 4.0 * seg->entries[s], PrSegment<T>, dim3(gl + gm + gs), blk, sa);
+        }
         launches++;
       }
       bytes -= 4.0 * seg->hot_entries;
@@ -837,5 +978,24 @@ extern "C" int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode,
       RunPageRank<double>(g, max_iters, epsilon, false, scores, stats, wt);
     else
       RunPageRank<float>(g, max_iters, epsilon, mode == GG_PR_GS, scores, stats, wt);
+  });
+}
+
+extern "C" int gg_pagerank_layout(gg_graph* g, int mode, int64_t* info) {
+  return Guard([&] {
+    GG_REQUIRE(g != nullptr && info != nullptr, GG_EINVAL, "null argument");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "PageRank needs an unweighted graph upload");
+    GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
+    GG_REQUIRE(mode == GG_PR_FP32 || mode == GG_PR_FP64, GG_EINVAL, "a Jacobi mode");
+    GG_HIP(hipSetDevice(g->device));
+    const gg_graph::PrSegments* L = mode == GG_PR_FP64 ? Segments<double>(g) : Segments<float>(g);
+    for (int i = 0; i < 6; i++) info[i] = 0;
+    if (!L) return;
+    info[0] = L->n_lds;
+    info[1] = L->ns - L->n_lds;
+    info[2] = L->pairs;
+    info[3] = L->hot_entries;
+    info[4] = L->hot;
+    info[5] = static_cast<int64_t>(L->build_ms * 1e3);
   });
 }

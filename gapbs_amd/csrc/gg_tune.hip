@@ -181,10 +181,16 @@ extern "C" int gg_tuning_set(gg_graph* g, const gg_tuning* in) {
     GG_REQUIRE(in->pr_small_max >= 1 && in->pr_small_max < in->pr_medium_max, GG_EINVAL,
                "PageRank cut-offs must satisfy 1 <= small < medium");
     GG_REQUIRE(in->pr_blocks_per_cu >= 1 && in->pr_blocks_per_cu <= 64, GG_EINVAL, "launch width out of range");
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
       GG_REQUIRE(in->pr_segment_log2[i] == -1 || in->pr_segment_log2[i] == 0 ||
                      (in->pr_segment_log2[i] >= 4 && in->pr_segment_log2[i] <= 30),
                  GG_EINVAL, "PageRank segment width out of range");
+      GG_REQUIRE(in->pr_lds_log2[i] == -1 || in->pr_lds_log2[i] == 0 ||
+                     (in->pr_lds_log2[i] >= 4 && in->pr_lds_log2[i] <= (i == 0 ? 15 : 14)),
+                 GG_EINVAL, "PageRank LDS segment width out of range");
+    }
+    GG_REQUIRE(in->pr_seg_min_density >= 0 && in->pr_seg_min_density <= 64, GG_EINVAL,
+               "PageRank segment density out of range");
     const bool bins_changed =
         in->pr_small_max != g->tune.pr_small_max || in->pr_medium_max != g->tune.pr_medium_max;
     g->tune = *in;

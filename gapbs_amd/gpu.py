@@ -62,12 +62,14 @@ class Tuning(C.Structure):
     _fields_ = [("bfs_alpha", C.c_int32), ("bfs_beta", C.c_int32), ("batch_alpha", C.c_int32),
                 ("sssp_delta", C.c_int32), ("pr_small_max", C.c_int32), ("pr_medium_max", C.c_int32),
                 ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4),
-                ("pr_segment_log2", C.c_int32 * 2)]
+                ("pr_segment_log2", C.c_int32 * 2), ("pr_lds_log2", C.c_int32 * 2),
+                ("pr_seg_min_density", C.c_int32)]
+    _ARRAYS = ("best_ms", "pr_segment_log2", "pr_lds_log2")
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("best_ms", "pr_segment_log2")}
-        d["best_ms"] = list(self.best_ms)
-        d["pr_segment_log2"] = list(self.pr_segment_log2)
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in self._ARRAYS}
+        for k in self._ARRAYS:
+            d[k] = list(getattr(self, k))
         return d
 
 
@@ -76,7 +78,8 @@ TUNE_BFS, TUNE_BATCH, TUNE_SSSP, TUNE_PR = 1, 2, 4, 8
 
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
-    "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_pagerank", "gg_cc", "gg_bc", "gg_bc_batch", "gg_tc",
+    "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_sssp_batch", "gg_pagerank", "gg_pagerank_layout",
+    "gg_cc", "gg_cc_noalloc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
     "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate", "gg_graph_read", "gg_cc_sv",
     "gg_autotune", "gg_tuning_get", "gg_tuning_set",
@@ -109,6 +112,7 @@ def lib() -> C.CDLL:
         L.gg_sssp_batch.argtypes = [vp, _i32p, C.c_int, C.c_int32, _i32p, C.POINTER(Stats)]
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
+        L.gg_pagerank_layout.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_cc_noalloc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_cc_sv.argtypes = [vp, _i32p, C.POINTER(Stats)]
@@ -405,6 +409,15 @@ def pagerank(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4, fp64: 
     return PRResult(out, st.as_dict())
 
 
+def pagerank_layout(dg: DeviceGraph, fp64: bool = False) -> dict:
+    """gg_pagerank_layout: the source segments a Jacobi run gathers through
+    (built on first use)."""
+    info = (C.c_int64 * 6)()
+    _check(lib().gg_pagerank_layout(dg.handle, PR_FP64 if fp64 else PR_FP32, info))
+    return {"lds_segments": info[0], "l2_segments": info[1], "pairs": info[2], "entries": info[3],
+            "source_bound": info[4], "build_seconds": info[5] / 1e6}
+
+
 def PageRankPullGS(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:
     """Scores of the Gauss-Seidel pull PageRank (src/pr.cc:34-61)."""
     return pagerank(dg, max_iters, epsilon, gauss_seidel=True).scores
@@ -542,8 +555,8 @@ def set_tuning(dg: DeviceGraph, values: dict) -> None:
     for k, v in values.items():
         if k == "best_ms":
             continue
-        if k == "pr_segment_log2":
-            t.pr_segment_log2[0], t.pr_segment_log2[1] = v
+        if k in ("pr_segment_log2", "pr_lds_log2"):
+            getattr(t, k)[0], getattr(t, k)[1] = v
             continue
         setattr(t, k, v)
     _check(lib().gg_tuning_set(dg.handle, C.byref(t)))

@@ -140,6 +140,16 @@ int gg_sssp_batch(gg_graph* g, const int32_t* roots, int k, int32_t delta, int32
 enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1, GG_PR_GS = 2 };
 int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* scores, gg_stats* stats);
 
+/* The source-segment layout gg_pagerank gathers through in a Jacobi mode
+ * (GG_PR_FP32 or GG_PR_FP64) on this handle, built here if it is not yet.
+ * No reference counterpart (the reference pulls every list whole); for
+ * measurement reports.  info[0] LDS segments, info[1] L2 segments, info[2]
+ * (destination, sub-list) pairs, info[3] in-entries held by the segments,
+ * info[4] the source id below which entries go by segment, info[5] the
+ * layout's build time in microseconds; all 0 when the gather runs without
+ * segments. */
+int gg_pagerank_layout(gg_graph* g, int mode, int64_t* info);
+
 /* Afforest connected components.
  * Replaces Afforest(const Graph&, bool logging, int32_t neighbor_rounds=2)
  * (src/cc.cc:95-151).  comp[n] (nullable): component label = the smallest
@@ -299,11 +309,18 @@ typedef struct gg_tuning {
   int32_t pr_blocks_per_cu;  /* gather launch width */
   int32_t tuned;             /* GG_TUNE_* bits tuned so far */
   double best_ms[4];         /* device ms of the chosen setting: bfs, batch, sssp, pr */
-  /* Jacobi PageRank source segments, [0] FP32 and [1] FP64: 0 picks the
-   * default width (2 MB of contributions per segment), -1 turns the
-   * segmented gather off (one pull per vertex), 4..30 sets log2 of the
-   * segment width in vertices. */
+  /* Jacobi PageRank source segments, [0] FP32 and [1] FP64.
+   * pr_segment_log2: width of the segments gathered through L2; 0 picks the
+   * default (2 MB of contributions), -1 turns segments off altogether (one
+   * pull per vertex), 4..30 sets log2 of the width in vertices.
+   * pr_lds_log2: segments staged in LDS, which come first: 0 or -1 none
+   * (the default), 4..15 (FP32) or 4..14 (FP64) log2 of their width. */
   int32_t pr_segment_log2[2];
+  int32_t pr_lds_log2[2];
+  /* Default widths keep the leading segments of each kind whose in-entries
+   * are at least this many per (destination, sub-list) pair; 0 picks the
+   * default (3), else 1..64. */
+  int32_t pr_seg_min_density;
 } gg_tuning;
 enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };
 int gg_autotune(gg_graph* g, int what, const int32_t* roots, int k, gg_tuning* out);

@@ -110,30 +110,31 @@ def test_pr_gauss_seidel(tag, make):
     assert np.abs(gs.scores.astype(np.float64) - fixed).sum() < 1e-4 * 0.85 / 0.15 + 1e-5
 
 
-def _segmented(dg, sb, eps=1e-4):
-    """Jacobi runs with the given source-segment width (None: the default;
-    -1: segments off), restoring the default afterwards."""
-    gpu.set_tuning(dg, {"pr_segment_log2": [sb, sb] if sb is not None else [0, 0]})
+def _segmented(dg, sb, eps=1e-4, lds=-1):
+    """Jacobi runs with the given L2 and LDS source-segment widths (log2; 0:
+    the default, -1: off), restoring the defaults afterwards."""
+    gpu.set_tuning(dg, {"pr_segment_log2": [sb, sb], "pr_lds_log2": [lds, lds]})
     try:
         return (gpu.pagerank(dg, 20, eps), gpu.pagerank(dg, 20, eps, fp64=True))
     finally:
-        gpu.set_tuning(dg, {"pr_segment_log2": [0, 0]})
+        gpu.set_tuning(dg, {"pr_segment_log2": [0, 0], "pr_lds_log2": [0, 0]})
 
 
 @pytest.mark.parametrize("ordered", [True, False])
-@pytest.mark.parametrize("sb", [6, 10, 13])
-def test_pr_source_segments_against_fp64_oracle(ordered, sb):
+@pytest.mark.parametrize("lds,sb", [(-1, 6), (-1, 13), (4, 8), (6, 10), (8, 12), (14, 16)])
+def test_pr_source_segments_against_fp64_oracle(ordered, lds, sb):
     # Source segments (the in-entries regrouped by source range, one launch
-    # per segment) forced at small widths on a graph whose contributions fit
-    # L2 anyway, so every part of the layout is exercised: many segments,
-    # block / wavefront / thread pairs, and the tail gather.  The segmented
-    # runs take more launches than the plain pull; both meet the oracle.
-    print(f"[pr] segments sb={sb} ordered={ordered}", flush=True)
+    # per segment; the first ones staged in LDS) forced at small widths on a
+    # graph whose contributions fit L2 anyway, so every part of the layout is
+    # exercised: up to 128 LDS and 64 L2 segments, block / wavefront / thread
+    # pairs, and the tail gather.  The segmented runs take more launches than
+    # the plain pull; both meet the oracle.
+    print(f"[pr] segments lds={lds} sb={sb} ordered={ordered}", flush=True)
     g = cached_graph("g16", GRAPHS[1][1])
     dg = gpu.DeviceGraph(g, order_by_degree=ordered)
     want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
     p32, p64 = _segmented(dg, -1)
-    s32, s64 = _segmented(dg, sb)
+    s32, s64 = _segmented(dg, sb, lds=lds)
     assert s32.stats["launches"] > p32.stats["launches"] and s64.stats["launches"] > p64.stats["launches"]
     for r in (s32, s64):
         assert r.stats["iterations"] == it
@@ -146,13 +147,13 @@ def test_pr_source_segments_against_fp64_oracle(ordered, sb):
     dg.free()
 
 
-@pytest.mark.parametrize("sb", [6, 11])
-def test_pr_source_segments_directed_hub_and_isolated(sb):
+@pytest.mark.parametrize("lds,sb", [(-1, 6), (-1, 11), (6, 9), (11, 12)])
+def test_pr_source_segments_directed_hub_and_isolated(lds, sb):
     # A directed graph with a hub of 6000 in-neighbours (wavefront pairs at
     # 64-vertex segments, block pairs at 2048), dangling and isolated
     # vertices; at 64-vertex segments the 64 candidate segments cover only
     # part of the ids and the tail gather finishes the rest.
-    print(f"[pr] segments directed sb={sb}", flush=True)
+    print(f"[pr] segments directed lds={lds} sb={sb}", flush=True)
     rng = np.random.default_rng(17)
     n = 9000
     u = np.concatenate([rng.integers(0, n - 50, 40000), np.arange(1, 6001), rng.integers(0, 200, 5000)])
@@ -160,7 +161,7 @@ def test_pr_source_segments_directed_hub_and_isolated(sb):
     g = Graph.from_edges(u, v, symmetrize=False)
     dg = gpu.DeviceGraph(g)
     want, it, _ = oracle.pagerank(g, 20, 0.0, fp64=True)
-    s32, s64 = _segmented(dg, sb, 0.0)
+    s32, s64 = _segmented(dg, sb, 0.0, lds)
     p32, _ = _segmented(dg, -1, 0.0)
     assert s32.stats["launches"] > p32.stats["launches"]
     assert s32.stats["iterations"] == it == 20
@@ -186,7 +187,7 @@ def test_pr_source_segments_stay_off_for_unsorted_lists():
     assert rc == 0
     dg = gpu.DeviceGraph._adopt(h, g.num_nodes)
     want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
-    s32, _ = _segmented(dg, 6)
+    s32, _ = _segmented(dg, 6, lds=4)
     p32, _ = _segmented(dg, -1)
     assert s32.stats["launches"] == p32.stats["launches"]
     assert s32.stats["iterations"] == it
@@ -196,8 +197,10 @@ def test_pr_source_segments_stay_off_for_unsorted_lists():
 
 def test_pr_segment_width_is_validated():
     dg = dev("g10", GRAPHS[0][1])
-    for bad in ([3, 0], [0, 31], [-2, 0]):
+    for key, bad in (("pr_segment_log2", [3, 0]), ("pr_segment_log2", [0, 31]), ("pr_segment_log2", [-2, 0]),
+                     ("pr_lds_log2", [16, 0]), ("pr_lds_log2", [0, 15]), ("pr_lds_log2", [3, 0])):
         with pytest.raises(gpu.GpuError) as e:
-            gpu.set_tuning(dg, {"pr_segment_log2": bad})
+            gpu.set_tuning(dg, {key: bad})
         assert e.value.code == -22
-    assert gpu.tuning(dg)["pr_segment_log2"] == [0, 0]
+    t = gpu.tuning(dg)
+    assert t["pr_segment_log2"] == [0, 0] and t["pr_lds_log2"] == [0, 0]

@@ -227,6 +227,22 @@ class TestKronecker26:
         dg, roots = k26["dg"], k26["roots"]
         assert np.array_equal(gpu.bc_batch(dg, roots).scores, gpu.bc(dg, roots).scores)
 
+    def test_pagerank_against_fp64_oracle(self, k26):
+        """The measured PageRank configuration (degree-ordered, default LDS and
+        L2 source segments) at full size: FP32 within 1e-6 max-abs of the FP64
+        oracle with the same iteration count; FP64 within float rounding."""
+        g, dg = k26["g"], k26["dg"]
+        say("[k26] pagerank fp32 / fp64")
+        r32 = gpu.pagerank(dg, 20, 1e-4)
+        r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
+        say("[k26] oracle pagerank fp64 on the host")
+        want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+        err = np.abs(r32.scores.astype(np.float64) - want)
+        say(f"[k26] pagerank FP32 vs FP64 oracle: max-abs {err.max():.3e}, L1 {err.sum():.3e}, {it} iterations")
+        assert r32.stats["iterations"] == it and r64.stats["iterations"] == it
+        assert err.max() < 1e-6
+        assert (np.abs(r64.scores.astype(np.float64) - want) / want).max() < 1e-6
+
     def test_batched_bc_against_oracle(self, k26):
         """The measured layout (degree-ordered) against the oracle's float
         Brandes (the reference's arithmetic, src/bc.cc:95-150) on the uploaded
