@@ -440,6 +440,8 @@ def main():
     ap.add_argument("--sssp-roots", type=int, default=4, help="single-root SSSP calls per step (Kronecker)")
     ap.add_argument("--bc-roots", type=int, default=4, help="roots of the BC precision report")
     ap.add_argument("--delta", type=int, default=32, help="SSSP bucket width without autotuning")
+    ap.add_argument("--batch-delta", type=int, default=None,
+                    help="batched SSSP bucket width (default: searched, or --delta without autotuning)")
     ap.add_argument("--alpha", type=int, default=15, help="direction switch for both BFS paths")
     ap.add_argument("--kernels", default=None, help="comma-separated lines (default: the workload's)")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per sampled CPU baseline")
@@ -506,17 +508,25 @@ def main():
     if W["cpu_delta"] is None:
         W["cpu_delta"] = delta
     # Batched SSSP shares its buckets between roots, so its best width is
-    # picked separately: the fastest of a few multiples of the single-root
-    # width, from one untimed call each (results do not depend on it).
-    batch_delta = delta
-    if "sssp_batch" in kernels and not args.no_autotune:
+    # picked separately: from 16 times the single-root width, widths four
+    # times larger, then four times smaller, while they get faster (one
+    # untimed call each; results do not depend on the width).
+    batch_delta = args.batch_delta or delta
+    if "sssp_batch" in kernels and not args.no_autotune and args.batch_delta is None:
         t0 = time.time()
         times = {}
-        for d in sorted({delta, 4 * delta, 16 * delta, 64 * delta}):
-            gpu.sssp_batch(dwg, roots, d, want_dist=False)  # allocations
-            times[d] = gpu.sssp_batch(dwg, roots, d, want_dist=False).stats["device_ms"]
-            if times[d] > 3 * min(times.values()):
-                break
+
+        def probe(d):
+            if d not in times:
+                times[d] = gpu.sssp_batch(dwg, roots, d, want_dist=False).stats["device_ms"]
+            return times[d]
+
+        start = min(16 * delta, 1 << 20)
+        gpu.sssp_batch(dwg, roots, start, want_dist=False)  # allocations
+        for step in (4, 0.25):
+            d = start
+            while 1 <= d * step <= (1 << 20) and probe(int(d * step)) < probe(d):
+                d = int(d * step)
         batch_delta = min(times, key=times.get)
         tuned = {**(tuned or {}), "sssp_batch_delta": batch_delta,
                  "sssp_batch_ms": {str(d): round(v, 3) for d, v in times.items()},
@@ -538,6 +548,7 @@ def main():
     # 16, generator seed = graph index), 8 roots each, in one traversal; the
     # same roots graph by graph (one batched call per graph) for comparison.
     packed = None
+    packed_alpha = batch_alpha
     if "packed" in kernels:
         t0 = time.time()
         scales = np.random.default_rng(0).integers(10, 17, 256)
@@ -547,6 +558,10 @@ def main():
         packed = (gpu.DeviceGraph(union), bounds, proots, [gpu.DeviceGraph(pg) for pg in pgraphs])
         log(f"packed workload: {union.num_nodes} vertices, {union.num_edges()} edges, built in "
             f"{time.time() - t0:.1f}s")
+        if not args.no_autotune:  # the packed union's own direction switch
+            packed_alpha = gpu.autotune(packed[0], gpu.TUNE_BATCH, proots[:, 0][:64])["batch_alpha"]
+            tuned = {**(tuned or {}), "packed_batch_alpha": packed_alpha}
+            log(f"packed batch alpha: {packed_alpha}")
 
     n = g.num_nodes
     host_out = {}
@@ -617,10 +632,10 @@ def main():
             db.free()
             return st
         elif k == "packed":
-            return gpu.bfs_packed(packed[0], packed[1], packed[2], batch_alpha, want_depths=False).stats
+            return gpu.bfs_packed(packed[0], packed[1], packed[2], packed_alpha, want_depths=False).stats
         elif k == "packed_per_graph":
             for i, (dgi, r) in enumerate(zip(packed[3], packed[2])):
-                gpu.bfs_batch(dgi, r - packed[1][i], batch_alpha, want_depths=False)
+                gpu.bfs_batch(dgi, r - packed[1][i], packed_alpha, want_depths=False)
         elif k == "delta_sweep":
             return delta_sweep()
         return None

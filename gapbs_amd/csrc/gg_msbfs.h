@@ -33,6 +33,7 @@
 // the array needs no initialisation.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -531,6 +532,15 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunkBuild(const int64_t* __restri
   }
 }
 
+// The first in-neighbour of every heavy chunk (the key the chunks are
+// ordered by, once per graph).
+__global__ __launch_bounds__(kBS) void MsChunkFirst(const int32_t* __restrict__ in_dst,
+                                                    const EdgeChunk* __restrict__ chunks, int32_t nch,
+                                                    int32_t* __restrict__ first) {
+  for (int64_t i = blockIdx.x * int64_t(kBS) + threadIdx.x; i < nch; i += int64_t(gridDim.x) * kBS)
+    first[i] = in_dst[chunks[i].beg];
+}
+
 // Heavy tier: one wavefront per chunk of a vertex that still needs live
 // bits; kUnroll x 64 in-neighbours per step, OR-reduced across the
 // wavefront, stopping once this chunk and the others (acc, read without
@@ -552,6 +562,10 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
     const EdgeChunk ch = chunks[it];
     const uint64_t need = live & ~seen[ch.u];
     if (need == 0) continue;
+    // Earlier chunks of the vertex may have found every needed bit already.
+    uint64_t before = lane == 0 ? acc[ch.u] : 0;
+    before = __shfl(before, 0);
+    if ((before & need) == need) continue;
     uint64_t got = 0;
     for (int64_t base = ch.beg; base < ch.end; base += kWave * kUnroll) {
       int32_t nbr[kUnroll];
@@ -800,6 +814,31 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
     GG_HIP(hipMemcpyAsync(&nc, &dc->heavy_chunks, sizeof(nc), hipMemcpyDeviceToHost, st));
     GG_HIP(hipStreamSynchronize(st));
     GG_REQUIRE(nc <= max_chunks, GG_EHIP, "heavy chunk count exceeds its bound");
+    // Chunks in the order of their first in-neighbour (ties by vertex).
+    // Lists are sorted, so chunks running side by side start among the same
+    // sources and share those frontier masks in L2, and every heavy
+    // vertex's first chunk -- its hub neighbours, where it most likely finds
+    // its bits -- runs before its later ones, which can then stop at once.
+    if (nc > 1) {
+      int32_t* first = g->Scratch<int32_t>("ms.hchunk_first", nc);
+      MsChunkFirst<<<dim3(GridFor(nc, kBS, g->cu_count * 4)), blk, 0, st>>>(g->in_dst, hchunks, nc, first);
+      GG_HIP(hipGetLastError());
+      std::vector<EdgeChunk> hc(nc);
+      std::vector<int32_t> key(nc);
+      GG_HIP(hipMemcpyAsync(hc.data(), hchunks, sizeof(EdgeChunk) * nc, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipMemcpyAsync(key.data(), first, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      std::vector<int32_t> idx(nc);
+      for (int32_t i = 0; i < nc; i++) idx[i] = i;
+      std::sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) {
+        return key[a] != key[b] ? key[a] < key[b] : (hc[a].u != hc[b].u ? hc[a].u < hc[b].u : hc[a].beg < hc[b].beg);
+      });
+      std::vector<EdgeChunk> sorted(nc);
+      for (int32_t i = 0; i < nc; i++) sorted[i] = hc[idx[i]];
+      GG_HIP(hipMemcpyAsync(hchunks, sorted.data(), sizeof(EdgeChunk) * nc, hipMemcpyHostToDevice, st));
+      GG_HIP(hipStreamSynchronize(st));
+      g->ws.Drop("ms.hchunk_first");
+    }
     g->ms_heavy = nh;
     g->ms_heavy_chunks = nc;
   }
