@@ -71,12 +71,12 @@ def test_batch_bc_repeated_isolated_and_heavy():
     # loop), a degree-0 root, and two hubs above the heavy-chunk threshold.
     print("[msbc] repeated / isolated / heavy", flush=True)
     a = np.zeros(3000, np.int32)
-    b = np.full(900, 1, np.int32)
-    u = np.concatenate([a, b, [0, 5000]])
-    v = np.concatenate([np.arange(2, 3002), np.arange(3002, 3902), [1, 5000]])
+    b = np.full(2500, 1, np.int32)
+    u = np.concatenate([a, b, [0, 7000]])
+    v = np.concatenate([np.arange(2, 3002), np.arange(3002, 5502), [1, 7000]])
     g = Graph.from_edges(u, v)
     dg = gpu.DeviceGraph(g)
-    check(g, dg, [5, 5, 3500, 0, 5000, 3001, 1, 5], single=True)
+    check(g, dg, [5, 5, 3500, 0, 7000, 3001, 1, 5], single=True)
 
 
 def test_batch_bc_directed():
