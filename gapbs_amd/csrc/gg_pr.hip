@@ -846,6 +846,7 @@ template <class T>
 void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, float* scores_out, gg_stats* stats,
                  const WallTimer& wt) {
   using Stored = typename Acc<T>::Stored;
+  const bool fp64 = sizeof(T) == 8;  // profile names carry the precision
   const int64_t n = g->n;
   const PrBins bins = GetBins(g);
   const gg_graph::PrSegments* seg = in_place ? nullptr : Segments<T>(g);
@@ -873,7 +874,7 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
 
   BeginTiming(g);
   Run(g, "pr_reset", 0, PrResetResidual, dim3(1), blk, residual);
-  Run(g, "pr_init", 8 * (N + 1) + 2 * sizeof(T) * N, PrInit<T>, dim3(GridFor(n, kBS, g->cu_count * 32)),
+  Run(g, fp64 ? "pr_init_fp64" : "pr_init", 8 * (N + 1) + 2 * sizeof(T) * N, PrInit<T>, dim3(GridFor(n, kBS, g->cu_count * 32)),
       blk, static_cast<const int64_t*>(g->out_off), score, ca, acc, n, init);
   launches += 2;
   int iters = 0;
@@ -903,10 +904,10 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
         if (s < seg->n_lds) {
           const int64_t lo = int64_t(s) << seg->sbl;
           const int32_t cnt = static_cast<int32_t>(std::min<int64_t>(int64_t(1) << seg->sbl, n - lo));
-          Run(g, "pr_segment_lds", 4.0 * seg->entries[s], PrSegmentLds<T>, dim3(g->cu_count), dim3(kLdsBS), sa, lo,
+          Run(g, fp64 ? "pr_segment_lds_fp64" : "pr_segment_lds", 4.0 * seg->entries[s], PrSegmentLds<T>, dim3(g->cu_count), dim3(kLdsBS), sa, lo,
               cnt);
         } else {
-          Run(g, "pr_segment", 4.0 * seg->entries[s], PrSegment<T>, dim3(gl + gm + gs), blk, sa);
+          Run(g, fp64 ? "pr_segment_fp64" : "pr_segment", 4.0 * seg->entries[s], PrSegment<T>, dim3(gl + gm + gs), blk, sa);
         }
         launches++;
       }
@@ -921,7 +922,7 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
     for (int pass = 0; pass < passes; pass++) {
       a.lo = static_cast<int32_t>(n * pass / passes);
       a.hi = static_cast<int32_t>(n * (pass + 1) / passes);
-      Run(g, "pr_gather", share * bytes, PrGather<T>, dim3(nb_large + nb_medium + nb_small), blk, a,
+      Run(g, fp64 ? "pr_gather_fp64" : "pr_gather", share * bytes, PrGather<T>, dim3(nb_large + nb_medium + nb_small), blk, a,
           static_cast<const int32_t*>(bins.small), static_cast<int32_t>(bins.n_small),
           static_cast<const int32_t*>(bins.medium), static_cast<int32_t>(bins.n_medium),
           static_cast<const int32_t*>(bins.large), static_cast<int32_t>(bins.n_large), nb_large, nb_medium);
