@@ -418,7 +418,7 @@ def make_workload(args, kernels):
         if ordered:
             W["order_ms"]["weighted_graph"] = round(W["dwg"].order_stats["device_ms"], 3)
     W["relabel_tc"] = "tc" in kernels and g.worth_relabelling()
-    W["tc_cpu"] = "reference" if name == "kron22" else "port"
+    W["tc_cpu"] = "reference" if name == "kron22" or args.tc_cpu == "reference" else "port"
     W["bc_cpu_roots"] = {"kron26": 1, "kron22": 4, "uniform25": 2}.get(name, 1)
     W["workload"] = W["desc"] + (", degree-ordered layout" if ordered else "")
     # m_r per root, from the device's own reach statistics.
@@ -446,6 +446,9 @@ def main():
     ap.add_argument("--kernels", default=None, help="comma-separated lines (default: the workload's)")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per sampled CPU baseline")
     ap.add_argument("--tc-stride", type=int, default=64, help="CPU TC sample: one vertex in stride")
+    ap.add_argument("--tc-cpu", choices=("sample", "reference"), default="sample",
+                    help="CPU TC baseline at scale 26: a sampled OrderedCount (default, seconds) or the "
+                         "reference's whole Hybrid (about 150 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the timings with host result copies")
     ap.add_argument("--no-autotune", action="store_true",
