@@ -561,10 +561,14 @@ def main():
         packed = (gpu.DeviceGraph(union), bounds, proots, [gpu.DeviceGraph(pg) for pg in pgraphs])
         log(f"packed workload: {union.num_nodes} vertices, {union.num_edges()} edges, built in "
             f"{time.time() - t0:.1f}s")
-        if not args.no_autotune:  # the packed union's own direction switch
-            packed_alpha = gpu.autotune(packed[0], gpu.TUNE_BATCH, proots[:, 0][:64])["batch_alpha"]
-            tuned = {**(tuned or {}), "packed_batch_alpha": packed_alpha}
-            log(f"packed batch alpha: {packed_alpha}")
+        if not args.no_autotune:  # the packed traversal's own direction switch, from one call per candidate
+            gpu.bfs_packed(packed[0], bounds, proots, batch_alpha, want_depths=False)  # allocations, part check
+            ptimes = {a: gpu.bfs_packed(packed[0], bounds, proots, a, want_depths=False).stats["device_ms"]
+                      for a in (2, 4, 8, 15, 30, 60, 120, 240)}
+            packed_alpha = min(ptimes, key=ptimes.get)
+            tuned = {**(tuned or {}), "packed_batch_alpha": packed_alpha,
+                     "packed_ms": {str(a): round(v, 3) for a, v in ptimes.items()}}
+            log(f"packed batch alpha: {packed_alpha} ({ptimes})")
 
     n = g.num_nodes
     host_out = {}

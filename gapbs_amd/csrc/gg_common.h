@@ -137,6 +137,7 @@ struct gg_graph {
   hipGraphExec_t ms_exec = nullptr;      // batched traversal: two captured levels, replayed
   std::vector<uintptr_t> ms_exec_key;    // the buffers and arguments captured in it
   int ms_exec_kernels = 0;               // kernels per replay
+  std::vector<int64_t> pk_parts;         // packed traversal: the part list the lists were last checked against
   hipGraphExec_t sb_exec = nullptr;      // batched SSSP: captured rounds, replayed
   std::vector<uintptr_t> sb_exec_key;    // the buffers and arguments captured in it
   gg::Workspace ws;

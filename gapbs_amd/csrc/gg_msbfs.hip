@@ -3,6 +3,8 @@
 // Reference counterpart: none directly -- the reference runs one root per
 // trial (src/benchmark.h:106, src/bfs.cc:256-259).  Each root's depths
 // equal what DOBFS gives for it alone (bfs.cc:122-179).
+#include <algorithm>
+
 #include "gg_msbfs.h"
 
 namespace gg {
@@ -183,13 +185,20 @@ extern "C" int gg_bfs_packed(gg_graph* g, const int64_t* part, int nparts, const
     GG_HIP(hipMemcpyAsync(dpart, part, sizeof(int64_t) * (nparts + 1), hipMemcpyHostToDevice, st));
     GG_HIP(hipMemcpyAsync(droots, roots, sizeof(int32_t) * count, hipMemcpyHostToDevice, st));
     GG_HIP(hipMemcpyAsync(dbits, bits.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, st));
-    GG_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), st));
-    PkCheckParts<<<grid_n, blk, 0, st>>>(g->out_off, g->out_dst, n, dpart, nparts, bad);
-    if (g->directed) PkCheckParts<<<grid_n, blk, 0, st>>>(g->in_off, g->in_dst, n, dpart, nparts, bad);
-    int32_t hbad = 0;
-    GG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st));
-    GG_HIP(hipStreamSynchronize(st));
-    GG_REQUIRE(hbad == 0, GG_EINVAL, "an edge joins two parts");
+    // The lists are checked against the parts once per handle and part
+    // list (the graph does not change between calls).
+    if (g->pk_parts.size() != static_cast<size_t>(nparts) + 1 ||
+        !std::equal(g->pk_parts.begin(), g->pk_parts.end(), part)) {
+      g->pk_parts.clear();
+      GG_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), st));
+      PkCheckParts<<<grid_n, blk, 0, st>>>(g->out_off, g->out_dst, n, dpart, nparts, bad);
+      if (g->directed) PkCheckParts<<<grid_n, blk, 0, st>>>(g->in_off, g->in_dst, n, dpart, nparts, bad);
+      int32_t hbad = 0;
+      GG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+      GG_REQUIRE(hbad == 0, GG_EINVAL, "an edge joins two parts");
+      g->pk_parts.assign(part, part + nparts + 1);
+    }
     uint16_t* depth16 = g->Scratch<uint16_t>("ms.depth16", n * 64);
     const uint64_t full = R == 64 ? ~0ull : (1ull << R) - 1;
     BeginTiming(g);
