@@ -268,9 +268,97 @@ __device__ __forceinline__ void RelaxFlush(RelaxCtx& c, unsigned long long* s_re
   }
 }
 
-// Expands the frontier q[parity].  A group per vertex: take its pending
-// roots, put back those beyond the bucket, relax the rest (hubs: hand their
-// edges to the chunk list).
+// Expands frontier vertex u with its whole group: take its pending roots,
+// put back those beyond the bucket, relax the rest (hubs: hand their edges
+// to the chunk list).  Group-uniform control flow.
+__device__ __forceinline__ void GroupExpand(RelaxCtx& c, const Group& g, int32_t u, const int64_t* __restrict__ off,
+                                            SbChunk* __restrict__ chunks, SbState* S, int lg) {
+  unsigned long long m = 0;
+  if (LaneId() == g.leader) m = atomicExch(reinterpret_cast<unsigned long long*>(&c.chg[u]), 0ull);
+  m = __shfl(m, g.leader);
+  const bool mine = (m >> g.r) & 1;
+  const int32_t du = mine ? LoadC(&c.dist[(int64_t(u) << lg) + g.r]) : kDistInf;
+  const bool active = mine && du < c.hi;
+  const uint64_t later = GroupBallot(g, mine && !active);
+  const uint64_t act = GroupBallot(g, active);
+  if (later && LaneId() == g.leader)
+    atomicOr(reinterpret_cast<unsigned long long*>(&c.chg[u]), static_cast<unsigned long long>(later));
+  if (!act) return;
+  const int64_t b = off[u], e = off[u + 1];
+  if (e - b >= kSerialMax) {
+    if (LaneId() == g.leader) {
+      const int64_t nc = (e - b + kChunkE - 1) / kChunkE;
+      const int at = atomicAdd(&S->nchunks, static_cast<int>(nc));
+      for (int64_t x = 0; x < nc; x++) {
+        const int64_t cb = b + x * kChunkE;
+        chunks[at + x] = SbChunk{cb, cb + kChunkE < e ? cb + kChunkE : e, u, 0, act};
+      }
+    }
+    return;
+  }
+  RelaxRange(c, g, u, b, e, active, du);
+}
+
+// Expands frontier vertex u with one lane, for the few roots pending there
+// (the same rules as GroupExpand, root by root).  Lanes of a wavefront may
+// diverge; the queue pushes are wave-wide over the lanes still active.
+__device__ __forceinline__ void LaneExpand(RelaxCtx& c, int32_t u, const int64_t* __restrict__ off,
+                                           SbChunk* __restrict__ chunks, SbState* S, int lg) {
+  const uint64_t m = atomicExch(reinterpret_cast<unsigned long long*>(&c.chg[u]), 0ull);
+  const int64_t b = off[u], e = off[u + 1];
+  const bool hub = e - b >= kSerialMax;
+  uint64_t later = 0, act = 0;
+  for (uint64_t rest = m; rest; rest &= rest - 1) {
+    const int r = __builtin_ctzll(rest);
+    const int32_t du = LoadC(&c.dist[(int64_t(u) << lg) + r]);
+    if (du >= c.hi) {
+      later |= 1ull << r;
+      continue;
+    }
+    act |= 1ull << r;
+    if (hub) continue;
+    for (int64_t k = b; k < e; k++) {
+      const int2 vw = c.wdst[k];
+      const int32_t v = vw.x;
+      const int64_t nd = static_cast<int64_t>(du) + vw.y;
+      int32_t* dv = &c.dist[(int64_t(v) << lg) + r];
+      bool push_near = false, push_fresh = false;
+      c.relaxed++;
+      if (nd < kDistInf && nd < *dv) {
+        const int32_t nd32 = static_cast<int32_t>(nd);
+        if (nd32 < atomicMin(dv, nd32)) {
+          c.improved++;
+          atomicOr(reinterpret_cast<unsigned long long*>(&c.chg[v]), 1ull << r);
+          if (nd < c.hi) {
+            push_near = atomicExch(&c.stamp[v], c.near_tag) != c.near_tag;
+          } else {
+            push_fresh = atomicExch(&c.freshstamp[v], c.fresh_tag) != c.fresh_tag;
+            if (nd32 < c.fmin) c.fmin = nd32;
+          }
+        }
+      }
+      LdsQueuePush(*c.lnear, push_near, v, c.near, &c.S->near_tail);
+      LdsQueuePush(*c.lfresh, push_fresh, v, c.fresh, &c.S->fresh_tail);
+    }
+    c.scanned += static_cast<unsigned long long>(e - b);
+  }
+  if (later) atomicOr(reinterpret_cast<unsigned long long*>(&c.chg[u]), static_cast<unsigned long long>(later));
+  if (hub && act) {
+    const int64_t nc = (e - b + kChunkE - 1) / kChunkE;
+    const int at = atomicAdd(&S->nchunks, static_cast<int>(nc));
+    for (int64_t x = 0; x < nc; x++) {
+      const int64_t cb = b + x * kChunkE;
+      chunks[at + x] = SbChunk{cb, cb + kChunkE < e ? cb + kChunkE : e, u, 0, act};
+    }
+  }
+}
+
+// Expands the frontier q[parity].  With groups of 8 or more lanes, a
+// vertex with few pending roots (at most G / 16, and at least one) would
+// leave most of its group idle -- on graphs whose roots rarely share a
+// bucket at a vertex (grids, uniform graphs) that is nearly every vertex --
+// so the block first reads its tile's pending masks: those vertices are
+// expanded one lane each, the others queued in LDS for a group each.
 __global__ __launch_bounds__(kBS) void SbRelax(const int64_t* __restrict__ off, const int2* __restrict__ wdst,
                                                int32_t* __restrict__ dist, uint64_t* __restrict__ chg,
                                                int32_t* __restrict__ stamp, int32_t* __restrict__ freshstamp,
@@ -281,6 +369,8 @@ __global__ __launch_bounds__(kBS) void SbRelax(const int64_t* __restrict__ off, 
   __shared__ LdsQueue<kQueueCap> lnear, lfresh;
   __shared__ unsigned long long s_red[kBS / kWave];
   __shared__ int32_t s_min[kBS / kWave];
+  __shared__ int32_t s_dense[kBS];
+  __shared__ int s_ndense;
   LdsQueueInit(lnear);
   LdsQueueInit(lfresh);
   const int parity = S->parity;
@@ -291,37 +381,36 @@ __global__ __launch_bounds__(kBS) void SbRelax(const int64_t* __restrict__ off, 
              S->fresh_tag, lg, 0, 0, 0, kDistInf};
   const Group g = MakeGroup(lg);
   const int groups = kWave >> lg;
-  const int64_t wave = (blockIdx.x * int64_t(kBS) + threadIdx.x) / kWave;
-  const int64_t waves = int64_t(gridDim.x) * (kBS / kWave);
   const int gi = LaneId() >> lg;
-  for (int64_t base = wave * groups; base < fsize; base += waves * groups) {
-    const int64_t i = base + gi;
-    if (i >= fsize) continue;  // group-uniform
-    const int32_t u = F[i];
-    unsigned long long m = 0;
-    if (LaneId() == g.leader) m = atomicExch(reinterpret_cast<unsigned long long*>(&chg[u]), 0ull);
-    m = __shfl(m, g.leader);
-    const bool mine = (m >> g.r) & 1;
-    const int32_t du = mine ? LoadC(&dist[(int64_t(u) << lg) + g.r]) : kDistInf;
-    const bool active = mine && du < c.hi;
-    const uint64_t later = GroupBallot(g, mine && !active);
-    const uint64_t act = GroupBallot(g, active);
-    if (later && LaneId() == g.leader)
-      atomicOr(reinterpret_cast<unsigned long long*>(&chg[u]), static_cast<unsigned long long>(later));
-    if (!act) continue;
-    const int64_t b = off[u], e = off[u + 1];
-    if (e - b >= kSerialMax) {
-      if (LaneId() == g.leader) {
-        const int64_t nc = (e - b + kChunkE - 1) / kChunkE;
-        const int at = atomicAdd(&S->nchunks, static_cast<int>(nc));
-        for (int64_t x = 0; x < nc; x++) {
-          const int64_t cb = b + x * kChunkE;
-          chunks[at + x] = SbChunk{cb, cb + kChunkE < e ? cb + kChunkE : e, u, 0, act};
-        }
+  if (lg >= 3) {
+    const int sparse_max = lg >= 4 ? 1 << (lg - 4) : 1;
+    const int wid = threadIdx.x / kWave;
+    for (int64_t tile = int64_t(blockIdx.x) * kBS; tile < fsize; tile += int64_t(gridDim.x) * kBS) {
+      if (threadIdx.x == 0) s_ndense = 0;
+      __syncthreads();
+      const int64_t i = tile + threadIdx.x;
+      if (i < fsize) {
+        const int32_t u = F[i];
+        const uint64_t pend =
+            __hip_atomic_load(reinterpret_cast<unsigned long long*>(&chg[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__popcll(pend) <= sparse_max)
+          LaneExpand(c, u, off, chunks, S, lg);
+        else
+          s_dense[atomicAdd(&s_ndense, 1)] = u;
       }
-      continue;
+      __syncthreads();
+      const int nd = s_ndense;
+      for (int j = wid * groups + gi; j < nd; j += (kBS / kWave) * groups) GroupExpand(c, g, s_dense[j], off, chunks, S, lg);
+      __syncthreads();
     }
-    RelaxRange(c, g, u, b, e, active, du);
+  } else {
+    const int64_t wave = (blockIdx.x * int64_t(kBS) + threadIdx.x) / kWave;
+    const int64_t waves = int64_t(gridDim.x) * (kBS / kWave);
+    for (int64_t base = wave * groups; base < fsize; base += waves * groups) {
+      const int64_t i = base + gi;
+      if (i >= fsize) continue;  // group-uniform
+      GroupExpand(c, g, F[i], off, chunks, S, lg);
+    }
   }
   RelaxFlush(c, s_red, s_min);
 }
