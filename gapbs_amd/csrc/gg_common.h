@@ -107,7 +107,7 @@ struct gg_graph {
   int32_t* fold_dst = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
-  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}, {0, 0}, 0};  // gg_autotune results
+  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}, {0, 0}, 0, 0};  // gg_autotune results
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
   // Jacobi PageRank source-segment layout, one per precision (gg_pr.hip):
@@ -213,6 +213,18 @@ const int32_t* UploadOrderLists(gg_graph* g);
 // Looks up, once per device, the device address of gg_cc_noalloc's fixed
 // scratch array on `device` (the current device); called by gg_init.
 void NoAllocSetup(int device);
+// The source-segment layout of the in-lists (gg_pr.hip) for 4-byte (fp64
+// false) or 8-byte (true) per-vertex values, built on first use; null when
+// the lists get none (small, unsorted or too sparse).  Its device arrays are
+// the workspace buffers SegArrays names.  Call outside timing.
+const gg_graph::PrSegments* SourceSegments(gg_graph* g, bool fp64);
+struct SegArrays {
+  const int32_t* src;   // sub-lists, segment-major
+  const int32_t* dst;   // destination of each pair
+  const int64_t* off;   // pair p's sub-list: src[off[p], off[p + 1])
+  const int32_t* tail;  // per vertex: where its list continues past the segments (from in_off[v])
+};
+SegArrays SourceSegmentArrays(gg_graph* g, const gg_graph::PrSegments* L, bool fp64);
 
 // Direction-optimizing BFS (gg_bfs.hip) from device id `source` on
 // g->stream, inside the caller's timed region: parents in the workspace

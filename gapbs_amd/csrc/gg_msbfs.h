@@ -74,6 +74,7 @@ struct MsCounters {
   unsigned long long scout[kStripes];     // out-degree of vertices with new bits
   unsigned long long examined[kStripes];  // adjacency entries read (top-down, light bottom-up)
   unsigned long long heavy_ex[kStripes];  // adjacency entries read by the heavy tier
+  unsigned long long seg_ex[kStripes];    // adjacency entries read by the segment pass
   unsigned long long finished[kStripes];  // in-edges of vertices that became reached by all roots
   unsigned long long live[kStripes];      // OR of the new bits
 };
@@ -94,10 +95,12 @@ struct MsLevel {
   int32_t listed;      // the current frontier is listed in the active list
   int32_t clear;       // to clear in the array just consumed: 0 nothing, 1 its listed entries, 2 all
   int32_t done;        // no frontier left
+  int32_t seg;         // this (bottom-up) level gathers through the source segments first
+  int64_t seg_min;     // ... when more in-edges than this are unfinished
 };
 
 // Mail posted by MsStep for the host-driven loop.
-enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailValues };
+enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailSegEx, kMailValues };
 
 __device__ __forceinline__ uint64_t WaveOr(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
@@ -120,8 +123,10 @@ __device__ __forceinline__ unsigned long long BlockOr(unsigned long long v, unsi
 
 // Starts the level state after the roots (their distinct vertices are the
 // first active list).
-__global__ void MsLevelInit(MsCounters* c, MsLevel* L, int64_t unfinished, uint64_t full) {
+__global__ void MsLevelInit(MsCounters* c, MsLevel* L, int64_t unfinished, uint64_t full, int64_t seg_min) {
   if (threadIdx.x == 0) {
+    L->seg = 0;
+    L->seg_min = seg_min;
     const int64_t na = c->active_tail;
     c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
     L->na = na;
@@ -146,15 +151,16 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
                        unsigned long long seq) {
   if (threadIdx.x == 0) {
     if (!L->done) {
-      unsigned long long a = 0, s = 0, ex = 0, hx = 0, fin = 0, lv = 0;
+      unsigned long long a = 0, s = 0, ex = 0, hx = 0, sx = 0, fin = 0, lv = 0;
       for (int i = 0; i < kStripes; i++) {
         a += c->awake[i];
         s += c->scout[i];
         ex += c->examined[i];
         hx += c->heavy_ex[i];
+        sx += c->seg_ex[i];
         fin += c->finished[i];
         lv |= c->live[i];
-        c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->finished[i] = c->live[i] = 0;
+        c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->seg_ex[i] = c->finished[i] = c->live[i] = 0;
       }
       const int64_t na_next = static_cast<int64_t>(a) + c->active_tail;
       c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
@@ -163,9 +169,10 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
         L->bu++;
       else
         L->td++;
-      L->examined += static_cast<int64_t>(ex + hx);
+      L->examined += static_cast<int64_t>(ex + hx + sx);
       L->unfinished -= static_cast<int64_t>(fin);
       const int next_bu = static_cast<int64_t>(s) > L->unfinished / alpha;
+      L->seg = next_bu && L->unfinished > L->seg_min;
       L->clear = (!keep && na_next > 0 && !next_bu) ? (L->listed ? 1 : 2) : 0;
       L->na_prev = L->na;
       L->listed = !was_bu;
@@ -178,11 +185,11 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
         const unsigned long long vals[kMailValues] = {static_cast<unsigned long long>(na_next),
                                                       static_cast<unsigned long long>(next_bu),
                                                       static_cast<unsigned long long>(L->done), ex, hx,
-                                                      static_cast<unsigned long long>(L->level)};
+                                                      static_cast<unsigned long long>(L->level), sx};
         MailPost(mb, vals, kMailValues, seq);
       }
     } else if (mb) {
-      const unsigned long long vals[kMailValues] = {0, 0, 1, 0, 0, static_cast<unsigned long long>(L->level)};
+      const unsigned long long vals[kMailValues] = {0, 0, 1, 0, 0, static_cast<unsigned long long>(L->level), 0};
       MailPost(mb, vals, kMailValues, seq);
     }
   }
@@ -409,8 +416,12 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
                                                   const int64_t* __restrict__ out_off,
                                                   const uint64_t* __restrict__ frontier, uint64_t* __restrict__ seen,
                                                   uint64_t* __restrict__ fnext, uint16_t* depth, int64_t n,
-                                                  const MsLevel* L, uint64_t full, MsCounters* c) {
+                                                  const MsLevel* L, uint64_t full, MsCounters* c,
+                                                  const int32_t* __restrict__ tail, uint64_t* __restrict__ acc) {
   if (L->done || !L->bottom_up) return;
+  // After a segment pass a light vertex's list continues at tail[i], with
+  // the bits found so far in acc[i] (taken here).
+  const bool seg = L->seg && tail;
   const uint16_t level = static_cast<uint16_t>(L->level);
   const uint64_t live = L->live;
   __shared__ int64_t s_beg[kBS];
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
     const int64_t i = base + tid;
     uint64_t s = 0;
     int64_t b = 0;
-    int d = 0;
+    int d = 0, deg_full = 0;
     bool light = false;
     if (i < n) {
       s = seen[i];
@@ -435,18 +446,28 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
         const int64_t deg = in_off[i + 1] - b;
         if (deg < kHeavyEdges) {
           light = true;
-          d = static_cast<int>(deg);
+          d = deg_full = static_cast<int>(deg);
         }
       }
     }
     s_got[tid] = 0;
+    if (seg && light) {
+      const int t = tail[i];
+      b += t;
+      d -= t;
+      const uint64_t a = acc[i];
+      if (a) {
+        s_got[tid] = a;
+        acc[i] = 0;
+      }
+    }
     const uint64_t need = live & ~s;
     // Rounds over list positions [kRoundEnd[r-1], kRoundEnd[r]): a vertex
     // takes part in a round only while it still lacks a needed bit (lists
     // are sorted, so in a degree-ordered layout the hubs come first).
     for (int round = 0, r_lo = 0; round < kRounds; r_lo = kRoundEnd[round], round++) {
       int take = 0;
-      if (light && r_lo < d && (round == 0 || (s_got[tid] & need) != need))
+      if (light && r_lo < d && (s_got[tid] & need) != need)
         take = (d < kRoundEnd[round] ? d : kRoundEnd[round]) - r_lo;
       s_beg[tid] = b + r_lo;
       int total = 0;
@@ -494,7 +515,7 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
       if (nb) {
         seen[i] = s | nb;
         scout += static_cast<unsigned long long>(out_off[i + 1] - out_off[i]);
-        if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(d);
+        if (((s | nb) & full) == full) fin += static_cast<unsigned long long>(deg_full);
         lv |= nb;
         awake++;
       }
@@ -552,8 +573,10 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
                                                      const uint64_t* __restrict__ frontier,
                                                      const uint64_t* __restrict__ seen, uint64_t* acc, const MsLevel* L,
                                                      const EdgeChunk* __restrict__ chunks, int32_t nch,
-                                                     MsCounters* c) {
+                                                     MsCounters* c, const int64_t* __restrict__ in_off,
+                                                     const int32_t* __restrict__ tail) {
   if (L->done || !L->bottom_up) return;
+  const bool seg = L->seg && tail;  // the lists continue at in_off[u] + tail[u]
   const uint64_t live = L->live;
   __shared__ unsigned long long s_red[kBS / kWave];
   const int lane = LaneId();
@@ -568,8 +591,14 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
     uint64_t before = lane == 0 ? acc[ch.u] : 0;
     before = __shfl(before, 0);
     if ((before & need) == need) continue;
+    int64_t beg = ch.beg;
+    if (seg) {
+      const int64_t from = in_off[ch.u] + tail[ch.u];
+      if (from >= ch.end) continue;
+      if (from > beg) beg = from;
+    }
     uint64_t got = 0;
-    for (int64_t base = ch.beg; base < ch.end; base += kWave * kUnroll) {
+    for (int64_t base = beg; base < ch.end; base += kWave * kUnroll) {
       int32_t nbr[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
@@ -640,6 +669,152 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
     if (fin) atomicAdd(&c->finished[slot], fin);
     if (lv) atomicOr(&c->live[slot], lv);
   }
+}
+
+// Segment pass of a bottom-up level (when MsStep set L->seg): the in-entries
+// regrouped by source segment of 2^18 vertices (2 MB of frontier masks, the
+// source-segment layout of gg_pr.hip), pair by pair in segment order, so the
+// masks gathered at any moment lie in one or two segments and stay in L2.
+// Each (destination, sub-list) pair of a vertex that still needs live bits
+// ORs its sources' masks into acc[u], stopping once acc holds every needed
+// bit; the light and heavy tiers then take the rest of each list from
+// tail[u].  One launch per segment, so a vertex has at most one pair in
+// flight and acc needs no atomics.  Work items ("virtual blocks", handed
+// out grid-stride) come from the parts table: per segment, its block pairs
+// (>= 1024 entries, a block each), wavefront pairs (32..1023, four per
+// item) and thread pairs (< 32, 256 per item).
+struct MsSegPart {
+  int64_t first_vb;  // first work item
+  int64_t p0;        // first pair
+  int64_t count;     // pairs
+  int32_t kind;      // 0 block, 1 wavefront, 2 thread per pair
+  int32_t pad;
+};
+constexpr int kSegBlockPair = 1024;
+constexpr int kSegWavePair = 32;
+
+// ORs the masks of src[b, e) (this wavefront's share: positions b + lane,
+// stepping by `step`) while u still lacks a needed bit; wave-uniform b, e.
+__device__ __forceinline__ uint64_t MsSegWaveOr(const int32_t* __restrict__ src, const uint64_t* __restrict__ frontier,
+                                                const uint64_t* acc, int32_t u, uint64_t need, int64_t b, int64_t e,
+                                                int step, unsigned long long* ex) {
+  const int lane = LaneId();
+  uint64_t got = 0;
+  for (int64_t base = b; base < e; base += int64_t(step) * kUnroll) {
+    int32_t nbr[kUnroll];
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++) {
+      const int64_t k = base + int64_t(q) * step + lane;
+      nbr[q] = k < e ? src[k] : -1;
+    }
+    uint64_t f = 0;
+#pragma unroll
+    for (int q = 0; q < kUnroll; q++)
+      if (nbr[q] >= 0) f |= frontier[nbr[q]];
+    got |= WaveOr(f);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < kUnroll; q++) {
+        const int64_t k0 = base + int64_t(q) * step;
+        if (k0 < e) *ex += static_cast<unsigned long long>(e - k0 < kWave ? e - k0 : kWave);
+      }
+    }
+    uint64_t other = lane == 0 ? acc[u] : 0;
+    other = __shfl(other, 0);
+    if (((got | other) & need) == need) break;
+  }
+  return got;
+}
+
+__global__ __launch_bounds__(kBS) void MsSegOr(const MsSegPart* __restrict__ parts, int nparts, int64_t vb0,
+                                               int64_t vb1, const int32_t* __restrict__ src,
+                                               const int32_t* __restrict__ pdst, const int64_t* __restrict__ poff,
+                                               const uint64_t* __restrict__ frontier,
+                                               const uint64_t* __restrict__ seen, uint64_t* acc, const MsLevel* L,
+                                               MsCounters* c) {
+  if (L->done || !L->bottom_up || !L->seg) return;
+  const uint64_t live = L->live;
+  __shared__ unsigned long long s_red[kBS / kWave];
+  __shared__ unsigned long long s_or[kBS / kWave];
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  unsigned long long ex = 0;
+  for (int64_t vb = vb0 + blockIdx.x; vb < vb1; vb += gridDim.x) {
+    int lo = 0, hi = nparts;  // parts[lo].first_vb <= vb < parts[hi].first_vb
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (parts[mid].first_vb <= vb)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const MsSegPart P = parts[lo];
+    const int64_t rel = vb - P.first_vb;
+    if (P.kind == 2) {  // a thread per pair
+      const int64_t t = rel * kBS + threadIdx.x;
+      if (t < P.count) {
+        const int64_t p = P.p0 + t;
+        const int32_t u = pdst[p];
+        const uint64_t need = live & ~seen[u];
+        if (need && (acc[u] & need) != need) {
+          const int64_t b = poff[p], e = poff[p + 1];
+          uint64_t got = 0;
+          int64_t k = b;
+          for (; k + kUnroll <= e; k += kUnroll) {
+            int32_t nbr[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; q++) nbr[q] = src[k + q];
+#pragma unroll
+            for (int q = 0; q < kUnroll; q++) got |= frontier[nbr[q]];
+          }
+          for (; k < e; k++) got |= frontier[src[k]];
+          ex += static_cast<unsigned long long>(e - b);
+          if (got & need) acc[u] |= got & need;
+        }
+      }
+    } else if (P.kind == 1) {
+      // A wavefront per pair, four pairs per item.  Wave-uniform from here.
+      const int64_t w = rel * (kBS / kWave) + wid;
+      if (w < P.count) {
+        const int64_t p = P.p0 + w;
+        const int32_t u = pdst[p];
+        const uint64_t need = live & ~seen[u];
+        uint64_t before = lane == 0 ? acc[u] : 0;
+        before = __shfl(before, 0);
+        if (need && (before & need) != need) {
+          unsigned long long wex = 0;
+          const uint64_t got = MsSegWaveOr(src, frontier, acc, u, need, poff[p], poff[p + 1], kWave, &wex);
+          if (lane == 0) {
+            ex += wex;
+            if (got & need) acc[u] = before | (got & need);
+          }
+        }
+      }
+    } else {
+      // A block per pair, its wavefronts strided over the sub-list; the
+      // block's bits are combined in LDS and written once.  Block-uniform.
+      const int64_t p = P.p0 + rel;
+      const int32_t u = pdst[p];
+      const uint64_t need = live & ~seen[u];
+      uint64_t before = lane == 0 ? acc[u] : 0;
+      before = __shfl(before, 0);
+      uint64_t got = 0;
+      if (need && (before & need) != need) {
+        unsigned long long wex = 0;
+        got = MsSegWaveOr(src, frontier, acc, u, need, poff[p] + wid * kWave, poff[p + 1], kBS, &wex);
+        if (lane == 0) ex += wex;
+      }
+      if (lane == 0) s_or[wid] = got;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint64_t all = 0;
+        for (int q = 0; q < kBS / kWave; q++) all |= s_or[q];
+        if (all & need) acc[u] |= all & need;
+      }
+      __syncthreads();
+    }
+  }
+  ex = BlockSum<kBS>(ex, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&c->seg_ex[blockIdx.x & (kStripes - 1)], ex);
 }
 
 // Dense frontier -> list, before a top-down level that follows a bottom-up
@@ -716,10 +891,16 @@ struct MsBuffers {
   EdgeChunk* chunks;  // null: no chunking (no vertex reaches kExpandBS edges)
   uint16_t* depth16;
   uint64_t full;
+  // Source segments for the bottom-up levels that read most in-edges (all
+  // null when the graph has none).
+  const MsSegPart* seg_parts;
+  int seg_nparts;
+  const std::vector<int64_t>* seg_vb;  // segment s owns work items [seg_vb[s], seg_vb[s + 1])
+  SegArrays seg;
 };
 
 struct MsLaunched {
-  int td = -1, bu = -1, heavy = -1;  // profile records to credit once the counts are known
+  int td = -1, bu = -1, heavy = -1, seg = -1;  // profile records to credit once the counts are known
   int kernels = 0;
 };
 
@@ -758,15 +939,29 @@ inline MsLaunched MsLaunchLevel(gg_graph* g, const MsBuffers& b, const uint64_t*
     r.kernels++;
   }
   if (bu) {
-    // Per vertex: seen (8 B) and fnext (8 B); per in-edge gathered: 4 B
-    // and the neighbour's 8 B frontier mask (credited by the caller).
+    // Per in-edge gathered: 4 B and the neighbour's 8 B frontier mask
+    // (credited by the caller).
+    if (b.seg_parts) {
+      const std::vector<int64_t>& vb = *b.seg_vb;
+      for (size_t sg = 0; sg + 1 < vb.size(); sg++) {
+        if (vb[sg + 1] == vb[sg]) continue;
+        const int rec = Run(g, "msbfs_segments", 0, MsSegOr,
+                            dim3(GridFor(vb[sg + 1] - vb[sg], 1, g->cu_count * kLevelBlocksPerCu)), blk, b.seg_parts,
+                            b.seg_nparts, vb[sg], vb[sg + 1], b.seg.src, b.seg.dst, b.seg.off, fcur,
+                            static_cast<const uint64_t*>(b.seen), b.acc, static_cast<const MsLevel*>(b.L), b.dc);
+        if (r.seg < 0) r.seg = rec;  // the level's entries are credited to its first segment launch
+        r.kernels++;
+      }
+    }
+    // Per vertex: seen (8 B) and fnext (8 B).
     r.bu = Run(g, "msbfs_bottom_up", 16 * N, MsBottomUp, grid_n, blk, static_cast<const int64_t*>(g->in_off),
                static_cast<const int32_t*>(g->in_dst), static_cast<const int64_t*>(g->out_off), fcur, b.seen,
-               fnext, b.depth16, n, static_cast<const MsLevel*>(b.L), b.full, b.dc);
+               fnext, b.depth16, n, static_cast<const MsLevel*>(b.L), b.full, b.dc, b.seg.tail, b.acc);
     r.heavy = Run(g, "msbfs_bottom_up_heavy", 0, MsHeavyChunks,
                   dim3(GridFor(int64_t(b.nhchunks) * kWave, kBS, g->cu_count * kLevelBlocksPerCu)), blk,
                   static_cast<const int32_t*>(g->in_dst), fcur, static_cast<const uint64_t*>(b.seen), b.acc,
-                  static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc);
+                  static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc,
+                  static_cast<const int64_t*>(g->in_off), b.seg.tail);
     Run(g, "msbfs_heavy_finish", 32.0 * b.nheavy, MsHeavyFinish, dim3(GridFor(b.nheavy, kBS, g->cu_count * 4)), blk,
         b.heavy, b.nheavy, b.seen, b.acc, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
         static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, b.dc);
@@ -859,8 +1054,50 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   int32_t* touched = g->Scratch<int32_t>("ms.touched", n + 64);
   const bool use_chunks = g->max_degree >= kExpandBS;
   EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("ms.chunks", MaxChunks(n, g->out_entries)) : nullptr;
+  // Source segments (8-byte values: the frontier masks), built on first use;
+  // the parts table is rebuilt per call (a few hundred bytes).  Off unless
+  // asked for: at Kronecker 26 (64 roots, alpha 4) the segment pass took
+  // 20 ms of a 34.2 ms traversal against 32.0 ms without it, the light and
+  // heavy tiers' early exit already stopping most lists after a few entries.
+  const gg_graph::PrSegments* SL = g->tune.bfs_seg_pct > 0 ? SourceSegments(g, true) : nullptr;
+  MsSegPart* seg_parts = nullptr;
+  int seg_nparts = 0;
+  int64_t seg_nvb = 0, seg_min = INT64_MAX;
+  std::vector<int64_t> seg_vb;
+  SegArrays seg_arrays{nullptr, nullptr, nullptr, nullptr};
+  if (SL) {
+    std::vector<MsSegPart> parts;
+    for (int sg = 0; sg < SL->ns; sg++) {
+      seg_vb.push_back(seg_nvb);
+      const int64_t p0 = SL->first_pair[sg], nl = SL->large[sg], nm = SL->medium[sg];
+      const int64_t nsm = SL->first_pair[sg + 1] - p0 - nl - nm;
+      if (nl) {
+        parts.push_back(MsSegPart{seg_nvb, p0, nl, 0, 0});
+        seg_nvb += nl;
+      }
+      if (nm) {
+        parts.push_back(MsSegPart{seg_nvb, p0 + nl, nm, 1, 0});
+        seg_nvb += (nm + kBS / kWave - 1) / (kBS / kWave);
+      }
+      if (nsm) {
+        parts.push_back(MsSegPart{seg_nvb, p0 + nl + nm, nsm, 2, 0});
+        seg_nvb += (nsm + kBS - 1) / kBS;
+      }
+    }
+    seg_vb.push_back(seg_nvb);
+    if (seg_nvb > 0) {
+      seg_nparts = static_cast<int>(parts.size());
+      seg_parts = g->Scratch<MsSegPart>("ms.seg_parts", parts.size());
+      GG_HIP(hipMemcpyAsync(seg_parts, parts.data(), sizeof(MsSegPart) * parts.size(), hipMemcpyHostToDevice, st));
+      seg_arrays = SourceSegmentArrays(g, SL, true);
+      const int pct = g->tune.bfs_seg_pct;
+      seg_min = g->in_entries * pct / 100;
+      if (pct == 100) seg_min = g->in_entries - 1;
+    }
+  }
   const MsBuffers b{dc,   dl,  heavy,   hchunks, static_cast<int32_t>(g->ms_heavy), static_cast<int32_t>(g->ms_heavy_chunks),
-                    seen, acc, touched, chunks,  depth16,                            full};
+                    seen, acc, touched, chunks,  depth16,                            full,
+                    seg_parts, seg_nparts, &seg_vb, seg_arrays};
   const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
   const double N = static_cast<double>(n);
   int64_t& launches = res.launches;
@@ -869,7 +1106,7 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   Run(g, "msbfs_init", 32 * N, MsInit, grid_n, blk, seen, fa, fb, acc, n, dc);
   Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), droots, dbits, count, seen, fa, depth16,
       act_a, dc);
-  Run(g, "msbfs_level_init", 0, MsLevelInit, dim3(1), dim3(64), dc, dl, g->in_entries, full);
+  Run(g, "msbfs_level_init", 0, MsLevelInit, dim3(1), dim3(64), dc, dl, g->in_entries, full, seg_min);
   launches += 3;
 
   MsLevel hl;
@@ -881,7 +1118,10 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
         reinterpret_cast<uintptr_t>(seen),    reinterpret_cast<uintptr_t>(acc),     reinterpret_cast<uintptr_t>(touched),
         reinterpret_cast<uintptr_t>(chunks),  reinterpret_cast<uintptr_t>(hchunks), reinterpret_cast<uintptr_t>(heavy),
         reinterpret_cast<uintptr_t>(depth16), static_cast<uintptr_t>(full),         static_cast<uintptr_t>(alpha),
-        static_cast<uintptr_t>(g->ms_heavy),  static_cast<uintptr_t>(g->ms_heavy_chunks)};
+        static_cast<uintptr_t>(g->ms_heavy),  static_cast<uintptr_t>(g->ms_heavy_chunks),
+        reinterpret_cast<uintptr_t>(seg_parts), static_cast<uintptr_t>(seg_nparts), static_cast<uintptr_t>(seg_nvb),
+        reinterpret_cast<uintptr_t>(seg_arrays.src), reinterpret_cast<uintptr_t>(seg_arrays.dst),
+        reinterpret_cast<uintptr_t>(seg_arrays.off), reinterpret_cast<uintptr_t>(seg_arrays.tail)};
     int per_replay = 0;
     if (!g->ms_exec || g->ms_exec_key != key) {
       if (g->ms_exec) {
@@ -961,6 +1201,7 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
       if (r.td >= 0) CreditBytes(g, r.td, 20.0 * mail[kMailExamined]);
       if (r.bu >= 0) CreditBytes(g, r.bu, 12.0 * mail[kMailExamined]);
       if (r.heavy >= 0) CreditBytes(g, r.heavy, 12.0 * mail[kMailHeavyEx]);
+      if (r.seg >= 0) CreditBytes(g, r.seg, 12.0 * mail[kMailSegEx]);
       const int64_t na_next = static_cast<int64_t>(mail[kMailNa]);
       const bool next_bu = mail[kMailBottomUp] != 0;
       // The clears MsStep chose for the array just consumed.

@@ -960,6 +960,19 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
 }
 
 }  // namespace
+
+const gg_graph::PrSegments* SourceSegments(gg_graph* g, bool fp64) {
+  return fp64 ? Segments<double>(g) : Segments<float>(g);
+}
+
+SegArrays SourceSegmentArrays(gg_graph* g, const gg_graph::PrSegments* L, bool fp64) {
+  const std::string tag = fp64 ? "pr.seg64." : "pr.seg32.";
+  return SegArrays{g->Scratch<int32_t>((tag + "src").c_str(), L->hot_entries),
+                   g->Scratch<int32_t>((tag + "dst").c_str(), L->pairs),
+                   g->Scratch<int64_t>((tag + "off").c_str(), L->pairs + 1),
+                   g->Scratch<int32_t>((tag + "tail").c_str(), g->n)};
+}
+
 }  // namespace gg
 
 using namespace gg;

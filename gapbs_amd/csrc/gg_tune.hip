@@ -191,6 +191,8 @@ extern "C" int gg_tuning_set(gg_graph* g, const gg_tuning* in) {
     }
     GG_REQUIRE(in->pr_seg_min_density >= 0 && in->pr_seg_min_density <= 64, GG_EINVAL,
                "PageRank segment density out of range");
+    GG_REQUIRE(in->bfs_seg_pct >= -1 && in->bfs_seg_pct <= 100, GG_EINVAL,
+               "batched BFS segment threshold out of range");
     const bool bins_changed =
         in->pr_small_max != g->tune.pr_small_max || in->pr_medium_max != g->tune.pr_medium_max;
     g->tune = *in;

@@ -321,6 +321,12 @@ typedef struct gg_tuning {
    * are at least this many per (destination, sub-list) pair; 0 picks the
    * default (3), else 1..64. */
   int32_t pr_seg_min_density;
+  /* Batched BFS: a bottom-up level gathers the frontier masks through the
+   * 8-byte source-segment layout (the one FP64 PageRank uses) when more than
+   * this percentage of the in-edges belong to vertices some root has not
+   * reached; 0 and -1 never (the default: slower than the tiers' own early
+   * exit on the graphs measured), 1..100. */
+  int32_t bfs_seg_pct;
 } gg_tuning;
 enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };
 int gg_autotune(gg_graph* g, int what, const int32_t* roots, int k, gg_tuning* out);
