@@ -674,7 +674,8 @@ def main():
             run_kernel(k)
         if k == "bc" and warm == 0:  # allocations and the fold lists, outside the timed step
             gpu.bc(dg, roots[:1], want_scores=False)
-        if k in ("pr", "pr_fp64"):  # the source-segment layout is built once per graph, outside timing
+        if k in ("pr", "pr_fp64"):  # the Jacobi layout is built once per graph, outside timing
+            gpu.pagerank_blocks(dg, fp64=(k == "pr_fp64"))
             gpu.pagerank_layout(dg, fp64=(k == "pr_fp64"))
         last = None
         t = time.perf_counter()
@@ -710,8 +711,10 @@ def main():
             assert last["reached_edges"] // 2 == m_total, f"{k}: reach differs from the single-root runs"
         if k in ("pr", "pr_fp64", "cc_sv") and last:
             r["iterations"] = last["iterations"]
-        if k in ("pr", "pr_fp64"):  # the source segments the Jacobi gathers went through (built in the warmup)
-            r["layout"] = gpu.pagerank_layout(dg, fp64=(k == "pr_fp64"))
+        if k in ("pr", "pr_fp64"):  # the layout the Jacobi iterations went through (built before timing)
+            blocks = gpu.pagerank_blocks(dg, fp64=(k == "pr_fp64"))
+            r["layout"] = ({"propagation_blocks": blocks} if blocks["width_log2"] else
+                           {"source_segments": gpu.pagerank_layout(dg, fp64=(k == "pr_fp64"))})
         if k == "tc" and last is not None:
             r["probes"] = last["edges_examined"]
             r["triangles"] = last.get("triangles")

@@ -107,7 +107,7 @@ struct gg_graph {
   int32_t* fold_dst = nullptr;
   int64_t max_degree = 0;  // max out-degree (in-degree for the inverse), set at upload
   int64_t no_in_count = -1;  // vertices without in-neighbours (computed on first BFS)
-  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}, {0, 0}, 0, 0};  // gg_autotune results
+  gg_tuning tune = {15, 18, 15, 32, 32, 1024, 16, 0, {0, 0, 0, 0}, {0, 0}, {0, 0}, 0, 0, 0};  // gg_autotune results
   int64_t pr_bins[3] = {-1, -1, -1};     // PageRank in-degree bins (vertex counts)
   int64_t pr_bin_edges[3] = {0, 0, 0};   // and their in-edge counts
   // Jacobi PageRank source-segment layout, one per precision (gg_pr.hip):
@@ -129,6 +129,18 @@ struct gg_graph {
     std::vector<int64_t> entries;            // its in-entries
     double build_ms = 0;
   } pr_seg[2];
+  // Jacobi PageRank propagation-block layout (gg_pr.hip), shared by both
+  // precisions: in-entries grouped by (source tile, destination bin) of 2^lg
+  // vertices each; device arrays in the workspace ("pr.pb.*").
+  struct PrBlocks {
+    bool built = false;
+    int lg = 0;                       // log2 of the tile / bin width; 0: no blocks
+    int64_t k = 0;                    // tiles (= bins)
+    int64_t octets = 0;               // padded entries / 8
+    int64_t n_write = 0, n_sum = 0;   // chunks of the write and sum phases
+    int64_t n_multi = 0, slots = 0;   // bins summed in several chunks, their partial-sum slots
+    double build_ms = 0;
+  } pr_blocks;
   int64_t cc_samples_n = -1;             // n the uploaded Afforest sample positions are for
   int64_t ms_heavy = -1;                 // batched BFS: heavy vertices (listed once per graph)
   int64_t ms_heavy_chunks = 0;           // and their in-edge chunks

@@ -14,6 +14,12 @@
 // vertex's new score and, for the next iteration, its new contribution into a
 // second buffer, so the reference's separate contrib pass is fused away.
 //
+// Jacobi layouts.  On graphs of 2^20 to 2^27 vertices the iterations run
+// through propagation blocks (section "Propagation blocks" below): two
+// streaming phases per iteration, no random reads, so the bytes moved scale
+// with the element size.  Source segments (next paragraph) are the layout
+// when a segment width is set explicitly or blocks are turned off.
+//
 // Source segments (Jacobi modes).  A pull gather reads contrib[v] for every
 // in-entry at random.  Measured on the MI355X, random 4-byte gathers run at
 // about 1350 G/s from LDS, 200 G/s from an L2-resident range and 52 G/s from
@@ -36,9 +42,11 @@
 // Kronecker 26 they measured slower than L2 segments (their per-pair
 // gathers are latency-bound at 16 waves per CU), so they are off by default.
 // Two precisions:
-//   GG_PR_FP32  float scores and contributions, each vertex's sum kept as a
-//               compensated (Neumaier) pair and merged across lanes with
-//               TwoSum, so the sum carries about twice float's precision;
+//   GG_PR_FP32  float scores and contributions; in the gathers each vertex's
+//               sum is kept as a compensated (Neumaier) pair and merged
+//               across lanes with TwoSum, so the sum carries about twice
+//               float's precision; through propagation blocks it is exact
+//               (64-bit fixed point), rounded to float once;
 //   GG_PR_FP64  double scores, contributions and sums.
 // The convergence residual is always accumulated in double.
 #include "gg_common.h"
@@ -835,6 +843,451 @@ const gg_graph::PrSegments* Segments(gg_graph* g) {
   return &L;
 }
 
+// ---------------------------------------------------------------------------
+// Propagation blocks (Jacobi modes; the default on graphs of 2^20 to 2^27
+// vertices).
+//
+// After Beamer, Asanovic and Patterson, "Reducing PageRank Communication via
+// Propagation Blocking" (IPDPS 2017).  A pull gather reads contrib[v] at
+// random for every in-entry, and a random read costs one cache-line
+// transaction whether it wants 4 or 8 bytes.  Here an iteration instead
+// writes every in-entry's contribution into a value array whose order lets
+// both phases stream.  The in-entries are grouped into blocks (t, b):
+// source in tile t, destination in bin b, both 2^lg consecutive ids.  The
+// blocks are stored tile-major, and each block's entries are kept together
+// by destination and padded to a multiple of 8 (an octet) with (source
+// offset 0, destination 0xFFFF).  Ids are stored as 16-bit offsets within
+// their tile or bin.
+//   write  a block of threads per chunk of one tile's octets: the tile's
+//          contributions are staged in LDS, then the chunk's source offsets
+//          (2 B per entry) are streamed and each entry's contribution
+//          (4 or 8 B) is written in the same order;
+//   sum    a block per chunk of one bin's octets, read through an octet
+//          index kept bin-major (4 B per octet): each value is added to its
+//          destination's accumulator in LDS, then the bin's vertices are
+//          finished (or, for a bin split into several chunks, the chunk's
+//          partial sums are stored);
+//   merge  for the bins of several chunks: their partial sums are merged
+//          in chunk order and the vertices finished.
+// Bytes per in-entry: 2 + 4 + 0.5 + 2 + 4 = 12.5 in FP32 and 20.5 in FP64,
+// so the contribution traffic scales with the element size.
+// Accumulators (8 bytes per destination in LDS, added to with atomics by
+// the block's lanes):
+//   FP32  an exact sum in 64-bit fixed point, units of 2^-63.  Every
+//         contribution is score / out-degree <= 1, and a vertex's sum is at
+//         most the sum of all scores, which PageRank without dangling
+//         redistribution keeps at or below 1, so sums fit below 2.  A value
+//         is rounded once, to the nearest unit (an absolute error of at
+//         most 2^-64 per entry), and the rest is integer arithmetic:
+//         exact, the same in any order, and one 64-bit atomic per run of
+//         equal destinations.  The finished sum is rounded to float once.
+//         (A compensated float pair, the other FP32 paths' accumulator,
+//         would take two atomics per run here, one of them returning the
+//         old value for its TwoSum, and measured 2.2x slower at Kronecker
+//         scale 26.)
+//   FP64  a double, added in arrival order.
+constexpr int kPbMaxLog = 14;      // 2^14 accumulators of 8 bytes fill 128 KB of LDS
+constexpr int kPbMaxTiles = 8192;  // build tables hold k x k octet counts
+constexpr int kPbWriteBS = 1024;
+constexpr int kPbSumBS = 1024;
+constexpr int kPbBuildBS = 1024;
+constexpr uint32_t kPbPad = 0xFFFFu;
+
+// The per-destination accumulator word of each precision.
+template <class T>
+struct PbAcc;
+template <>
+struct PbAcc<float> {
+  using Word = unsigned long long;  // 64-bit fixed point, units of 2^-63
+};
+template <>
+struct PbAcc<double> {
+  using Word = double;
+};
+
+struct PbChunk {
+  uint32_t part;   // the tile (write phase) or bin (sum phase)
+  int32_t slot;    // sum phase: partial-sum slot, -1 when the chunk is the whole bin
+  uint32_t q0, q1;  // write: octets [q0, q1); sum: octet-index positions [q0, q1)
+};
+struct PbMulti {
+  uint32_t bin, slot0, nslots, pad;  // a bin summed in nslots chunks, slots slot0...
+};
+
+template <class T>
+struct PbArgs {
+  const T* contrib;                 // this iteration's contributions
+  T* val;                           // one value per (padded) entry, tile-major
+  const uint4* src16;               // per octet: 8 source offsets within the tile
+  const uint4* dst16;               // per octet: 8 destination offsets within the bin
+  const uint32_t* oct;              // each bin's octets, bin-major
+  typename PbAcc<T>::Word* part;    // partial sums, 2^lg per slot
+  const int64_t* out_off;
+  T* score;
+  T* contrib_out;
+  T base, damp;
+  double* residual;
+  int64_t n;
+  int lg;
+};
+
+// Entries per lane in the write phase: 4 floats or 2 doubles, so that each
+// lane stores 16 contiguous bytes and a wavefront's store is one contiguous
+// kilobyte.
+template <class T>
+struct PbWriteShape;
+template <>
+struct PbWriteShape<float> {
+  static constexpr int kEntries = 4, kItems = 1;  // entries per item, items in flight per thread
+  using Ids = uint2;
+  using Vals = float4;
+  __device__ static void Unpack(const uint2 w, uint32_t (&o)[4]) {
+    o[0] = w.x & 0xFFFFu, o[1] = w.x >> 16, o[2] = w.y & 0xFFFFu, o[3] = w.y >> 16;
+  }
+  __device__ static float4 Pack(const float (&v)[4]) { return make_float4(v[0], v[1], v[2], v[3]); }
+};
+template <>
+struct PbWriteShape<double> {
+  static constexpr int kEntries = 2, kItems = 2;
+  using Ids = uint32_t;
+  using Vals = double2;
+  __device__ static void Unpack(const uint32_t w, uint32_t (&o)[2]) { o[0] = w & 0xFFFFu, o[1] = w >> 16; }
+  __device__ static double2 Pack(const double (&v)[2]) { return make_double2(v[0], v[1]); }
+};
+
+__device__ __forceinline__ void LoadOctet(const float* val, int64_t q, float (&v)[8]) {
+  const float4* p = reinterpret_cast<const float4*>(val) + 2 * q;
+  const float4 a = p[0], b = p[1];
+  v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+}
+__device__ __forceinline__ void LoadOctet(const double* val, int64_t q, double (&v)[8]) {
+  const double2* p = reinterpret_cast<const double2*>(val) + 4 * q;
+  const double2 a = p[0], b = p[1], c = p[2], d = p[3];
+  v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y, v[4] = c.x, v[5] = c.y, v[6] = d.x, v[7] = d.y;
+}
+__device__ __forceinline__ void Unpack16(const uint4 w, uint32_t (&o)[8]) {
+  o[0] = w.x & 0xFFFFu, o[1] = w.x >> 16, o[2] = w.y & 0xFFFFu, o[3] = w.y >> 16;
+  o[4] = w.z & 0xFFFFu, o[5] = w.z >> 16, o[6] = w.w & 0xFFFFu, o[7] = w.w >> 16;
+}
+
+// A contribution (0 <= x <= 1) in fixed-point units of 2^-63, rounded to
+// the nearest unit; and a fixed-point sum back to float (one rounding).
+__device__ __forceinline__ unsigned long long ToFixed(float x) { return __float2ull_rn(__fmul_rn(x, 0x1p63f)); }
+__device__ __forceinline__ float FromFixed(unsigned long long w) { return __fmul_rn(__ull2float_rn(w), 0x1p-63f); }
+
+// Adds one octet's entries to the LDS accumulators: each run of equal
+// destinations is summed in registers and added at its last entry.
+// d: the 8 destination offsets (kPbPad: padding, only after real entries).
+__device__ __forceinline__ void PbAddOctet(unsigned long long* acc, const uint32_t (&d)[8], const float (&v)[8],
+                                           uint32_t m) {
+  unsigned long long run = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    if (d[i] == kPbPad) break;
+    run = (i > 0 && d[i] != d[i - 1] ? 0ull : run) + ToFixed(v[i]);
+    if (i == 7 || d[i + 1] != d[i]) atomicAdd(&acc[d[i] & m], run);
+  }
+}
+__device__ __forceinline__ void PbAddOctet(double* acc, const uint32_t (&d)[8], const double (&v)[8], uint32_t m) {
+  double run = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    if (d[i] == kPbPad) break;
+    run = i > 0 && d[i] != d[i - 1] ? v[i] : __dadd_rn(run, v[i]);
+    if (i == 7 || d[i + 1] != d[i]) atomicAdd(&acc[d[i] & m], run);
+  }
+}
+__device__ __forceinline__ float PbValue(unsigned long long w) { return FromFixed(w); }
+__device__ __forceinline__ double PbValue(double w) { return w; }
+
+template <class T>
+__device__ __forceinline__ double PbFinish(const PbArgs<T>& a, int64_t u, T sum) {
+  const T old = a.score[u];
+  const T now = Combine(a.base, a.damp, sum);
+  a.score[u] = now;
+  const int64_t deg = a.out_off[u + 1] - a.out_off[u];
+  a.contrib_out[u] = now / static_cast<T>(deg);
+  return fabs(static_cast<double>(now) - static_cast<double>(old));
+}
+
+template <class T>
+__global__ __launch_bounds__(kPbWriteBS) void PbWrite(PbArgs<T> a, const PbChunk* __restrict__ chunks) {
+  using S = PbWriteShape<T>;
+  constexpr int E = S::kEntries, U = S::kItems;
+  __shared__ T slice[1 << kPbMaxLog];
+  const PbChunk c = chunks[blockIdx.x];
+  const int64_t base = int64_t(c.part) << a.lg;
+  const int64_t cnt = min(int64_t(1) << a.lg, a.n - base);
+  for (int64_t i = threadIdx.x; i < cnt; i += kPbWriteBS) slice[i] = a.contrib[base + i];
+  __syncthreads();
+  const uint32_t m = (1u << a.lg) - 1;  // keeps every slice index below 2^lg
+  const typename S::Ids* ids = reinterpret_cast<const typename S::Ids*>(a.src16);
+  typename S::Vals* out = reinterpret_cast<typename S::Vals*>(a.val);
+  const int64_t i1 = int64_t(c.q1) * (8 / E);  // items of E entries
+  for (int64_t i = int64_t(c.q0) * (8 / E) + threadIdx.x; i < i1; i += int64_t(kPbWriteBS) * U) {
+    typename S::Ids w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i + u * kPbWriteBS < i1) w[u] = ids[i + u * kPbWriteBS];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i + u * kPbWriteBS >= i1) break;
+      uint32_t sid[E];
+      S::Unpack(w[u], sid);
+      T v[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) v[e] = slice[sid[e] & m];
+      out[i + u * kPbWriteBS] = S::Pack(v);
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kPbSumBS) void PbSum(PbArgs<T> a, const PbChunk* __restrict__ chunks) {
+  using Word = typename PbAcc<T>::Word;
+  __shared__ Word s_acc[1 << kPbMaxLog];
+  __shared__ double s_red[kPbSumBS / kWave];
+  const PbChunk c = chunks[blockIdx.x];
+  const int w = 1 << a.lg;
+  const uint32_t m = static_cast<uint32_t>(w - 1);
+  for (int i = threadIdx.x; i < w; i += kPbSumBS) s_acc[i] = Word(0);
+  __syncthreads();
+  for (int64_t q = int64_t(c.q0) + threadIdx.x; q < c.q1; q += kPbSumBS) {
+    const uint32_t o = a.oct[q];
+    uint32_t d[8];
+    Unpack16(a.dst16[o], d);
+    T v[8];
+    LoadOctet(a.val, o, v);
+    PbAddOctet(s_acc, d, v, m);
+  }
+  __syncthreads();
+  const int64_t first = int64_t(c.part) << a.lg;
+  const int cnt = static_cast<int>(min(int64_t(w), a.n - first));
+  double r = 0.0;
+  if (c.slot < 0) {
+    for (int i = threadIdx.x; i < cnt; i += kPbSumBS) r += PbFinish(a, first + i, PbValue(s_acc[i]));
+  } else {
+    Word* p = a.part + (int64_t(c.slot) << a.lg);
+    for (int i = threadIdx.x; i < cnt; i += kPbSumBS) p[i] = s_acc[i];
+  }
+  r = BlockSum<kPbSumBS>(r, s_red);
+  if (threadIdx.x == 0 && r != 0.0) atomicAdd(&a.residual[blockIdx.x & (kStripes - 1)], r);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBS) void PbMerge(PbArgs<T> a, const PbMulti* __restrict__ multi, int32_t n_multi) {
+  __shared__ double s_red[kBS / kWave];
+  const int64_t w = int64_t(1) << a.lg, total = int64_t(n_multi) << a.lg;
+  double r = 0.0;
+  for (int64_t j = blockIdx.x * int64_t(kBS) + threadIdx.x; j < total; j += int64_t(gridDim.x) * kBS) {
+    const PbMulti M = multi[j >> a.lg];
+    const int64_t i = j & (w - 1), u = (int64_t(M.bin) << a.lg) + i;
+    if (u >= a.n) continue;
+    typename PbAcc<T>::Word sum = a.part[(int64_t(M.slot0) << a.lg) + i];
+    for (uint32_t s = 1; s < M.nslots; s++) sum += a.part[(int64_t(M.slot0 + s) << a.lg) + i];  // in slot order
+    r += PbFinish(a, u, PbValue(sum));
+  }
+  AddResidual(a.residual, r, s_red);
+}
+
+// Build, one block per bin: the in-entries of the bin's vertices, a
+// wavefront per vertex, are cut into runs of equal source tile (64 entries
+// at a time; sorted lists give one run per tile).
+struct PbRun {
+  int32_t t;       // the lane's source tile (-1 past the list)
+  int first, len;  // its run: first lane, and length (valid in the run's first lane)
+  uint64_t starts;
+};
+__device__ __forceinline__ PbRun PbRunOf(int32_t src, bool valid, int lg, int nvalid) {
+  const int lane = LaneId();
+  PbRun r;
+  r.t = valid ? (src >> lg) : -1;
+  const int32_t tp = __shfl_up(r.t, 1);
+  const bool start = valid && (lane == 0 || tp != r.t);
+  r.starts = __ballot(start);
+  const uint64_t rest = lane == kWave - 1 ? 0 : (r.starts >> (lane + 1));
+  r.len = start ? (rest ? 1 + __builtin_ctzll(rest) : nvalid - lane) : 0;
+  const uint64_t upto = r.starts & ((uint64_t(2) << lane) - 1);
+  r.first = upto ? 63 - __builtin_clzll(upto) : 0;
+  return r;
+}
+
+// Octets per (tile, bin) block, stored tile-major (tm[t * k + b]) and
+// bin-major (bm[b * k + t]).
+__global__ __launch_bounds__(kPbBuildBS) void PbCount(const int64_t* __restrict__ in_off,
+                                                      const int32_t* __restrict__ in_dst, int64_t n, int lg,
+                                                      int64_t k, uint32_t* __restrict__ tm, uint32_t* __restrict__ bm) {
+  __shared__ uint32_t cnt[kPbMaxTiles];
+  const int64_t b = blockIdx.x;
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) cnt[t] = 0;
+  __syncthreads();
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  const int64_t u1 = min(n, (b + 1) << lg);
+  for (int64_t u = (b << lg) + wid; u < u1; u += kPbBuildBS / kWave) {
+    const int64_t e = in_off[u + 1];
+    for (int64_t k0 = in_off[u]; k0 < e; k0 += kWave) {
+      const bool valid = k0 + lane < e;
+      const PbRun r = PbRunOf(valid ? in_dst[k0 + lane] : 0, valid, lg, static_cast<int>(min<int64_t>(kWave, e - k0)));
+      if (r.len > 0) atomicAdd(&cnt[r.t], static_cast<uint32_t>(r.len));
+    }
+  }
+  __syncthreads();
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) {
+    const uint32_t o = (cnt[t] + 7) >> 3;
+    tm[t * k + b] = o;
+    bm[b * k + t] = o;
+  }
+}
+
+// Writes each entry's offsets at its block's next free position (a run at a
+// time, so a destination's entries of one tile stay together), then pads
+// each block to its octets.  run: the exclusive scan of tm.
+__global__ __launch_bounds__(kPbBuildBS) void PbEmit(const int64_t* __restrict__ in_off,
+                                                     const int32_t* __restrict__ in_dst, int64_t n, int lg, int64_t k,
+                                                     const uint32_t* __restrict__ run, uint16_t* __restrict__ src16,
+                                                     uint16_t* __restrict__ dst16) {
+  __shared__ uint32_t cur[kPbMaxTiles];
+  const int64_t b = blockIdx.x;
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) cur[t] = 0;
+  __syncthreads();
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  const uint32_t m = (1u << lg) - 1;
+  const int64_t u1 = min(n, (b + 1) << lg);
+  for (int64_t u = (b << lg) + wid; u < u1; u += kPbBuildBS / kWave) {
+    const int64_t e = in_off[u + 1];
+    for (int64_t k0 = in_off[u]; k0 < e; k0 += kWave) {
+      const bool valid = k0 + lane < e;
+      const int32_t s = valid ? in_dst[k0 + lane] : 0;
+      const PbRun r = PbRunOf(s, valid, lg, static_cast<int>(min<int64_t>(kWave, e - k0)));
+      uint32_t at = 0;
+      if (r.len > 0) at = atomicAdd(&cur[r.t], static_cast<uint32_t>(r.len));
+      at = __shfl(at, r.first);
+      if (valid) {
+        const int64_t pos = int64_t(run[int64_t(r.t) * k + b]) * 8 + at + (lane - r.first);
+        src16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(s) & m);
+        dst16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(u) & m);
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) {
+    const int64_t p1 = int64_t(run[t * k + b + 1]) * 8;
+    for (int64_t p = int64_t(run[t * k + b]) * 8 + cur[t]; p < p1; p++) {
+      src16[p] = 0;
+      dst16[p] = static_cast<uint16_t>(kPbPad);
+    }
+  }
+}
+
+// The bin-major octet index: bin b lists the octets of blocks (0, b),
+// (1, b), ... in turn.  bpos: the exclusive scan of bm.
+__global__ __launch_bounds__(kPbBuildBS) void PbOctIndex(int64_t k, const uint32_t* __restrict__ run,
+                                                         const uint32_t* __restrict__ bpos, uint32_t* __restrict__ oct) {
+  const int64_t b = blockIdx.x;
+  const int lane = LaneId(), wid = threadIdx.x / kWave;
+  for (int64_t t = wid; t < k; t += kPbBuildBS / kWave) {
+    const uint32_t r0 = run[t * k + b], len = run[t * k + b + 1] - r0, d0 = bpos[b * k + t];
+    for (uint32_t i = lane; i < len; i += kWave) oct[d0 + i] = r0 + i;
+  }
+}
+
+// The width the Jacobi run of this precision uses (0: no blocks).
+int BlockWidth(const gg_graph* g, bool fp64) {
+  const int knob = g->tune.pr_blocks_log2;
+  if (knob > 0) return knob;
+  if (knob < 0 || g->tune.pr_segment_log2[fp64 ? 1 : 0] != 0) return 0;
+  return g->n >= (int64_t(1) << 20) && g->n <= (int64_t(1) << 27) ? kPbMaxLog : 0;
+}
+
+// The propagation-block layout, built on first use (and again when the
+// width changes); null when the Jacobi run of this precision goes without.
+const gg_graph::PrBlocks* Blocks(gg_graph* g, bool fp64) {
+  const int lg = BlockWidth(g, fp64);
+  if (lg == 0) return nullptr;
+  gg_graph::PrBlocks& B = g->pr_blocks;
+  if (B.built && B.lg == lg) return &B;
+  const int64_t n = g->n, k = (n + (int64_t(1) << lg) - 1) >> lg;
+  GG_REQUIRE(k <= kPbMaxTiles, GG_EINVAL, "PageRank block width too small for this graph (more than 8192 tiles)");
+  // Octet positions are 32-bit: at most 7 padding entries per block.
+  GG_REQUIRE((g->in_entries + 7 * k * k) / 8 < (int64_t(1) << 32), GG_EINVAL, "too many in-entries for PageRank blocks");
+  g->ws.Drop("pr.pb.");
+  WallTimer wt;
+  B = gg_graph::PrBlocks{};
+  const bool prof = g->profile;
+  g->profile = false;  // the build is not part of any measured call
+  struct Restore {
+    gg_graph* g;
+    bool prof;
+    ~Restore() { g->profile = prof; }
+  } restore{g, prof};
+  hipStream_t st = g->stream;
+  const int64_t kk = k * k;
+  uint32_t* tm = g->Scratch<uint32_t>("pr.pbb.tm", kk + 1);
+  uint32_t* bm = g->Scratch<uint32_t>("pr.pbb.bm", kk + 1);
+  PbCount<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, n, lg, k, tm, bm);
+  GG_HIP(hipGetLastError());
+  DeviceScan<uint32_t>(g, "pr.pbb.st.", tm, kk, tm + kk);
+  DeviceScan<uint32_t>(g, "pr.pbb.sb.", bm, kk, bm + kk);
+  // Tile starts tm[t * k] and bin starts bm[b * k], t, b = 0..k (the last
+  // one is the total).
+  std::vector<uint32_t> tile_at(k + 1), bin_at(k + 1);
+  GG_HIP(hipMemcpy2DAsync(tile_at.data(), sizeof(uint32_t), tm, k * sizeof(uint32_t), sizeof(uint32_t), k + 1,
+                          hipMemcpyDeviceToHost, st));
+  GG_HIP(hipMemcpy2DAsync(bin_at.data(), sizeof(uint32_t), bm, k * sizeof(uint32_t), sizeof(uint32_t), k + 1,
+                          hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  const int64_t octets = tile_at[k];
+  GG_REQUIRE(bin_at[k] == tile_at[k], GG_EHIP, "PageRank block tables disagree");
+  uint16_t* src16 = g->Scratch<uint16_t>("pr.pb.src", 8 * octets + 8);
+  uint16_t* dst16 = g->Scratch<uint16_t>("pr.pb.dst", 8 * octets + 8);
+  uint32_t* oct = g->Scratch<uint32_t>("pr.pb.oct", octets + 1);
+  PbEmit<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, n, lg, k, tm, src16, dst16);
+  PbOctIndex<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(k, tm, bm, oct);
+  GG_HIP(hipGetLastError());
+  // Chunks of at most 4 * 2^lg octets: the write phase's per tile, the sum
+  // phase's per bin (every bin has one, so every vertex is finished).
+  const int64_t cap = int64_t(4) << lg;
+  std::vector<PbChunk> wc, sc;
+  std::vector<PbMulti> multi;
+  int64_t slots = 0;
+  for (int64_t t = 0; t < k; t++)
+    for (int64_t q = tile_at[t]; q < tile_at[t + 1]; q += cap)
+      wc.push_back(PbChunk{static_cast<uint32_t>(t), -1, static_cast<uint32_t>(q),
+                           static_cast<uint32_t>(std::min<int64_t>(q + cap, tile_at[t + 1]))});
+  for (int64_t b = 0; b < k; b++) {
+    const int64_t q0 = bin_at[b], q1 = bin_at[b + 1];
+    const int64_t pieces = std::max<int64_t>(1, (q1 - q0 + cap - 1) / cap);
+    if (pieces == 1) {
+      sc.push_back(PbChunk{static_cast<uint32_t>(b), -1, static_cast<uint32_t>(q0), static_cast<uint32_t>(q1)});
+      continue;
+    }
+    multi.push_back(PbMulti{static_cast<uint32_t>(b), static_cast<uint32_t>(slots), static_cast<uint32_t>(pieces), 0});
+    for (int64_t i = 0; i < pieces; i++)
+      sc.push_back(PbChunk{static_cast<uint32_t>(b), static_cast<int32_t>(slots + i), static_cast<uint32_t>(q0 + i * cap),
+                           static_cast<uint32_t>(std::min(q0 + (i + 1) * cap, q1))});
+    slots += pieces;
+  }
+  auto upload = [&](const char* name, const void* p, size_t bytes) {
+    void* d = g->ws.Get(name, std::max<size_t>(bytes, 16));
+    if (bytes) GG_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, st));
+  };
+  upload("pr.pb.wchunks", wc.data(), wc.size() * sizeof(PbChunk));
+  upload("pr.pb.schunks", sc.data(), sc.size() * sizeof(PbChunk));
+  upload("pr.pb.multi", multi.data(), multi.size() * sizeof(PbMulti));
+  GG_HIP(hipStreamSynchronize(st));
+  g->ws.Drop("pr.pbb.");
+  B.built = true;
+  B.lg = lg;
+  B.k = k;
+  B.octets = octets;
+  B.n_write = static_cast<int64_t>(wc.size());
+  B.n_sum = static_cast<int64_t>(sc.size());
+  B.n_multi = static_cast<int64_t>(multi.size());
+  B.slots = slots;
+  B.build_ms = wt.ms();
+  return &B;
+}
+
 // in_place: one contribution array, each vertex's contribution replaced as
 // soon as its score is (PageRankPullGS, pr.cc:34-61), and each sweep runs as
 // kGsPasses launches over consecutive vertex-id ranges, so later ranges
@@ -849,7 +1302,8 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
   const bool fp64 = sizeof(T) == 8;  // profile names carry the precision
   const int64_t n = g->n;
   const PrBins bins = GetBins(g);
-  const gg_graph::PrSegments* seg = in_place ? nullptr : Segments<T>(g);
+  const gg_graph::PrBlocks* pb = in_place ? nullptr : Blocks(g, fp64);
+  const gg_graph::PrSegments* seg = in_place || pb ? nullptr : Segments<T>(g);
   const std::string seg_tag = sizeof(T) == 4 ? "pr.seg32." : "pr.seg64.";
   const int32_t* seg_src = seg ? g->Scratch<int32_t>((seg_tag + "src").c_str(), seg->hot_entries) : nullptr;
   const int32_t* seg_dst = seg ? g->Scratch<int32_t>((seg_tag + "dst").c_str(), seg->pairs) : nullptr;
@@ -879,7 +1333,47 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
   launches += 2;
   int iters = 0;
   double error = 0.0;
-  for (int it = 0; it < max_iters; it++) {
+  // Propagation blocks: per iteration a write, a sum and (for bins summed
+  // in several chunks) a merge launch.  Credited bytes: the write reads
+  // 2 B of offsets and writes a value per padded entry, plus the tile
+  // contributions; the sum reads 4 B of index per octet, 2 B of offsets and
+  // a value per entry, plus each vertex's score, offsets and new values
+  // (or its partial sums); the merge reads the partial sums.
+  T* pb_val = pb ? g->Scratch<T>(fp64 ? "pr.pb.val64" : "pr.pb.val32", 8 * pb->octets + 8) : nullptr;
+  using Word = typename PbAcc<T>::Word;
+  Word* pb_part = pb && pb->slots ? g->Scratch<Word>(fp64 ? "pr.pb.part64" : "pr.pb.part32", pb->slots << pb->lg)
+                                  : nullptr;
+  for (int it = 0; pb && it < max_iters; it++) {
+    PbArgs<T> a{ca, pb_val,
+                static_cast<const uint4*>(g->ws.Get("pr.pb.src", 0)), static_cast<const uint4*>(g->ws.Get("pr.pb.dst", 0)),
+                static_cast<const uint32_t*>(g->ws.Get("pr.pb.oct", 0)), pb_part, g->out_off, score, cb, base, damp,
+                residual, n, pb->lg};
+    const double ent = 8.0 * pb->octets, vs = sizeof(T), ps = sizeof(Word);
+    if (pb->n_write)
+      Run(g, fp64 ? "pr_blocks_write_fp64" : "pr_blocks_write", ent * (2 + vs) + vs * N, PbWrite<T>,
+          dim3(static_cast<unsigned>(pb->n_write)), dim3(kPbWriteBS), a,
+          static_cast<const PbChunk*>(g->ws.Get("pr.pb.wchunks", 0)));
+    Run(g, fp64 ? "pr_blocks_sum_fp64" : "pr_blocks_sum",
+        ent * (2 + vs) + 0.5 * ent + N * (8 + 3 * vs) + ps * double(pb->slots << pb->lg), PbSum<T>,
+        dim3(static_cast<unsigned>(pb->n_sum)), dim3(kPbSumBS), a, static_cast<const PbChunk*>(g->ws.Get("pr.pb.schunks", 0)));
+    launches += pb->n_write ? 2 : 1;
+    if (pb->n_multi) {
+      Run(g, fp64 ? "pr_blocks_merge_fp64" : "pr_blocks_merge", ps * double(pb->slots << pb->lg), PbMerge<T>,
+          dim3(GridFor(pb->n_multi << pb->lg, kBS, g->cu_count * 16)), blk, a,
+          static_cast<const PbMulti*>(g->ws.Get("pr.pb.multi", 0)), static_cast<int32_t>(pb->n_multi));
+      launches++;
+    }
+    const unsigned long long seq = MailNext(g);
+    Run(g, "pr_publish", 0, PrPublish, dim3(1), blk, residual, g->mail_dev, seq);
+    launches++;
+    unsigned long long bits = 0;
+    MailWait(g, seq, &bits, 1);
+    std::memcpy(&error, &bits, sizeof(error));
+    std::swap(ca, cb);
+    iters++;
+    if (error < epsilon) break;
+  }
+  for (int it = 0; !pb && it < max_iters; it++) {
     PrArgs<T> a{g->in_off, g->in_dst, g->out_off, ca, in_place ? ca : cb, score, base, damp, residual, 0, 0,
                 seg_tail, acc};
     const int passes = in_place ? kGsPasses : 1;
@@ -1002,8 +1496,9 @@ extern "C" int gg_pagerank_layout(gg_graph* g, int mode, int64_t* info) {
     GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
     GG_REQUIRE(mode == GG_PR_FP32 || mode == GG_PR_FP64, GG_EINVAL, "a Jacobi mode");
     GG_HIP(hipSetDevice(g->device));
-    const gg_graph::PrSegments* L = mode == GG_PR_FP64 ? Segments<double>(g) : Segments<float>(g);
     for (int i = 0; i < 6; i++) info[i] = 0;
+    if (BlockWidth(g, mode == GG_PR_FP64) > 0) return;  // the run goes by propagation blocks
+    const gg_graph::PrSegments* L = mode == GG_PR_FP64 ? Segments<double>(g) : Segments<float>(g);
     if (!L) return;
     info[0] = L->n_lds;
     info[1] = L->ns - L->n_lds;
@@ -1011,5 +1506,24 @@ extern "C" int gg_pagerank_layout(gg_graph* g, int mode, int64_t* info) {
     info[3] = L->hot_entries;
     info[4] = L->hot;
     info[5] = static_cast<int64_t>(L->build_ms * 1e3);
+  });
+}
+
+extern "C" int gg_pagerank_blocks(gg_graph* g, int mode, int64_t* info) {
+  return Guard([&] {
+    GG_REQUIRE(g != nullptr && info != nullptr, GG_EINVAL, "null argument");
+    GG_REQUIRE(!g->weighted, GG_ENOTSUP, "PageRank needs an unweighted graph upload");
+    GG_REQUIRE(g->n > 0, GG_EINVAL, "empty graph");
+    GG_REQUIRE(mode == GG_PR_FP32 || mode == GG_PR_FP64, GG_EINVAL, "a Jacobi mode");
+    GG_HIP(hipSetDevice(g->device));
+    for (int i = 0; i < 6; i++) info[i] = 0;
+    const gg_graph::PrBlocks* B = Blocks(g, mode == GG_PR_FP64);
+    if (!B) return;
+    info[0] = B->lg;
+    info[1] = B->k;
+    info[2] = 8 * B->octets;
+    info[3] = B->n_write;
+    info[4] = B->n_sum;
+    info[5] = static_cast<int64_t>(B->build_ms * 1e3);
   });
 }

@@ -383,6 +383,7 @@ extern "C" int gg_graph_order_by_degree(gg_graph* g, gg_stats* stats) {
     g->no_in_count = -1;
     g->pr_bins[0] = g->pr_bins[1] = g->pr_bins[2] = -1;
     g->pr_seg[0].built = g->pr_seg[1].built = false;
+    g->pr_blocks.built = false;
     g->pk_parts.clear();
     g->ms_heavy = -1;
     g->ms_heavy_chunks = 0;

@@ -193,6 +193,9 @@ extern "C" int gg_tuning_set(gg_graph* g, const gg_tuning* in) {
                "PageRank segment density out of range");
     GG_REQUIRE(in->bfs_seg_pct >= -1 && in->bfs_seg_pct <= 100, GG_EINVAL,
                "batched BFS segment threshold out of range");
+    GG_REQUIRE(in->pr_blocks_log2 == -1 || in->pr_blocks_log2 == 0 ||
+                   (in->pr_blocks_log2 >= 4 && in->pr_blocks_log2 <= 14),
+               GG_EINVAL, "PageRank block width out of range");
     const bool bins_changed =
         in->pr_small_max != g->tune.pr_small_max || in->pr_medium_max != g->tune.pr_medium_max;
     g->tune = *in;

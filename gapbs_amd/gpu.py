@@ -63,7 +63,7 @@ class Tuning(C.Structure):
                 ("sssp_delta", C.c_int32), ("pr_small_max", C.c_int32), ("pr_medium_max", C.c_int32),
                 ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4),
                 ("pr_segment_log2", C.c_int32 * 2), ("pr_lds_log2", C.c_int32 * 2),
-                ("pr_seg_min_density", C.c_int32), ("bfs_seg_pct", C.c_int32)]
+                ("pr_seg_min_density", C.c_int32), ("bfs_seg_pct", C.c_int32), ("pr_blocks_log2", C.c_int32)]
     _ARRAYS = ("best_ms", "pr_segment_log2", "pr_lds_log2")
 
     def as_dict(self) -> dict:
@@ -79,6 +79,7 @@ TUNE_BFS, TUNE_BATCH, TUNE_SSSP, TUNE_PR = 1, 2, 4, 8
 EXPORTED = [
     "gg_init", "gg_graph_upload", "gg_graph_free", "gg_graph_device_bytes",
     "gg_bfs", "gg_bfs_batch", "gg_bfs_packed", "gg_sssp", "gg_sssp_batch", "gg_pagerank", "gg_pagerank_layout",
+    "gg_pagerank_blocks",
     "gg_cc", "gg_cc_noalloc", "gg_bc", "gg_bc_batch", "gg_tc",
     "gg_relabel_by_degree", "gg_graph_copy_out", "gg_graph_order_by_degree", "gg_graph_vertex_order",
     "gg_graph_build", "gg_graph_copy_in", "gg_graph_generate", "gg_graph_read", "gg_cc_sv",
@@ -113,6 +114,7 @@ def lib() -> C.CDLL:
         L.gg_pagerank.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_float),
                                   C.POINTER(Stats)]
         L.gg_pagerank_layout.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+        L.gg_pagerank_blocks.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
         L.gg_cc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_cc_noalloc.argtypes = [vp, C.c_int, _i32p, C.POINTER(Stats)]
         L.gg_cc_sv.argtypes = [vp, _i32p, C.POINTER(Stats)]
@@ -416,6 +418,15 @@ def pagerank_layout(dg: DeviceGraph, fp64: bool = False) -> dict:
     _check(lib().gg_pagerank_layout(dg.handle, PR_FP64 if fp64 else PR_FP32, info))
     return {"lds_segments": info[0], "l2_segments": info[1], "pairs": info[2], "entries": info[3],
             "source_bound": info[4], "build_seconds": info[5] / 1e6}
+
+
+def pagerank_blocks(dg: DeviceGraph, fp64: bool = False) -> dict:
+    """gg_pagerank_blocks: the propagation-block layout a Jacobi run of the
+    precision uses (built on first use; width_log2 0 when it runs without)."""
+    info = (C.c_int64 * 6)()
+    _check(lib().gg_pagerank_blocks(dg.handle, PR_FP64 if fp64 else PR_FP32, info))
+    return {"width_log2": info[0], "tiles": info[1], "padded_entries": info[2], "write_chunks": info[3],
+            "sum_chunks": info[4], "build_seconds": info[5] / 1e6}
 
 
 def PageRankPullGS(dg: DeviceGraph, max_iters: int = 20, epsilon: float = 1e-4) -> np.ndarray:

@@ -141,7 +141,8 @@ enum { GG_PR_FP32 = 0, GG_PR_FP64 = 1, GG_PR_GS = 2 };
 int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* scores, gg_stats* stats);
 
 /* The source-segment layout gg_pagerank gathers through in a Jacobi mode
- * (GG_PR_FP32 or GG_PR_FP64) on this handle, built here if it is not yet.
+ * (GG_PR_FP32 or GG_PR_FP64) on this handle, built here if it is not yet
+ * (all 0 when the mode runs through propagation blocks instead).
  * No reference counterpart (the reference pulls every list whole); for
  * measurement reports.  info[0] LDS segments, info[1] L2 segments, info[2]
  * (destination, sub-list) pairs, info[3] in-entries held by the segments,
@@ -149,6 +150,18 @@ int gg_pagerank(gg_graph* g, int max_iters, double epsilon, int mode, float* sco
  * layout's build time in microseconds; all 0 when the gather runs without
  * segments. */
 int gg_pagerank_layout(gg_graph* g, int mode, int64_t* info);
+
+/* The propagation-block layout a Jacobi gg_pagerank (mode GG_PR_FP32 or
+ * GG_PR_FP64) runs through on this handle when gg_tuning.pr_blocks_log2
+ * selects it (by default on graphs of 2^20 to 2^27 vertices), built here if
+ * it is not yet; it replaces the source segments, and one layout serves
+ * both precisions.  No reference counterpart (the reference pulls every
+ * list whole, pr_spmv.cc:47-53); for measurement reports.  info[0] log2 of
+ * the tile / bin width (0: no blocks on this handle), info[1] tiles (=
+ * bins), info[2] in-entries after padding each (tile, bin) block to a
+ * multiple of 8, info[3] chunks of the write phase, info[4] chunks of the
+ * sum phase, info[5] the build time in microseconds. */
+int gg_pagerank_blocks(gg_graph* g, int mode, int64_t* info);
 
 /* Afforest connected components.
  * Replaces Afforest(const Graph&, bool logging, int32_t neighbor_rounds=2)
@@ -327,6 +340,12 @@ typedef struct gg_tuning {
    * reached; 0 and -1 never (the default: slower than the tiers' own early
    * exit on the graphs measured), 1..100. */
   int32_t bfs_seg_pct;
+  /* Jacobi PageRank propagation blocks (both precisions): log2 of the
+   * tile / bin width in vertices.  0 picks the default: width 2^14 on
+   * graphs of 2^20 to 2^27 vertices whose pr_segment_log2 for the
+   * precision is left at 0, else no blocks; -1 never; 4..14 always (the
+   * graph then needs at most 8192 tiles). */
+  int32_t pr_blocks_log2;
 } gg_tuning;
 enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };
 int gg_autotune(gg_graph* g, int what, const int32_t* roots, int k, gg_tuning* out);

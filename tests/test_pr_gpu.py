@@ -204,3 +204,105 @@ def test_pr_segment_width_is_validated():
         assert e.value.code == -22
     t = gpu.tuning(dg)
     assert t["pr_segment_log2"] == [0, 0] and t["pr_lds_log2"] == [0, 0]
+
+
+def _blocked(dg, lg, eps=1e-4):
+    """Jacobi runs through propagation blocks of width 2^lg (the layout's
+    description too), restoring the default afterwards."""
+    gpu.set_tuning(dg, {"pr_blocks_log2": lg})
+    try:
+        info = gpu.pagerank_blocks(dg)
+        return gpu.pagerank(dg, 20, eps), gpu.pagerank(dg, 20, eps, fp64=True), info
+    finally:
+        gpu.set_tuning(dg, {"pr_blocks_log2": 0})
+
+
+@pytest.mark.parametrize("ordered", [True, False])
+@pytest.mark.parametrize("lg", [4, 6, 9, 14])
+def test_pr_blocks_against_fp64_oracle(ordered, lg):
+    # Propagation blocks forced on K16 at widths from 16 vertices (4096 tiles
+    # and bins, most blocks holding one entry padded to eight) to 16384
+    # (four tiles).  A sum chunk holds at most 4 * 2^lg octets, so the hub
+    # bins are summed in several chunks and merged.  Both precisions meet
+    # the oracle, and FP32 stays within 1e-7 of the plain pull.
+    print(f"[pr] blocks lg={lg} ordered={ordered}", flush=True)
+    g = cached_graph("g16", GRAPHS[1][1])
+    dg = gpu.DeviceGraph(g, order_by_degree=ordered)
+    want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+    assert gpu.pagerank_blocks(dg)["width_log2"] == 0  # default: no blocks below 2^20 vertices
+    plain = gpu.pagerank(dg, 20, 1e-4)
+    b32, b64, info = _blocked(dg, lg)
+    k = (g.num_nodes + (1 << lg) - 1) >> lg
+    assert info["width_log2"] == lg and info["tiles"] == k
+    entries = int(g.in_offsets[-1])
+    assert info["padded_entries"] % 8 == 0 and entries <= info["padded_entries"] < entries + 8 * k * k
+    assert info["sum_chunks"] >= k and info["write_chunks"] >= 1
+    if lg <= 9:
+        assert info["sum_chunks"] > k  # some bins summed in several chunks
+    for r in (b32, b64):
+        assert r.stats["iterations"] == it
+    x32 = b32.scores.astype(np.float64)
+    assert np.abs(x32 - want).max() < 1e-6
+    assert (np.abs(x32 - want) / want).max() < 1e-5
+    assert (np.abs(b64.scores.astype(np.float64) - want) / want).max() < 1e-6
+    assert np.abs(x32 - plain.scores.astype(np.float64)).max() < 1e-7
+    assert g.verify_pr(b32.scores, 1e-4)
+    dg.free()
+
+
+@pytest.mark.parametrize("lg", [5, 8, 12])
+@pytest.mark.parametrize("shuffled", [False, True])
+def test_pr_blocks_directed_hub_isolated_and_unsorted(lg, shuffled):
+    # A directed graph (8951 vertices, not a multiple of any width) with a hub of
+    # 6000 in-neighbours, dangling and isolated vertices; shuffled: the
+    # lists uploaded unsorted (blocks do not need sorted lists).
+    print(f"[pr] blocks directed lg={lg} shuffled={shuffled}", flush=True)
+    import ctypes as C
+    rng = np.random.default_rng(17)
+    n = 9000
+    u = np.concatenate([rng.integers(0, n - 50, 40000), np.arange(1, 6001), rng.integers(0, 200, 5000)])
+    v = np.concatenate([rng.integers(0, n - 50, 40000), np.zeros(6000, np.int64), rng.integers(0, 40, 5000)])
+    g = Graph.from_edges(u, v, symmetrize=False)
+    n = g.num_nodes  # the largest id used + 1
+    if shuffled:
+        arrays = []
+        for off, dst in ((g.out_offsets, g.out_neighbors), (g.in_offsets, g.in_neighbors)):
+            off = np.ascontiguousarray(off, dtype=np.int64)
+            dst = np.array(dst, dtype=np.int32)
+            for x in range(n):
+                rng.shuffle(dst[off[x]:off[x + 1]])
+            arrays += [off, dst]
+        p64, p32 = C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+        h = C.c_void_p()
+        rc = gpu.lib().gg_graph_upload(n, arrays[0].ctypes.data_as(p64), arrays[1].ctypes.data_as(p32), 0, 1,
+                                       arrays[2].ctypes.data_as(p64), arrays[3].ctypes.data_as(p32), C.byref(h))
+        assert rc == 0, gpu.lib().gg_last_error()
+        dg = gpu.DeviceGraph._adopt(h, n)
+    else:
+        dg = gpu.DeviceGraph(g)
+    want, it, _ = oracle.pagerank(g, 20, 0.0, fp64=True)
+    b32, b64, info = _blocked(dg, lg, 0.0)
+    assert info["width_log2"] == lg
+    assert b32.stats["iterations"] == it == 20
+    assert (np.abs(b32.scores.astype(np.float64) - want) / want).max() < 1e-5
+    np.testing.assert_allclose(b64.scores, want.astype(np.float32), rtol=1e-6, atol=0)
+    dg.free()
+
+
+def test_pr_blocks_width_is_validated():
+    dg = dev("g10", GRAPHS[0][1])
+    for bad in (3, 15, -2):
+        with pytest.raises(gpu.GpuError) as e:
+            gpu.set_tuning(dg, {"pr_blocks_log2": bad})
+        assert e.value.code == -22
+    assert gpu.tuning(dg)["pr_blocks_log2"] == 0
+    # 2^18 vertices at width 2^4 would need 16384 tiles (more than 8192).
+    big = dev("k18s", GRAPHS[3][1])
+    gpu.set_tuning(big, {"pr_blocks_log2": 4})
+    try:
+        with pytest.raises(gpu.GpuError) as e:
+            gpu.pagerank(big, 20, 1e-4)
+        assert e.value.code == -22
+    finally:
+        gpu.set_tuning(big, {"pr_blocks_log2": 0})
+    assert gpu.pagerank(big, 20, 1e-4).stats["iterations"] > 0

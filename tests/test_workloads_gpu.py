@@ -89,10 +89,12 @@ class TestKronecker22:
         g, dg = k22["g"], k22["dg"]
         say("[k22] pagerank")
         want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
+        assert gpu.pagerank_blocks(dg)["width_log2"] == 14  # the default Jacobi layout at this size
         r32 = gpu.pagerank(dg, 20, 1e-4)
         r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
         assert r32.stats["iterations"] == it and r64.stats["iterations"] == it
         assert np.abs(r32.scores.astype(np.float64) - want).max() < 1e-6
+        assert np.array_equal(gpu.pagerank(dg, 20, 1e-4).scores, r32.scores)  # exact sums: run to run the same
         assert (np.abs(r64.scores.astype(np.float64) - want) / want).max() < 1e-6  # float output rounding
         assert np.abs(gpu.pagerank(dg, 20, 1e-4, gauss_seidel=True).scores - want).max() < 1e-4
         assert g.verify_pr(r32.scores, 1e-4)
@@ -228,11 +230,12 @@ class TestKronecker26:
         assert np.array_equal(gpu.bc_batch(dg, roots).scores, gpu.bc(dg, roots).scores)
 
     def test_pagerank_against_fp64_oracle(self, k26):
-        """The measured PageRank configuration (degree-ordered, default LDS and
-        L2 source segments) at full size: FP32 within 1e-6 max-abs of the FP64
-        oracle with the same iteration count; FP64 within float rounding."""
+        """The measured PageRank configuration (degree-ordered, propagation
+        blocks of 2^14 vertices) at full size: FP32 within 1e-6 max-abs of the
+        FP64 oracle with the same iteration count; FP64 within float rounding."""
         g, dg = k26["g"], k26["dg"]
         say("[k26] pagerank fp32 / fp64")
+        assert gpu.pagerank_blocks(dg)["width_log2"] == 14
         r32 = gpu.pagerank(dg, 20, 1e-4)
         r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
         say("[k26] oracle pagerank fp64 on the host")
