@@ -52,6 +52,8 @@
 #include "gg_common.h"
 #include "gg_sort.h"
 
+#include <algorithm>
+
 namespace gg {
 namespace {
 
@@ -1090,9 +1092,26 @@ __global__ __launch_bounds__(kBS) void PbMerge(PbArgs<T> a, const PbMulti* __res
   AddResidual(a.residual, r, s_red);
 }
 
-// Build, one block per bin: the in-entries of the bin's vertices, a
-// wavefront per vertex, are cut into runs of equal source tile (64 entries
-// at a time; sorted lists give one run per tile).
+// Build.  Each block walks in-lists, a wavefront 64 entries at a time, cut
+// into runs of equal source tile (sorted lists give one run per tile).
+// Work items: the first blocks take a range of one bin's vertices (about
+// kPbItemEntries in-entries, cut at multiples of 256 vertices) and walk the
+// lists shorter than kPbHubMin, a wavefront per vertex; the blocks after
+// them take one chunk (kPbHubChunk entries at most) of a longer list each,
+// its wavefronts strided over the chunk.
+constexpr int64_t kPbHubMin = int64_t(1) << 15;
+constexpr int64_t kPbHubChunk = int64_t(1) << 16;
+constexpr int64_t kPbItemEntries = int64_t(1) << 20;
+
+struct PbItem {
+  int64_t u0, u1;  // vertices [u0, u1), all in one bin
+};
+
+struct PbHubChunk {
+  int64_t e0, e1;  // in-entries [e0, e1) of vertex u
+  int32_t u, pad;
+};
+
 struct PbRun {
   int32_t t;       // the lane's source tile (-1 past the list)
   int first, len;  // its run: first lane, and length (valid in the run's first lane)
@@ -1112,67 +1131,119 @@ __device__ __forceinline__ PbRun PbRunOf(int32_t src, bool valid, int lg, int nv
   return r;
 }
 
-// Octets per (tile, bin) block, stored tile-major (tm[t * k + b]) and
-// bin-major (bm[b * k + t]).
-__global__ __launch_bounds__(kPbBuildBS) void PbCount(const int64_t* __restrict__ in_off,
-                                                      const int32_t* __restrict__ in_dst, int64_t n, int lg,
-                                                      int64_t k, uint32_t* __restrict__ tm, uint32_t* __restrict__ bm) {
-  __shared__ uint32_t cnt[kPbMaxTiles];
-  const int64_t b = blockIdx.x;
-  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) cnt[t] = 0;
-  __syncthreads();
+// Calls f(run, source, valid) for every 64-entry window of this block's work
+// item; returns the item's bin.
+template <class F>
+__device__ __forceinline__ int64_t PbWalk(const int64_t* __restrict__ in_off, const int32_t* __restrict__ in_dst,
+                                          int lg, const PbItem* __restrict__ items, int64_t n_items,
+                                          const PbHubChunk* __restrict__ hubs, F f) {
   const int lane = LaneId(), wid = threadIdx.x / kWave;
-  const int64_t u1 = min(n, (b + 1) << lg);
-  for (int64_t u = (b << lg) + wid; u < u1; u += kPbBuildBS / kWave) {
-    const int64_t e = in_off[u + 1];
-    for (int64_t k0 = in_off[u]; k0 < e; k0 += kWave) {
-      const bool valid = k0 + lane < e;
-      const PbRun r = PbRunOf(valid ? in_dst[k0 + lane] : 0, valid, lg, static_cast<int>(min<int64_t>(kWave, e - k0)));
-      if (r.len > 0) atomicAdd(&cnt[r.t], static_cast<uint32_t>(r.len));
+  constexpr int kWaves = kPbBuildBS / kWave;
+  auto window = [&](int64_t k0, int64_t e, int32_t u) {
+    const bool valid = k0 + lane < e;
+    const int32_t s = valid ? in_dst[k0 + lane] : 0;
+    f(PbRunOf(s, valid, lg, static_cast<int>(min<int64_t>(kWave, e - k0))), s, valid, u);
+  };
+  if (blockIdx.x < n_items) {
+    const PbItem it = items[blockIdx.x];
+    for (int64_t u = it.u0 + wid; u < it.u1; u += kWaves) {
+      const int64_t e0 = in_off[u], e = in_off[u + 1];
+      if (e - e0 >= kPbHubMin) continue;  // cut into chunks
+      for (int64_t k0 = e0; k0 < e; k0 += kWave) window(k0, e, static_cast<int32_t>(u));
     }
+    return it.u0 >> lg;
   }
-  __syncthreads();
-  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) {
-    const uint32_t o = (cnt[t] + 7) >> 3;
-    tm[t * k + b] = o;
-    bm[b * k + t] = o;
+  const PbHubChunk h = hubs[blockIdx.x - n_items];
+  for (int64_t k0 = h.e0 + int64_t(wid) * kWave; k0 < h.e1; k0 += int64_t(kWaves) * kWave) window(k0, h.e1, h.u);
+  return h.u >> lg;
+}
+
+__global__ __launch_bounds__(kPbBuildBS) void PbFindHubs(const int64_t* __restrict__ in_off, int64_t n,
+                                                         PbHubChunk* __restrict__ hubs, unsigned* __restrict__ count,
+                                                         unsigned cap) {
+  for (int64_t u = blockIdx.x * int64_t(kPbBuildBS) + threadIdx.x; u < n; u += int64_t(gridDim.x) * kPbBuildBS) {
+    const int64_t e0 = in_off[u], e1 = in_off[u + 1];
+    if (e1 - e0 < kPbHubMin) continue;
+    const unsigned i = atomicAdd(count, 1u);
+    if (i < cap) hubs[i] = PbHubChunk{e0, e1, static_cast<int32_t>(u), 0};
   }
 }
 
-// Writes each entry's offsets at its block's next free position (a run at a
-// time, so a destination's entries of one tile stay together), then pads
-// each block to its octets.  run: the exclusive scan of tm.
-__global__ __launch_bounds__(kPbBuildBS) void PbEmit(const int64_t* __restrict__ in_off,
-                                                     const int32_t* __restrict__ in_dst, int64_t n, int lg, int64_t k,
-                                                     const uint32_t* __restrict__ run, uint16_t* __restrict__ src16,
-                                                     uint16_t* __restrict__ dst16) {
-  __shared__ uint32_t cur[kPbMaxTiles];
-  const int64_t b = blockIdx.x;
-  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) cur[t] = 0;
+// In-entries per (tile, bin) block, added into cnt[t * k + b] (zeroed).
+__global__ __launch_bounds__(kPbBuildBS) void PbCount(const int64_t* __restrict__ in_off,
+                                                      const int32_t* __restrict__ in_dst, int lg, int64_t k,
+                                                      const PbItem* __restrict__ items, int64_t n_items,
+                                                      const PbHubChunk* __restrict__ hubs, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t c[kPbMaxTiles];
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) c[t] = 0;
   __syncthreads();
-  const int lane = LaneId(), wid = threadIdx.x / kWave;
-  const uint32_t m = (1u << lg) - 1;
-  const int64_t u1 = min(n, (b + 1) << lg);
-  for (int64_t u = (b << lg) + wid; u < u1; u += kPbBuildBS / kWave) {
-    const int64_t e = in_off[u + 1];
-    for (int64_t k0 = in_off[u]; k0 < e; k0 += kWave) {
-      const bool valid = k0 + lane < e;
-      const int32_t s = valid ? in_dst[k0 + lane] : 0;
-      const PbRun r = PbRunOf(s, valid, lg, static_cast<int>(min<int64_t>(kWave, e - k0)));
-      uint32_t at = 0;
-      if (r.len > 0) at = atomicAdd(&cur[r.t], static_cast<uint32_t>(r.len));
-      at = __shfl(at, r.first);
-      if (valid) {
-        const int64_t pos = int64_t(run[int64_t(r.t) * k + b]) * 8 + at + (lane - r.first);
-        src16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(s) & m);
-        dst16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(u) & m);
-      }
-    }
+  const int64_t b = PbWalk(in_off, in_dst, lg, items, n_items, hubs, [&](const PbRun& r, int32_t, bool, int32_t) {
+    if (r.len > 0) atomicAdd(&c[r.t], static_cast<uint32_t>(r.len));
+  });
+  __syncthreads();
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS)
+    if (c[t]) atomicAdd(&cnt[t * k + b], c[t]);
+}
+
+// Counts to octets, in place (tile-major) and transposed into bm (bin-major).
+__global__ __launch_bounds__(kPbBuildBS) void PbOctets(uint32_t* __restrict__ tm, uint32_t* __restrict__ bm, int64_t k) {
+  for (int64_t i = blockIdx.x * int64_t(kPbBuildBS) + threadIdx.x; i < k * k; i += int64_t(gridDim.x) * kPbBuildBS) {
+    const uint32_t o = (tm[i] + 7) >> 3;
+    tm[i] = o;
+    bm[(i % k) * k + i / k] = o;
   }
+}
+
+// Writes each entry's offsets into its block: the work item first reserves
+// a range per tile from the block's cursor (cur[t * k + b], zeroed), then
+// places each run there, so a run's entries (one destination's, within one
+// tile) stay together.  run: the exclusive scan of the octet counts.
+__global__ __launch_bounds__(kPbBuildBS) void PbEmit(const int64_t* __restrict__ in_off,
+                                                     const int32_t* __restrict__ in_dst, int lg, int64_t k,
+                                                     const PbItem* __restrict__ items, int64_t n_items,
+                                                     const PbHubChunk* __restrict__ hubs, const uint32_t* __restrict__ run,
+                                                     uint32_t* __restrict__ cur, uint16_t* __restrict__ src16,
+                                                     uint16_t* __restrict__ dst16) {
+  __shared__ uint32_t c[kPbMaxTiles], base[kPbMaxTiles];
+  for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) c[t] = 0;
+  __syncthreads();
+  const int64_t b = PbWalk(in_off, in_dst, lg, items, n_items, hubs, [&](const PbRun& r, int32_t, bool, int32_t) {
+    if (r.len > 0) atomicAdd(&c[r.t], static_cast<uint32_t>(r.len));
+  });
   __syncthreads();
   for (int64_t t = threadIdx.x; t < k; t += kPbBuildBS) {
-    const int64_t p1 = int64_t(run[t * k + b + 1]) * 8;
-    for (int64_t p = int64_t(run[t * k + b]) * 8 + cur[t]; p < p1; p++) {
+    base[t] = c[t] ? atomicAdd(&cur[t * k + b], c[t]) : 0;
+    c[t] = 0;
+  }
+  __syncthreads();
+  const uint32_t m = (1u << lg) - 1;
+  PbWalk(in_off, in_dst, lg, items, n_items, hubs, [&](const PbRun& r, int32_t s, bool valid, int32_t u) {
+    uint32_t at = 0;
+    if (r.len > 0) at = base[r.t] + atomicAdd(&c[r.t], static_cast<uint32_t>(r.len));
+    at = __shfl(at, r.first);
+    if (valid) {
+      const int64_t pos = int64_t(run[int64_t(r.t) * k + b]) * 8 + at + (LaneId() - r.first);
+      src16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(s) & m);
+      dst16[pos] = static_cast<uint16_t>(static_cast<uint32_t>(u) & m);
+    }
+  });
+}
+
+// dst[i] = src[i * stride] for i < count (strided values for the host).
+template <class T>
+__global__ __launch_bounds__(kPbBuildBS) void PbGather(const T* __restrict__ src, int64_t stride, int64_t count,
+                                                       T* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * int64_t(kPbBuildBS) + threadIdx.x; i < count; i += int64_t(gridDim.x) * kPbBuildBS)
+    dst[i] = src[i * stride];
+}
+
+// Pads each block from its entries to its octets with (0, kPbPad).
+__global__ __launch_bounds__(kPbBuildBS) void PbPad(const uint32_t* __restrict__ run, const uint32_t* __restrict__ cur,
+                                                    int64_t blocks, uint16_t* __restrict__ src16,
+                                                    uint16_t* __restrict__ dst16) {
+  for (int64_t i = blockIdx.x * int64_t(kPbBuildBS) + threadIdx.x; i < blocks; i += int64_t(gridDim.x) * kPbBuildBS) {
+    const int64_t p1 = int64_t(run[i + 1]) * 8;
+    for (int64_t p = int64_t(run[i]) * 8 + cur[i]; p < p1; p++) {
       src16[p] = 0;
       dst16[p] = static_cast<uint16_t>(kPbPad);
     }
@@ -1191,12 +1262,20 @@ __global__ __launch_bounds__(kPbBuildBS) void PbOctIndex(int64_t k, const uint32
   }
 }
 
-// The width the Jacobi run of this precision uses (0: no blocks).
+// The width the Jacobi run of this precision uses (0: no blocks).  By
+// default about 4096 tiles: 2^(ceil(log2 n) - 12) vertices, from 2^10 to
+// 2^14 (measured on Kronecker graphs of 2^20 to 2^26 vertices: the
+// narrowest width with few enough tiles to keep the padding and the build
+// tables small, since narrower blocks give more bins, so fewer bins summed
+// in several chunks, and smaller LDS footprints).
 int BlockWidth(const gg_graph* g, bool fp64) {
   const int knob = g->tune.pr_blocks_log2;
   if (knob > 0) return knob;
   if (knob < 0 || g->tune.pr_segment_log2[fp64 ? 1 : 0] != 0) return 0;
-  return g->n >= (int64_t(1) << 20) && g->n <= (int64_t(1) << 27) ? kPbMaxLog : 0;
+  const int64_t n = g->n;
+  if (n < (int64_t(1) << 20) || n > (int64_t(1) << 27)) return 0;
+  const int ceil_log2 = 64 - __builtin_clzll(static_cast<unsigned long long>(n - 1));
+  return std::min(kPbMaxLog, std::max(10, ceil_log2 - 12));
 }
 
 // The propagation-block layout, built on first use (and again when the
@@ -1222,31 +1301,89 @@ const gg_graph::PrBlocks* Blocks(gg_graph* g, bool fp64) {
   } restore{g, prof};
   hipStream_t st = g->stream;
   const int64_t kk = k * k;
+  // Lists of kPbHubMin entries or more, cut into chunks (their own work items).
+  const unsigned hub_cap = static_cast<unsigned>(g->in_entries / kPbHubMin + 1);
+  PbHubChunk* hubs = g->Scratch<PbHubChunk>("pr.pbb.hubs", hub_cap);
+  unsigned* nh = g->Scratch<unsigned>("pr.pbb.nh", 1);
+  GG_HIP(hipMemsetAsync(nh, 0, sizeof(unsigned), st));
+  PbFindHubs<<<GridFor(n, kPbBuildBS, g->cu_count * 8), kPbBuildBS, 0, st>>>(g->in_off, n, hubs, nh, hub_cap);
+  unsigned nhubs = 0;
+  GG_HIP(hipMemcpyAsync(&nhubs, nh, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  GG_REQUIRE(nhubs <= hub_cap, GG_EHIP, "PageRank block build: hub list overflow");
+  std::vector<PbHubChunk> hl(nhubs), chunks;
+  if (nhubs) GG_HIP(hipMemcpy(hl.data(), hubs, nhubs * sizeof(PbHubChunk), hipMemcpyDeviceToHost));
+  std::sort(hl.begin(), hl.end(), [](const PbHubChunk& x, const PbHubChunk& y) { return x.u < y.u; });
+  for (const PbHubChunk& h : hl)
+    for (int64_t e = h.e0; e < h.e1; e += kPbHubChunk)
+      chunks.push_back(PbHubChunk{e, std::min(e + kPbHubChunk, h.e1), h.u, 0});
+  // Vertex-range items: each bin cut at multiples of G vertices once its
+  // short lists pass kPbItemEntries entries (offsets sampled every G).
+  const int64_t G = std::min<int64_t>(256, int64_t(1) << lg), ng = (n + G - 1) / G;
+  std::vector<int64_t> goff(ng + 1), hub_in(ng, 0);
+  int64_t* dgoff = g->Scratch<int64_t>("pr.pbb.goff", ng);
+  PbGather<int64_t><<<GridFor(ng, kPbBuildBS, g->cu_count * 4), kPbBuildBS, 0, st>>>(g->in_off, G, ng, dgoff);
+  GG_HIP(hipMemcpyAsync(goff.data(), dgoff, ng * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipMemcpyAsync(&goff[ng], g->in_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  for (const PbHubChunk& h : hl) hub_in[h.u / G] += h.e1 - h.e0;
+  std::vector<PbItem> il;
+  for (int64_t b = 0; b < k; b++) {
+    const int64_t g0 = (b << lg) / G, g1 = std::min(ng, ((b + 1) << lg) / G);
+    int64_t start = g0, acc = 0;
+    for (int64_t gi = g0; gi < g1; gi++) {
+      const int64_t e = goff[gi + 1] - goff[gi] - hub_in[gi];
+      if (acc > 0 && acc + e > kPbItemEntries) {
+        il.push_back(PbItem{start * G, gi * G});
+        start = gi;
+        acc = 0;
+      }
+      acc += e;
+    }
+    il.push_back(PbItem{start * G, std::min(n, g1 * G)});
+  }
+  const int64_t n_items = static_cast<int64_t>(il.size());
+  PbItem* ditems = g->Scratch<PbItem>("pr.pbb.items", il.size());
+  GG_HIP(hipMemcpyAsync(ditems, il.data(), il.size() * sizeof(PbItem), hipMemcpyHostToDevice, st));
+  PbHubChunk* dchunks = g->Scratch<PbHubChunk>("pr.pbb.chunks", chunks.size() + 1);
+  if (!chunks.empty())
+    GG_HIP(hipMemcpyAsync(dchunks, chunks.data(), chunks.size() * sizeof(PbHubChunk), hipMemcpyHostToDevice, st));
+  const unsigned items = static_cast<unsigned>(n_items + static_cast<int64_t>(chunks.size()));
   uint32_t* tm = g->Scratch<uint32_t>("pr.pbb.tm", kk + 1);
   uint32_t* bm = g->Scratch<uint32_t>("pr.pbb.bm", kk + 1);
-  PbCount<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, n, lg, k, tm, bm);
+  uint32_t* cur = g->Scratch<uint32_t>("pr.pbb.cur", kk);
+  GG_HIP(hipMemsetAsync(tm, 0, kk * sizeof(uint32_t), st));
+  GG_HIP(hipMemsetAsync(cur, 0, kk * sizeof(uint32_t), st));
+  PbCount<<<items, kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, lg, k, ditems, n_items, dchunks, tm);
+  PbOctets<<<GridFor(kk, kPbBuildBS, g->cu_count * 8), kPbBuildBS, 0, st>>>(tm, bm, k);
   GG_HIP(hipGetLastError());
   DeviceScan<uint32_t>(g, "pr.pbb.st.", tm, kk, tm + kk);
   DeviceScan<uint32_t>(g, "pr.pbb.sb.", bm, kk, bm + kk);
   // Tile starts tm[t * k] and bin starts bm[b * k], t, b = 0..k (the last
   // one is the total).
   std::vector<uint32_t> tile_at(k + 1), bin_at(k + 1);
-  GG_HIP(hipMemcpy2DAsync(tile_at.data(), sizeof(uint32_t), tm, k * sizeof(uint32_t), sizeof(uint32_t), k + 1,
-                          hipMemcpyDeviceToHost, st));
-  GG_HIP(hipMemcpy2DAsync(bin_at.data(), sizeof(uint32_t), bm, k * sizeof(uint32_t), sizeof(uint32_t), k + 1,
-                          hipMemcpyDeviceToHost, st));
+  uint32_t* starts = g->Scratch<uint32_t>("pr.pbb.starts", 2 * (k + 1));
+  PbGather<uint32_t><<<GridFor(k + 1, kPbBuildBS), kPbBuildBS, 0, st>>>(tm, k, k + 1, starts);
+  PbGather<uint32_t><<<GridFor(k + 1, kPbBuildBS), kPbBuildBS, 0, st>>>(bm, k, k + 1, starts + k + 1);
+  GG_HIP(hipMemcpyAsync(tile_at.data(), starts, (k + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipMemcpyAsync(bin_at.data(), starts + k + 1, (k + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   GG_HIP(hipStreamSynchronize(st));
   const int64_t octets = tile_at[k];
   GG_REQUIRE(bin_at[k] == tile_at[k], GG_EHIP, "PageRank block tables disagree");
   uint16_t* src16 = g->Scratch<uint16_t>("pr.pb.src", 8 * octets + 8);
   uint16_t* dst16 = g->Scratch<uint16_t>("pr.pb.dst", 8 * octets + 8);
   uint32_t* oct = g->Scratch<uint32_t>("pr.pb.oct", octets + 1);
-  PbEmit<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, n, lg, k, tm, src16, dst16);
+  PbEmit<<<items, kPbBuildBS, 0, st>>>(g->in_off, g->in_dst, lg, k, ditems, n_items, dchunks, tm, cur, src16, dst16);
+  PbPad<<<GridFor(kk, kPbBuildBS, g->cu_count * 8), kPbBuildBS, 0, st>>>(tm, cur, kk, src16, dst16);
   PbOctIndex<<<static_cast<unsigned>(k), kPbBuildBS, 0, st>>>(k, tm, bm, oct);
   GG_HIP(hipGetLastError());
-  // Chunks of at most 4 * 2^lg octets: the write phase's per tile, the sum
-  // phase's per bin (every bin has one, so every vertex is finished).
-  const int64_t cap = int64_t(4) << lg;
+  // Chunks: the write phase's per tile (at most 4 * 2^lg octets), the sum
+  // phase's per bin (at most 16 * 2^lg octets, so fewer bins need partial
+  // sums and a merge; every bin has one, so every vertex is finished).  On
+  // smaller graphs the caps shrink to give each phase about four chunks per
+  // CU (down to 2^12 octets a chunk).
+  const int64_t per_cu = std::max<int64_t>(int64_t(1) << 12, octets / (4 * int64_t(g->cu_count)));
+  const int64_t cap = std::min(int64_t(4) << lg, per_cu), scap = std::min(int64_t(16) << lg, per_cu);
   std::vector<PbChunk> wc, sc;
   std::vector<PbMulti> multi;
   int64_t slots = 0;
@@ -1256,15 +1393,15 @@ const gg_graph::PrBlocks* Blocks(gg_graph* g, bool fp64) {
                            static_cast<uint32_t>(std::min<int64_t>(q + cap, tile_at[t + 1]))});
   for (int64_t b = 0; b < k; b++) {
     const int64_t q0 = bin_at[b], q1 = bin_at[b + 1];
-    const int64_t pieces = std::max<int64_t>(1, (q1 - q0 + cap - 1) / cap);
+    const int64_t pieces = std::max<int64_t>(1, (q1 - q0 + scap - 1) / scap);
     if (pieces == 1) {
       sc.push_back(PbChunk{static_cast<uint32_t>(b), -1, static_cast<uint32_t>(q0), static_cast<uint32_t>(q1)});
       continue;
     }
     multi.push_back(PbMulti{static_cast<uint32_t>(b), static_cast<uint32_t>(slots), static_cast<uint32_t>(pieces), 0});
     for (int64_t i = 0; i < pieces; i++)
-      sc.push_back(PbChunk{static_cast<uint32_t>(b), static_cast<int32_t>(slots + i), static_cast<uint32_t>(q0 + i * cap),
-                           static_cast<uint32_t>(std::min(q0 + (i + 1) * cap, q1))});
+      sc.push_back(PbChunk{static_cast<uint32_t>(b), static_cast<int32_t>(slots + i), static_cast<uint32_t>(q0 + i * scap),
+                           static_cast<uint32_t>(std::min(q0 + (i + 1) * scap, q1))});
     slots += pieces;
   }
   auto upload = [&](const char* name, const void* p, size_t bytes) {

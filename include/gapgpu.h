@@ -341,10 +341,11 @@ typedef struct gg_tuning {
    * exit on the graphs measured), 1..100. */
   int32_t bfs_seg_pct;
   /* Jacobi PageRank propagation blocks (both precisions): log2 of the
-   * tile / bin width in vertices.  0 picks the default: width 2^14 on
-   * graphs of 2^20 to 2^27 vertices whose pr_segment_log2 for the
-   * precision is left at 0, else no blocks; -1 never; 4..14 always (the
-   * graph then needs at most 8192 tiles). */
+   * tile / bin width in vertices.  0 picks the default: on graphs of 2^20
+   * to 2^27 vertices whose pr_segment_log2 for the precision is left at 0,
+   * width 2^(ceil(log2 n) - 12) clamped to 2^10..2^14 (about 4096 tiles),
+   * else no blocks; -1 never; 4..14 always (the graph then needs at most
+   * 8192 tiles). */
   int32_t pr_blocks_log2;
 } gg_tuning;
 enum { GG_TUNE_BFS = 1, GG_TUNE_BATCH = 2, GG_TUNE_SSSP = 4, GG_TUNE_PR = 8 };

@@ -222,8 +222,8 @@ def _blocked(dg, lg, eps=1e-4):
 def test_pr_blocks_against_fp64_oracle(ordered, lg):
     # Propagation blocks forced on K16 at widths from 16 vertices (4096 tiles
     # and bins, most blocks holding one entry padded to eight) to 16384
-    # (four tiles).  A sum chunk holds at most 4 * 2^lg octets, so the hub
-    # bins are summed in several chunks and merged.  Both precisions meet
+    # (four tiles).  A sum chunk holds at most 16 * 2^lg octets, so at the
+    # small widths the hub bins are summed in several chunks and merged.  Both precisions meet
     # the oracle, and FP32 stays within 1e-7 of the plain pull.
     print(f"[pr] blocks lg={lg} ordered={ordered}", flush=True)
     g = cached_graph("g16", GRAPHS[1][1])
@@ -237,7 +237,7 @@ def test_pr_blocks_against_fp64_oracle(ordered, lg):
     entries = int(g.in_offsets[-1])
     assert info["padded_entries"] % 8 == 0 and entries <= info["padded_entries"] < entries + 8 * k * k
     assert info["sum_chunks"] >= k and info["write_chunks"] >= 1
-    if lg <= 9:
+    if lg <= 6:
         assert info["sum_chunks"] > k  # some bins summed in several chunks
     for r in (b32, b64):
         assert r.stats["iterations"] == it

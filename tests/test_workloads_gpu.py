@@ -89,7 +89,7 @@ class TestKronecker22:
         g, dg = k22["g"], k22["dg"]
         say("[k22] pagerank")
         want, it, _ = oracle.pagerank(g, 20, 1e-4, fp64=True)
-        assert gpu.pagerank_blocks(dg)["width_log2"] == 14  # the default Jacobi layout at this size
+        assert gpu.pagerank_blocks(dg)["width_log2"] == 10  # the default Jacobi layout: 4096 tiles of 2^10
         r32 = gpu.pagerank(dg, 20, 1e-4)
         r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
         assert r32.stats["iterations"] == it and r64.stats["iterations"] == it
@@ -231,11 +231,11 @@ class TestKronecker26:
 
     def test_pagerank_against_fp64_oracle(self, k26):
         """The measured PageRank configuration (degree-ordered, propagation
-        blocks of 2^14 vertices) at full size: FP32 within 1e-6 max-abs of the
+        blocks of 2^14 vertices, 4096 tiles) at full size: FP32 within 1e-6 max-abs of the
         FP64 oracle with the same iteration count; FP64 within float rounding."""
         g, dg = k26["g"], k26["dg"]
         say("[k26] pagerank fp32 / fp64")
-        assert gpu.pagerank_blocks(dg)["width_log2"] == 14
+        assert gpu.pagerank_blocks(dg)["width_log2"] == 14  # 4096 tiles of 2^14
         r32 = gpu.pagerank(dg, 20, 1e-4)
         r64 = gpu.pagerank(dg, 20, 1e-4, fp64=True)
         say("[k26] oracle pagerank fp64 on the host")
