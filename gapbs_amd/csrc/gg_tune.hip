@@ -105,7 +105,35 @@ void TuneSssp(gg_graph* g, const int32_t* roots, int k) {
   g->tune.tuned |= GG_TUNE_SSSP;
 }
 
+// Graphs whose Jacobi runs go through propagation blocks at the default
+// width: the width and its two neighbours (at most 8192 tiles) are timed
+// (device time excludes each width's layout build) and the fastest kept.
+// Returns false when the graph runs without blocks.
+bool TunePrBlocks(gg_graph* g) {
+  int64_t info[6];
+  Check(gg_pagerank_blocks(g, GG_PR_FP32, info));
+  if (info[0] == 0 || g->tune.pr_blocks_log2 != 0) return false;
+  const int d = static_cast<int>(info[0]);
+  auto run = [&]() { return Time([&](gg_stats* st) { return gg_pagerank(g, kPrIters, 0.0, GG_PR_FP32, nullptr, st); }); };
+  double best = run();
+  int choice = 0;
+  for (int lg : {d - 1, d + 1}) {
+    if (lg < 4 || lg > 14 || ((g->n + (int64_t(1) << lg) - 1) >> lg) > 8192) continue;
+    g->tune.pr_blocks_log2 = lg;
+    const double t = run();
+    if (t < best) {
+      best = t;
+      choice = lg;
+    }
+  }
+  g->tune.pr_blocks_log2 = choice;  // 0: the default width
+  g->tune.best_ms[3] = best / kPrIters;
+  g->tune.tuned |= GG_TUNE_PR;
+  return true;
+}
+
 void TunePr(gg_graph* g) {
+  if (TunePrBlocks(g)) return;
   auto run = [&]() {
     g->pr_bins[0] = -1;  // the cut-offs changed: rebuild the bins
     return Time([&](gg_stats* st) { return gg_pagerank(g, kPrIters, 0.0, GG_PR_FP32, nullptr, st); });

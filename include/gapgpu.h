@@ -306,9 +306,12 @@ int gg_graph_vertex_order(gg_graph* g, int32_t* new_id);
  *   GG_TUNE_BATCH   gg_bfs_batch / gg_bfs_packed alpha over {1..1000}
  *   GG_TUNE_SSSP    gg_sssp delta over powers of two 1..4096 (weighted
  *                   uploads only; slow candidates are cut off early)
- *   GG_TUNE_PR      PageRank in-degree bin cut-offs (thread / wavefront /
- *                   block per vertex) and the gather launch width, by
- *                   coordinate descent over 3-iteration runs
+ *   GG_TUNE_PR      on a graph whose Jacobi PageRank runs through
+ *                   propagation blocks at the default width: that width or
+ *                   a neighbour (pr_blocks_log2), by 3-iteration runs;
+ *                   otherwise the in-degree bin cut-offs (thread /
+ *                   wavefront / block per vertex) and the gather launch
+ *                   width, by coordinate descent over 3-iteration runs
  * alpha, beta and delta are arguments of their entry points: callers read
  * them back (gg_tuning_get) and pass them; the PageRank settings apply to
  * every later gg_pagerank on the handle.  gg_tuning_set installs a saved

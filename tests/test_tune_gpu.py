@@ -72,6 +72,26 @@ def test_tune_pagerank_and_explicit_bins():
     assert e.value.code == -22
 
 
+def test_tune_pagerank_block_width():
+    # A graph of 2^20 vertices runs Jacobi PageRank through propagation
+    # blocks (default width 2^10): the tuner times that width and its
+    # neighbours, keeps one, and the scores stay within 1e-7 of the plain
+    # pull's at any of them.
+    print("[tune] pagerank blocks k20", flush=True)
+    dg, _ = gpu.generate_graph(0, 20, 16, 27491095)
+    assert gpu.pagerank_blocks(dg)["width_log2"] == 10
+    gpu.set_tuning(dg, {"pr_blocks_log2": -1, "pr_segment_log2": [-1, -1]})
+    plain = gpu.pagerank(dg, 20, 1e-4)
+    gpu.set_tuning(dg, {"pr_blocks_log2": 0, "pr_segment_log2": [0, 0]})
+    t = gpu.autotune(dg, gpu.TUNE_PR)
+    assert t["pr_blocks_log2"] in {0, 9, 11} and t["tuned"] & gpu.TUNE_PR and t["best_ms"][3] > 0
+    assert gpu.pagerank_blocks(dg)["width_log2"] == {0: 10}.get(t["pr_blocks_log2"], t["pr_blocks_log2"])
+    r = gpu.pagerank(dg, 20, 1e-4)
+    assert r.stats["iterations"] == plain.stats["iterations"]
+    assert np.abs(r.scores.astype(np.float64) - plain.scores).max() < 1e-7
+    dg.free()
+
+
 @pytest.mark.parametrize("kernel,extra", [("bfs", ["-n2"]), ("sssp", ["-n2"]), ("pr", ["-n1"])])
 def test_driver_autotune_flag(kernel, extra):
     print(f"[tune] driver {kernel} -q", flush=True)
