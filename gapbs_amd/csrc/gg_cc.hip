@@ -62,8 +62,18 @@ namespace {
 constexpr int kBS = 256;
 constexpr int kSamples = 1024;
 
+// Labels are read and written at agent scope.  The MI355X's eight XCDs
+// each have their own L2: a plain store can stay in the writer's L2 while
+// another XCD reads the label from memory, and Shiloach-Vishkin takes an
+// iteration without a hook as proof that every edge already joins equal
+// labels.  With plain stores for its hooks and pointer jumps, 5 of 20 runs
+// on a degree-ordered Kronecker 22 graph ended with a vertex still its own
+// root beside 558 neighbours of label 0; with agent-scope stores, 0 of 20.
 __device__ __forceinline__ int32_t Load(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void Store(int32_t* p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void Link(int32_t u, int32_t v, int32_t* comp) {
@@ -103,7 +113,7 @@ __global__ __launch_bounds__(kBS) void CcCompress(int32_t* comp, int64_t n) {
       c = cc;
       cc = Load(&comp[c]);
     }
-    if (Load(&comp[i]) != c) comp[i] = c;
+    if (Load(&comp[i]) != c) Store(&comp[i], c);
   }
 }
 
@@ -153,8 +163,8 @@ __global__ __launch_bounds__(kBS) void CcFinalLinkChunks(const int32_t* __restri
 // Shiloach-Vishkin's hook (src/cc_sv.cc:58-71): over edge (u, v), the higher
 // of the two labels, while it is still a root, takes the lower one -- the
 // min-max swap that makes lower labels win whatever the edge's direction.
-// A plain store, as the reference's; `hooked` records that this thread
-// changed a label.
+// An agent-scope store where the reference has a plain one (see Load);
+// `hooked` records that this thread changed a label.
 struct HookVisit {
   const int32_t* dst;
   int32_t* comp;
@@ -166,7 +176,7 @@ struct HookVisit {
     const int32_t high = cu > cv ? cu : cv;
     const int32_t low = cu + (cv - high);
     if (Load(&comp[high]) == high) {
-      comp[high] = low;
+      Store(&comp[high], low);
       *hooked = true;
     }
   }

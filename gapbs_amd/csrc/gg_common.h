@@ -225,18 +225,31 @@ const int32_t* UploadOrderLists(gg_graph* g);
 // Looks up, once per device, the device address of gg_cc_noalloc's fixed
 // scratch array on `device` (the current device); called by gg_init.
 void NoAllocSetup(int device);
-// The source-segment layout of the in-lists (gg_pr.hip) for 4-byte (fp64
-// false) or 8-byte (true) per-vertex values, built on first use; null when
-// the lists get none (small, unsorted or too sparse).  Its device arrays are
-// the workspace buffers SegArrays names.  Call outside timing.
-const gg_graph::PrSegments* SourceSegments(gg_graph* g, bool fp64);
-struct SegArrays {
-  const int32_t* src;   // sub-lists, segment-major
-  const int32_t* dst;   // destination of each pair
-  const int64_t* off;   // pair p's sub-list: src[off[p], off[p + 1])
-  const int32_t* tail;  // per vertex: where its list continues past the segments (from in_off[v])
+// Propagation blocks (gg_pr.hip): the in-entries grouped by (source tile,
+// destination bin) of 2^lg vertices, stored tile-major in octets of 8
+// entries (16-bit offsets within the tile and the bin; padding entries
+// have destination kPbPad), with a bin-major octet index and chunk tables.
+constexpr uint32_t kPbPad = 0xFFFFu;
+struct PbChunk {
+  uint32_t part;    // the tile (write phase) or bin (sum phase)
+  int32_t slot;     // sum phase: partial-result slot, -1 when the chunk is the whole bin
+  uint32_t q0, q1;  // write: octets [q0, q1); sum: octet-index positions [q0, q1)
 };
-SegArrays SourceSegmentArrays(gg_graph* g, const gg_graph::PrSegments* L, bool fp64);
+struct PbMulti {
+  uint32_t bin, slot0, nslots, pad;  // a bin summed in nslots chunks, slots slot0...
+};
+struct PbArrays {
+  const uint4* src16;        // per octet: 8 source offsets within the tile
+  const uint4* dst16;        // per octet: 8 destination offsets within the bin
+  const uint32_t* oct;       // each bin's octets, bin-major
+  const PbChunk* wchunks;    // write-phase chunks (PrBlocks::n_write)
+  const PbChunk* schunks;    // sum-phase chunks (PrBlocks::n_sum)
+  const PbMulti* multi;      // bins of several sum chunks (PrBlocks::n_multi)
+};
+// The layout FP32 Jacobi PageRank runs through on g, built on first use
+// (null when it runs without blocks), and its device arrays.
+const gg_graph::PrBlocks* PropagationBlocks(gg_graph* g);
+PbArrays PropagationBlockArrays(gg_graph* g);
 
 // Direction-optimizing BFS (gg_bfs.hip) from device id `source` on
 // g->stream, inside the caller's timed region: parents in the workspace

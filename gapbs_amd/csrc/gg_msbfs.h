@@ -74,7 +74,7 @@ struct MsCounters {
   unsigned long long scout[kStripes];     // out-degree of vertices with new bits
   unsigned long long examined[kStripes];  // adjacency entries read (top-down, light bottom-up)
   unsigned long long heavy_ex[kStripes];  // adjacency entries read by the heavy tier
-  unsigned long long seg_ex[kStripes];    // adjacency entries read by the segment pass
+  unsigned long long blk_ex[kStripes];    // in-entries read by the propagation-block pass
   unsigned long long finished[kStripes];  // in-edges of vertices that became reached by all roots
   unsigned long long live[kStripes];      // OR of the new bits
 };
@@ -95,12 +95,12 @@ struct MsLevel {
   int32_t listed;      // the current frontier is listed in the active list
   int32_t clear;       // to clear in the array just consumed: 0 nothing, 1 its listed entries, 2 all
   int32_t done;        // no frontier left
-  int32_t seg;         // this (bottom-up) level gathers through the source segments first
-  int64_t seg_min;     // ... when more in-edges than this are unfinished
+  int32_t blocks;      // this (bottom-up) level goes through the propagation blocks
+  int64_t blocks_min;  // ... (the first one after a top-down level) when more in-edges than this are unfinished
 };
 
 // Mail posted by MsStep for the host-driven loop.
-enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailSegEx, kMailValues };
+enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailBlocksEx, kMailValues };
 
 __device__ __forceinline__ uint64_t WaveOr(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
@@ -123,10 +123,10 @@ __device__ __forceinline__ unsigned long long BlockOr(unsigned long long v, unsi
 
 // Starts the level state after the roots (their distinct vertices are the
 // first active list).
-__global__ void MsLevelInit(MsCounters* c, MsLevel* L, int64_t unfinished, uint64_t full, int64_t seg_min) {
+__global__ void MsLevelInit(MsCounters* c, MsLevel* L, int64_t unfinished, uint64_t full, int64_t blocks_min) {
   if (threadIdx.x == 0) {
-    L->seg = 0;
-    L->seg_min = seg_min;
+    L->blocks = 0;
+    L->blocks_min = blocks_min;
     const int64_t na = c->active_tail;
     c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
     L->na = na;
@@ -157,10 +157,10 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
         s += c->scout[i];
         ex += c->examined[i];
         hx += c->heavy_ex[i];
-        sx += c->seg_ex[i];
+        sx += c->blk_ex[i];
         fin += c->finished[i];
         lv |= c->live[i];
-        c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->seg_ex[i] = c->finished[i] = c->live[i] = 0;
+        c->awake[i] = c->scout[i] = c->examined[i] = c->heavy_ex[i] = c->blk_ex[i] = c->finished[i] = c->live[i] = 0;
       }
       const int64_t na_next = static_cast<int64_t>(a) + c->active_tail;
       c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
@@ -172,7 +172,7 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
       L->examined += static_cast<int64_t>(ex + hx + sx);
       L->unfinished -= static_cast<int64_t>(fin);
       const int next_bu = static_cast<int64_t>(s) > L->unfinished / alpha;
-      L->seg = next_bu && L->unfinished > L->seg_min;
+      L->blocks = next_bu && !was_bu && L->unfinished > L->blocks_min;
       L->clear = (!keep && na_next > 0 && !next_bu) ? (L->listed ? 1 : 2) : 0;
       L->na_prev = L->na;
       L->listed = !was_bu;
@@ -417,11 +417,11 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
                                                   const uint64_t* __restrict__ frontier, uint64_t* __restrict__ seen,
                                                   uint64_t* __restrict__ fnext, uint16_t* depth, int64_t n,
                                                   const MsLevel* L, uint64_t full, MsCounters* c,
-                                                  const int32_t* __restrict__ tail, uint64_t* __restrict__ acc) {
+                                                  uint64_t* __restrict__ acc) {
   if (L->done || !L->bottom_up) return;
-  // After a segment pass a light vertex's list continues at tail[i], with
-  // the bits found so far in acc[i] (taken here).
-  const bool seg = L->seg && tail;
+  // After a propagation-block pass a light vertex's needed bits are in
+  // acc[i] (taken here) and its list is not walked.
+  const bool blocks = L->blocks;
   const uint16_t level = static_cast<uint16_t>(L->level);
   const uint64_t live = L->live;
   __shared__ int64_t s_beg[kBS];
@@ -451,10 +451,8 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
       }
     }
     s_got[tid] = 0;
-    if (seg && light) {
-      const int t = tail[i];
-      b += t;
-      d -= t;
+    if (blocks && light) {
+      d = 0;
       const uint64_t a = acc[i];
       if (a) {
         s_got[tid] = a;
@@ -573,10 +571,9 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
                                                      const uint64_t* __restrict__ frontier,
                                                      const uint64_t* __restrict__ seen, uint64_t* acc, const MsLevel* L,
                                                      const EdgeChunk* __restrict__ chunks, int32_t nch,
-                                                     MsCounters* c, const int64_t* __restrict__ in_off,
-                                                     const int32_t* __restrict__ tail) {
-  if (L->done || !L->bottom_up) return;
-  const bool seg = L->seg && tail;  // the lists continue at in_off[u] + tail[u]
+                                                     MsCounters* c) {
+  // After a propagation-block pass acc already holds every needed bit.
+  if (L->done || !L->bottom_up || L->blocks) return;
   const uint64_t live = L->live;
   __shared__ unsigned long long s_red[kBS / kWave];
   const int lane = LaneId();
@@ -591,12 +588,7 @@ __global__ __launch_bounds__(kBS) void MsHeavyChunks(const int32_t* __restrict__
     uint64_t before = lane == 0 ? acc[ch.u] : 0;
     before = __shfl(before, 0);
     if ((before & need) == need) continue;
-    int64_t beg = ch.beg;
-    if (seg) {
-      const int64_t from = in_off[ch.u] + tail[ch.u];
-      if (from >= ch.end) continue;
-      if (from > beg) beg = from;
-    }
+    const int64_t beg = ch.beg;
     uint64_t got = 0;
     for (int64_t base = beg; base < ch.end; base += kWave * kUnroll) {
       int32_t nbr[kUnroll];
@@ -671,150 +663,114 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
   }
 }
 
-// Segment pass of a bottom-up level (when MsStep set L->seg): the in-entries
-// regrouped by source segment of 2^18 vertices (2 MB of frontier masks, the
-// source-segment layout of gg_pr.hip), pair by pair in segment order, so the
-// masks gathered at any moment lie in one or two segments and stay in L2.
-// Each (destination, sub-list) pair of a vertex that still needs live bits
-// ORs its sources' masks into acc[u], stopping once acc holds every needed
-// bit; the light and heavy tiers then take the rest of each list from
-// tail[u].  One launch per segment, so a vertex has at most one pair in
-// flight and acc needs no atomics.  Work items ("virtual blocks", handed
-// out grid-stride) come from the parts table: per segment, its block pairs
-// (>= 1024 entries, a block each), wavefront pairs (32..1023, four per
-// item) and thread pairs (< 32, 256 per item).
-struct MsSegPart {
-  int64_t first_vb;  // first work item
-  int64_t p0;        // first pair
-  int64_t count;     // pairs
-  int32_t kind;      // 0 block, 1 wavefront, 2 thread per pair
-  int32_t pad;
-};
-constexpr int kSegBlockPair = 1024;
-constexpr int kSegWavePair = 32;
+// Propagation-block pass of a dense bottom-up level.  MsStep sets L->blocks
+// on the first bottom-up level after a top-down one while more than a set
+// share of the in-edges belongs to vertices some root has not reached: that
+// level reads almost every in-entry (no vertex has every bit yet, so the
+// early exit rarely stops a list), and gathering 8-byte frontier masks at
+// random moves a cache line for each.  Here the level runs through the
+// layout Jacobi PageRank uses, with the masks as the values: every in-entry
+// is read once, in two streams.
+//   write  a block per chunk of a source tile stages the tile's masks in
+//          LDS and writes each in-entry's source mask in block order
+//          (16 bytes per lane);
+//   or     a block per chunk of a destination bin ORs its entries' masks
+//          into one LDS word per destination (a lane ORs its octet's runs
+//          of equal destinations first), then stores, for each vertex that
+//          still needs live bits, the needed bits it found in acc (or the
+//          chunk's words, for a bin of several chunks);
+//   merge  ORs those chunks' words per vertex into acc.
+// The light tier then takes acc instead of walking its lists and the heavy
+// tier stands down; MsHeavyFinish and MsBottomUp finish the level as usual.
+constexpr int kMsBlkBS = 1024;
 
-// ORs the masks of src[b, e) (this wavefront's share: positions b + lane,
-// stepping by `step`) while u still lacks a needed bit; wave-uniform b, e.
-__device__ __forceinline__ uint64_t MsSegWaveOr(const int32_t* __restrict__ src, const uint64_t* __restrict__ frontier,
-                                                const uint64_t* acc, int32_t u, uint64_t need, int64_t b, int64_t e,
-                                                int step, unsigned long long* ex) {
-  const int lane = LaneId();
-  uint64_t got = 0;
-  for (int64_t base = b; base < e; base += int64_t(step) * kUnroll) {
-    int32_t nbr[kUnroll];
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++) {
-      const int64_t k = base + int64_t(q) * step + lane;
-      nbr[q] = k < e ? src[k] : -1;
-    }
-    uint64_t f = 0;
-#pragma unroll
-    for (int q = 0; q < kUnroll; q++)
-      if (nbr[q] >= 0) f |= frontier[nbr[q]];
-    got |= WaveOr(f);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < kUnroll; q++) {
-        const int64_t k0 = base + int64_t(q) * step;
-        if (k0 < e) *ex += static_cast<unsigned long long>(e - k0 < kWave ? e - k0 : kWave);
-      }
-    }
-    uint64_t other = lane == 0 ? acc[u] : 0;
-    other = __shfl(other, 0);
-    if (((got | other) & need) == need) break;
+__global__ __launch_bounds__(kMsBlkBS) void MsBlocksWrite(const uint64_t* __restrict__ frontier, int64_t n, int lg,
+                                                        const uint32_t* __restrict__ src16,
+                                                        const PbChunk* __restrict__ chunks,
+                                                        ulonglong2* __restrict__ val, const MsLevel* L) {
+  if (L->done || !L->bottom_up || !L->blocks) return;
+  __shared__ uint64_t slice[1 << 14];
+  const PbChunk c = chunks[blockIdx.x];
+  const int64_t base = int64_t(c.part) << lg;
+  const int64_t cnt = min(int64_t(1) << lg, n - base);
+  for (int64_t i = threadIdx.x; i < cnt; i += kMsBlkBS) slice[i] = frontier[base + i];
+  __syncthreads();
+  const uint32_t m = (1u << lg) - 1;  // keeps every slice index below 2^lg
+  // Pairs of entries: one 32-bit word of offsets in, 16 bytes of masks out.
+  const int64_t i1 = int64_t(c.q1) * 4;
+  for (int64_t i = int64_t(c.q0) * 4 + threadIdx.x; i < i1; i += 2 * kMsBlkBS) {
+    const bool two = i + kMsBlkBS < i1;
+    const uint32_t w0 = src16[i], w1 = two ? src16[i + kMsBlkBS] : 0u;
+    val[i] = make_ulonglong2(slice[w0 & m], slice[(w0 >> 16) & m]);
+    if (two) val[i + kMsBlkBS] = make_ulonglong2(slice[w1 & m], slice[(w1 >> 16) & m]);
   }
-  return got;
 }
 
-__global__ __launch_bounds__(kBS) void MsSegOr(const MsSegPart* __restrict__ parts, int nparts, int64_t vb0,
-                                               int64_t vb1, const int32_t* __restrict__ src,
-                                               const int32_t* __restrict__ pdst, const int64_t* __restrict__ poff,
-                                               const uint64_t* __restrict__ frontier,
-                                               const uint64_t* __restrict__ seen, uint64_t* acc, const MsLevel* L,
-                                               MsCounters* c) {
-  if (L->done || !L->bottom_up || !L->seg) return;
-  const uint64_t live = L->live;
-  __shared__ unsigned long long s_red[kBS / kWave];
-  __shared__ unsigned long long s_or[kBS / kWave];
-  const int lane = LaneId(), wid = threadIdx.x / kWave;
+__global__ __launch_bounds__(kMsBlkBS) void MsBlocksOr(const uint4* __restrict__ dst16, const uint32_t* __restrict__ oct,
+                                                     const ulonglong2* __restrict__ val,
+                                                     const PbChunk* __restrict__ chunks, int64_t n, int lg,
+                                                     const uint64_t* __restrict__ seen, uint64_t* __restrict__ acc,
+                                                     uint64_t* __restrict__ part, const MsLevel* L, MsCounters* c) {
+  if (L->done || !L->bottom_up || !L->blocks) return;
+  __shared__ unsigned long long s_acc[1 << 14];
+  __shared__ unsigned long long s_red[kMsBlkBS / kWave];
+  const PbChunk ch = chunks[blockIdx.x];
+  const int w = 1 << lg;
+  const uint32_t m = static_cast<uint32_t>(w - 1);
+  for (int i = threadIdx.x; i < w; i += kMsBlkBS) s_acc[i] = 0;
+  __syncthreads();
   unsigned long long ex = 0;
-  for (int64_t vb = vb0 + blockIdx.x; vb < vb1; vb += gridDim.x) {
-    int lo = 0, hi = nparts;  // parts[lo].first_vb <= vb < parts[hi].first_vb
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (parts[mid].first_vb <= vb)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const MsSegPart P = parts[lo];
-    const int64_t rel = vb - P.first_vb;
-    if (P.kind == 2) {  // a thread per pair
-      const int64_t t = rel * kBS + threadIdx.x;
-      if (t < P.count) {
-        const int64_t p = P.p0 + t;
-        const int32_t u = pdst[p];
-        const uint64_t need = live & ~seen[u];
-        if (need && (acc[u] & need) != need) {
-          const int64_t b = poff[p], e = poff[p + 1];
-          uint64_t got = 0;
-          int64_t k = b;
-          for (; k + kUnroll <= e; k += kUnroll) {
-            int32_t nbr[kUnroll];
+  for (int64_t q = int64_t(ch.q0) + threadIdx.x; q < ch.q1; q += kMsBlkBS) {
+    const uint32_t o = oct[q];
+    const uint4 dw = dst16[o];
+    const uint32_t d[8] = {dw.x & 0xFFFFu, dw.x >> 16, dw.y & 0xFFFFu, dw.y >> 16,
+                           dw.z & 0xFFFFu, dw.z >> 16, dw.w & 0xFFFFu, dw.w >> 16};
+    const ulonglong2* p = val + 4 * int64_t(o);
+    const ulonglong2 a = p[0], b = p[1], e = p[2], f = p[3];
+    const uint64_t v[8] = {a.x, a.y, b.x, b.y, e.x, e.y, f.x, f.y};
+    uint64_t run = 0;
 #pragma unroll
-            for (int q = 0; q < kUnroll; q++) nbr[q] = src[k + q];
-#pragma unroll
-            for (int q = 0; q < kUnroll; q++) got |= frontier[nbr[q]];
-          }
-          for (; k < e; k++) got |= frontier[src[k]];
-          ex += static_cast<unsigned long long>(e - b);
-          if (got & need) acc[u] |= got & need;
-        }
-      }
-    } else if (P.kind == 1) {
-      // A wavefront per pair, four pairs per item.  Wave-uniform from here.
-      const int64_t w = rel * (kBS / kWave) + wid;
-      if (w < P.count) {
-        const int64_t p = P.p0 + w;
-        const int32_t u = pdst[p];
-        const uint64_t need = live & ~seen[u];
-        uint64_t before = lane == 0 ? acc[u] : 0;
-        before = __shfl(before, 0);
-        if (need && (before & need) != need) {
-          unsigned long long wex = 0;
-          const uint64_t got = MsSegWaveOr(src, frontier, acc, u, need, poff[p], poff[p + 1], kWave, &wex);
-          if (lane == 0) {
-            ex += wex;
-            if (got & need) acc[u] = before | (got & need);
-          }
-        }
-      }
-    } else {
-      // A block per pair, its wavefronts strided over the sub-list; the
-      // block's bits are combined in LDS and written once.  Block-uniform.
-      const int64_t p = P.p0 + rel;
-      const int32_t u = pdst[p];
-      const uint64_t need = live & ~seen[u];
-      uint64_t before = lane == 0 ? acc[u] : 0;
-      before = __shfl(before, 0);
-      uint64_t got = 0;
-      if (need && (before & need) != need) {
-        unsigned long long wex = 0;
-        got = MsSegWaveOr(src, frontier, acc, u, need, poff[p] + wid * kWave, poff[p + 1], kBS, &wex);
-        if (lane == 0) ex += wex;
-      }
-      if (lane == 0) s_or[wid] = got;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        uint64_t all = 0;
-        for (int q = 0; q < kBS / kWave; q++) all |= s_or[q];
-        if (all & need) acc[u] |= all & need;
-      }
-      __syncthreads();
+    for (int i = 0; i < 8; i++) {
+      if (d[i] == kPbPad) break;  // padding only follows the real entries
+      ex++;
+      run = (i > 0 && d[i] != d[i - 1] ? 0ull : run) | v[i];
+      if ((i == 7 || d[i + 1] != d[i]) && run) atomicOr(&s_acc[d[i] & m], static_cast<unsigned long long>(run));
     }
   }
-  ex = BlockSum<kBS>(ex, s_red);
-  if (threadIdx.x == 0 && ex) atomicAdd(&c->seg_ex[blockIdx.x & (kStripes - 1)], ex);
+  __syncthreads();
+  const int64_t first = int64_t(ch.part) << lg;
+  const int cnt = static_cast<int>(min(int64_t(w), n - first));
+  if (ch.slot < 0) {
+    for (int i = threadIdx.x; i < cnt; i += kMsBlkBS) {
+      const uint64_t got = s_acc[i];
+      if (!got) continue;
+      const uint64_t need = got & ~seen[first + i];  // frontier masks hold only live bits
+      if (need) acc[first + i] = need;
+    }
+  } else {
+    uint64_t* out = part + (int64_t(ch.slot) << lg);
+    for (int i = threadIdx.x; i < cnt; i += kMsBlkBS) out[i] = s_acc[i];
+  }
+  ex = BlockSum<kMsBlkBS>(ex, s_red);
+  if (threadIdx.x == 0 && ex) atomicAdd(&c->blk_ex[blockIdx.x & (kStripes - 1)], ex);
+}
+
+__global__ __launch_bounds__(kBS) void MsBlocksMerge(const PbMulti* __restrict__ multi, int32_t n_multi, int lg,
+                                                     int64_t n, const uint64_t* __restrict__ part,
+                                                     const uint64_t* __restrict__ seen, uint64_t* __restrict__ acc,
+                                                     const MsLevel* L) {
+  if (L->done || !L->bottom_up || !L->blocks) return;
+  const int64_t w = int64_t(1) << lg, total = int64_t(n_multi) << lg;
+  for (int64_t j = blockIdx.x * int64_t(kBS) + threadIdx.x; j < total; j += int64_t(gridDim.x) * kBS) {
+    const PbMulti M = multi[j >> lg];
+    const int64_t i = j & (w - 1), u = (int64_t(M.bin) << lg) + i;
+    if (u >= n) continue;
+    uint64_t got = 0;
+    for (uint32_t s = 0; s < M.nslots; s++) got |= part[(int64_t(M.slot0 + s) << lg) + i];
+    if (!got) continue;
+    const uint64_t need = got & ~seen[u];
+    if (need) acc[u] = need;
+  }
 }
 
 // Dense frontier -> list, before a top-down level that follows a bottom-up
@@ -891,16 +847,17 @@ struct MsBuffers {
   EdgeChunk* chunks;  // null: no chunking (no vertex reaches kExpandBS edges)
   uint16_t* depth16;
   uint64_t full;
-  // Source segments for the bottom-up levels that read most in-edges (all
-  // null when the graph has none).
-  const MsSegPart* seg_parts;
-  int seg_nparts;
-  const std::vector<int64_t>* seg_vb;  // segment s owns work items [seg_vb[s], seg_vb[s + 1])
-  SegArrays seg;
+  // Propagation blocks for the densest bottom-up level (pb null when the
+  // graph runs without them): the layout, its arrays, the 8-byte value per
+  // padded entry and the partial results of bins of several chunks.
+  const gg_graph::PrBlocks* pb;
+  PbArrays pba;
+  uint64_t* pb_val;
+  uint64_t* pb_part;
 };
 
 struct MsLaunched {
-  int td = -1, bu = -1, heavy = -1, seg = -1;  // profile records to credit once the counts are known
+  int td = -1, bu = -1, heavy = -1, blk_write = -1, blk_or = -1;  // profile records to credit once the counts are known
   int kernels = 0;
 };
 
@@ -941,27 +898,33 @@ inline MsLaunched MsLaunchLevel(gg_graph* g, const MsBuffers& b, const uint64_t*
   if (bu) {
     // Per in-edge gathered: 4 B and the neighbour's 8 B frontier mask
     // (credited by the caller).
-    if (b.seg_parts) {
-      const std::vector<int64_t>& vb = *b.seg_vb;
-      for (size_t sg = 0; sg + 1 < vb.size(); sg++) {
-        if (vb[sg + 1] == vb[sg]) continue;
-        const int rec = Run(g, "msbfs_segments", 0, MsSegOr,
-                            dim3(GridFor(vb[sg + 1] - vb[sg], 1, g->cu_count * kLevelBlocksPerCu)), blk, b.seg_parts,
-                            b.seg_nparts, vb[sg], vb[sg + 1], b.seg.src, b.seg.dst, b.seg.off, fcur,
-                            static_cast<const uint64_t*>(b.seen), b.acc, static_cast<const MsLevel*>(b.L), b.dc);
-        if (r.seg < 0) r.seg = rec;  // the level's entries are credited to its first segment launch
+    if (b.pb) {
+      const gg_graph::PrBlocks& P = *b.pb;
+      if (P.n_write) {
+        r.blk_write = Run(g, "msbfs_blocks_write", 0, MsBlocksWrite, dim3(static_cast<unsigned>(P.n_write)), dim3(kMsBlkBS),
+                          fcur, n, P.lg, reinterpret_cast<const uint32_t*>(b.pba.src16), b.pba.wchunks,
+                          reinterpret_cast<ulonglong2*>(b.pb_val), static_cast<const MsLevel*>(b.L));
+        r.kernels++;
+      }
+      r.blk_or = Run(g, "msbfs_blocks_or", 0, MsBlocksOr, dim3(static_cast<unsigned>(P.n_sum)), dim3(kMsBlkBS), b.pba.dst16,
+                     b.pba.oct, reinterpret_cast<const ulonglong2*>(b.pb_val), b.pba.schunks, n, P.lg,
+                     static_cast<const uint64_t*>(b.seen), b.acc, b.pb_part, static_cast<const MsLevel*>(b.L), b.dc);
+      r.kernels++;
+      if (P.n_multi) {
+        Run(g, "msbfs_blocks_merge", 0, MsBlocksMerge, dim3(GridFor(P.n_multi << P.lg, kBS, g->cu_count * 8)), blk,
+            b.pba.multi, static_cast<int32_t>(P.n_multi), P.lg, n, static_cast<const uint64_t*>(b.pb_part),
+            static_cast<const uint64_t*>(b.seen), b.acc, static_cast<const MsLevel*>(b.L));
         r.kernels++;
       }
     }
     // Per vertex: seen (8 B) and fnext (8 B).
     r.bu = Run(g, "msbfs_bottom_up", 16 * N, MsBottomUp, grid_n, blk, static_cast<const int64_t*>(g->in_off),
                static_cast<const int32_t*>(g->in_dst), static_cast<const int64_t*>(g->out_off), fcur, b.seen,
-               fnext, b.depth16, n, static_cast<const MsLevel*>(b.L), b.full, b.dc, b.seg.tail, b.acc);
+               fnext, b.depth16, n, static_cast<const MsLevel*>(b.L), b.full, b.dc, b.acc);
     r.heavy = Run(g, "msbfs_bottom_up_heavy", 0, MsHeavyChunks,
                   dim3(GridFor(int64_t(b.nhchunks) * kWave, kBS, g->cu_count * kLevelBlocksPerCu)), blk,
                   static_cast<const int32_t*>(g->in_dst), fcur, static_cast<const uint64_t*>(b.seen), b.acc,
-                  static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc,
-                  static_cast<const int64_t*>(g->in_off), b.seg.tail);
+                  static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc);
     Run(g, "msbfs_heavy_finish", 32.0 * b.nheavy, MsHeavyFinish, dim3(GridFor(b.nheavy, kBS, g->cu_count * 4)), blk,
         b.heavy, b.nheavy, b.seen, b.acc, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
         static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, b.dc);
@@ -1054,50 +1017,25 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   int32_t* touched = g->Scratch<int32_t>("ms.touched", n + 64);
   const bool use_chunks = g->max_degree >= kExpandBS;
   EdgeChunk* chunks = use_chunks ? g->Scratch<EdgeChunk>("ms.chunks", MaxChunks(n, g->out_entries)) : nullptr;
-  // Source segments (8-byte values: the frontier masks), built on first use;
-  // the parts table is rebuilt per call (a few hundred bytes).  Off unless
-  // asked for: at Kronecker 26 (64 roots, alpha 4) the segment pass took
-  // 20 ms of a 34.2 ms traversal against 32.0 ms without it, the light and
-  // heavy tiers' early exit already stopping most lists after a few entries.
-  const gg_graph::PrSegments* SL = g->tune.bfs_seg_pct > 0 ? SourceSegments(g, true) : nullptr;
-  MsSegPart* seg_parts = nullptr;
-  int seg_nparts = 0;
-  int64_t seg_nvb = 0, seg_min = INT64_MAX;
-  std::vector<int64_t> seg_vb;
-  SegArrays seg_arrays{nullptr, nullptr, nullptr, nullptr};
-  if (SL) {
-    std::vector<MsSegPart> parts;
-    for (int sg = 0; sg < SL->ns; sg++) {
-      seg_vb.push_back(seg_nvb);
-      const int64_t p0 = SL->first_pair[sg], nl = SL->large[sg], nm = SL->medium[sg];
-      const int64_t nsm = SL->first_pair[sg + 1] - p0 - nl - nm;
-      if (nl) {
-        parts.push_back(MsSegPart{seg_nvb, p0, nl, 0, 0});
-        seg_nvb += nl;
-      }
-      if (nm) {
-        parts.push_back(MsSegPart{seg_nvb, p0 + nl, nm, 1, 0});
-        seg_nvb += (nm + kBS / kWave - 1) / (kBS / kWave);
-      }
-      if (nsm) {
-        parts.push_back(MsSegPart{seg_nvb, p0 + nl + nm, nsm, 2, 0});
-        seg_nvb += (nsm + kBS - 1) / kBS;
-      }
-    }
-    seg_vb.push_back(seg_nvb);
-    if (seg_nvb > 0) {
-      seg_nparts = static_cast<int>(parts.size());
-      seg_parts = g->Scratch<MsSegPart>("ms.seg_parts", parts.size());
-      GG_HIP(hipMemcpyAsync(seg_parts, parts.data(), sizeof(MsSegPart) * parts.size(), hipMemcpyHostToDevice, st));
-      seg_arrays = SourceSegmentArrays(g, SL, true);
-      const int pct = g->tune.bfs_seg_pct;
-      seg_min = g->in_entries * pct / 100;
-      if (pct == 100) seg_min = g->in_entries - 1;
-    }
+  // Propagation blocks (the layout FP32 Jacobi PageRank runs through, built
+  // on first use) for the first bottom-up level after a top-down one while
+  // more than bfs_blocks_pct percent (default 50) of the in-edges are
+  // unfinished.  The values share FP64 PageRank's arrays (8 bytes each).
+  const gg_graph::PrBlocks* PB = g->tune.bfs_blocks_pct >= 0 ? PropagationBlocks(g) : nullptr;
+  PbArrays pba{};
+  uint64_t* pb_val = nullptr;
+  uint64_t* pb_part = nullptr;
+  int64_t blocks_min = INT64_MAX;
+  if (PB) {
+    pba = PropagationBlockArrays(g);
+    pb_val = g->Scratch<uint64_t>("pr.pb.val64", 8 * PB->octets + 8);
+    pb_part = g->Scratch<uint64_t>("pr.pb.part64", std::max<int64_t>(1, PB->slots << PB->lg));
+    const int pct = g->tune.bfs_blocks_pct > 0 ? g->tune.bfs_blocks_pct : 50;
+    blocks_min = pct == 100 ? g->in_entries - 1 : g->in_entries * pct / 100;
   }
   const MsBuffers b{dc,   dl,  heavy,   hchunks, static_cast<int32_t>(g->ms_heavy), static_cast<int32_t>(g->ms_heavy_chunks),
                     seen, acc, touched, chunks,  depth16,                            full,
-                    seg_parts, seg_nparts, &seg_vb, seg_arrays};
+                    PB, pba, pb_val, pb_part};
   const dim3 grid_n(GridFor(n, kBS, g->cu_count * 16));
   const double N = static_cast<double>(n);
   int64_t& launches = res.launches;
@@ -1106,7 +1044,7 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   Run(g, "msbfs_init", 32 * N, MsInit, grid_n, blk, seen, fa, fb, acc, n, dc);
   Run(g, "msbfs_roots", 0, MsRoots, dim3(GridFor(count, 64)), dim3(64), droots, dbits, count, seen, fa, depth16,
       act_a, dc);
-  Run(g, "msbfs_level_init", 0, MsLevelInit, dim3(1), dim3(64), dc, dl, g->in_entries, full, seg_min);
+  Run(g, "msbfs_level_init", 0, MsLevelInit, dim3(1), dim3(64), dc, dl, g->in_entries, full, blocks_min);
   launches += 3;
 
   MsLevel hl;
@@ -1119,9 +1057,12 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
         reinterpret_cast<uintptr_t>(chunks),  reinterpret_cast<uintptr_t>(hchunks), reinterpret_cast<uintptr_t>(heavy),
         reinterpret_cast<uintptr_t>(depth16), static_cast<uintptr_t>(full),         static_cast<uintptr_t>(alpha),
         static_cast<uintptr_t>(g->ms_heavy),  static_cast<uintptr_t>(g->ms_heavy_chunks),
-        reinterpret_cast<uintptr_t>(seg_parts), static_cast<uintptr_t>(seg_nparts), static_cast<uintptr_t>(seg_nvb),
-        reinterpret_cast<uintptr_t>(seg_arrays.src), reinterpret_cast<uintptr_t>(seg_arrays.dst),
-        reinterpret_cast<uintptr_t>(seg_arrays.off), reinterpret_cast<uintptr_t>(seg_arrays.tail)};
+        reinterpret_cast<uintptr_t>(PB), reinterpret_cast<uintptr_t>(pba.src16), reinterpret_cast<uintptr_t>(pba.dst16),
+        reinterpret_cast<uintptr_t>(pba.oct), reinterpret_cast<uintptr_t>(pba.wchunks),
+        reinterpret_cast<uintptr_t>(pba.schunks), reinterpret_cast<uintptr_t>(pba.multi),
+        reinterpret_cast<uintptr_t>(pb_val), reinterpret_cast<uintptr_t>(pb_part),
+        static_cast<uintptr_t>(PB ? PB->lg : 0), static_cast<uintptr_t>(PB ? PB->n_write : 0),
+        static_cast<uintptr_t>(PB ? PB->n_sum : 0), static_cast<uintptr_t>(PB ? PB->n_multi : 0)};
     int per_replay = 0;
     if (!g->ms_exec || g->ms_exec_key != key) {
       if (g->ms_exec) {
@@ -1201,7 +1142,11 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
       if (r.td >= 0) CreditBytes(g, r.td, 20.0 * mail[kMailExamined]);
       if (r.bu >= 0) CreditBytes(g, r.bu, 12.0 * mail[kMailExamined]);
       if (r.heavy >= 0) CreditBytes(g, r.heavy, 12.0 * mail[kMailHeavyEx]);
-      if (r.seg >= 0) CreditBytes(g, r.seg, 12.0 * mail[kMailSegEx]);
+      if (mail[kMailBlocksEx] && b.pb) {  // the blocks pass ran: its streams per padded entry, and 8 B per vertex
+        const double ent = 8.0 * b.pb->octets;
+        if (r.blk_write >= 0) CreditBytes(g, r.blk_write, 10.0 * ent + 8 * N);
+        if (r.blk_or >= 0) CreditBytes(g, r.blk_or, 10.5 * ent + 8 * N);
+      }
       const int64_t na_next = static_cast<int64_t>(mail[kMailNa]);
       const bool next_bu = mail[kMailBottomUp] != 0;
       // The clears MsStep chose for the array just consumed.

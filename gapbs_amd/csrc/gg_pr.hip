@@ -893,7 +893,6 @@ constexpr int kPbMaxTiles = 8192;  // build tables hold k x k octet counts
 constexpr int kPbWriteBS = 1024;
 constexpr int kPbSumBS = 1024;
 constexpr int kPbBuildBS = 1024;
-constexpr uint32_t kPbPad = 0xFFFFu;
 
 // The per-destination accumulator word of each precision.
 template <class T>
@@ -907,14 +906,6 @@ struct PbAcc<double> {
   using Word = double;
 };
 
-struct PbChunk {
-  uint32_t part;   // the tile (write phase) or bin (sum phase)
-  int32_t slot;    // sum phase: partial-sum slot, -1 when the chunk is the whole bin
-  uint32_t q0, q1;  // write: octets [q0, q1); sum: octet-index positions [q0, q1)
-};
-struct PbMulti {
-  uint32_t bin, slot0, nslots, pad;  // a bin summed in nslots chunks, slots slot0...
-};
 
 template <class T>
 struct PbArgs {
@@ -1592,16 +1583,15 @@ void RunPageRank(gg_graph* g, int max_iters, double epsilon, bool in_place, floa
 
 }  // namespace
 
-const gg_graph::PrSegments* SourceSegments(gg_graph* g, bool fp64) {
-  return fp64 ? Segments<double>(g) : Segments<float>(g);
-}
+const gg_graph::PrBlocks* PropagationBlocks(gg_graph* g) { return Blocks(g, false); }
 
-SegArrays SourceSegmentArrays(gg_graph* g, const gg_graph::PrSegments* L, bool fp64) {
-  const std::string tag = fp64 ? "pr.seg64." : "pr.seg32.";
-  return SegArrays{g->Scratch<int32_t>((tag + "src").c_str(), L->hot_entries),
-                   g->Scratch<int32_t>((tag + "dst").c_str(), L->pairs),
-                   g->Scratch<int64_t>((tag + "off").c_str(), L->pairs + 1),
-                   g->Scratch<int32_t>((tag + "tail").c_str(), g->n)};
+PbArrays PropagationBlockArrays(gg_graph* g) {
+  return PbArrays{static_cast<const uint4*>(g->ws.Get("pr.pb.src", 0)),
+                  static_cast<const uint4*>(g->ws.Get("pr.pb.dst", 0)),
+                  static_cast<const uint32_t*>(g->ws.Get("pr.pb.oct", 0)),
+                  static_cast<const PbChunk*>(g->ws.Get("pr.pb.wchunks", 0)),
+                  static_cast<const PbChunk*>(g->ws.Get("pr.pb.schunks", 0)),
+                  static_cast<const PbMulti*>(g->ws.Get("pr.pb.multi", 0))};
 }
 
 }  // namespace gg

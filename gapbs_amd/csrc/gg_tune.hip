@@ -219,8 +219,8 @@ extern "C" int gg_tuning_set(gg_graph* g, const gg_tuning* in) {
     }
     GG_REQUIRE(in->pr_seg_min_density >= 0 && in->pr_seg_min_density <= 64, GG_EINVAL,
                "PageRank segment density out of range");
-    GG_REQUIRE(in->bfs_seg_pct >= -1 && in->bfs_seg_pct <= 100, GG_EINVAL,
-               "batched BFS segment threshold out of range");
+    GG_REQUIRE(in->bfs_blocks_pct >= -1 && in->bfs_blocks_pct <= 100, GG_EINVAL,
+               "batched BFS block threshold out of range");
     GG_REQUIRE(in->pr_blocks_log2 == -1 || in->pr_blocks_log2 == 0 ||
                    (in->pr_blocks_log2 >= 4 && in->pr_blocks_log2 <= 14),
                GG_EINVAL, "PageRank block width out of range");

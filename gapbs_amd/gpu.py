@@ -63,7 +63,7 @@ class Tuning(C.Structure):
                 ("sssp_delta", C.c_int32), ("pr_small_max", C.c_int32), ("pr_medium_max", C.c_int32),
                 ("pr_blocks_per_cu", C.c_int32), ("tuned", C.c_int32), ("best_ms", C.c_double * 4),
                 ("pr_segment_log2", C.c_int32 * 2), ("pr_lds_log2", C.c_int32 * 2),
-                ("pr_seg_min_density", C.c_int32), ("bfs_seg_pct", C.c_int32), ("pr_blocks_log2", C.c_int32)]
+                ("pr_seg_min_density", C.c_int32), ("bfs_blocks_pct", C.c_int32), ("pr_blocks_log2", C.c_int32)]
     _ARRAYS = ("best_ms", "pr_segment_log2", "pr_lds_log2")
 
     def as_dict(self) -> dict:

@@ -337,12 +337,12 @@ typedef struct gg_tuning {
    * are at least this many per (destination, sub-list) pair; 0 picks the
    * default (3), else 1..64. */
   int32_t pr_seg_min_density;
-  /* Batched BFS: a bottom-up level gathers the frontier masks through the
-   * 8-byte source-segment layout (the one FP64 PageRank uses) when more than
-   * this percentage of the in-edges belong to vertices some root has not
-   * reached; 0 and -1 never (the default: slower than the tiers' own early
-   * exit on the graphs measured), 1..100. */
-  int32_t bfs_seg_pct;
+  /* Batched BFS (and batched BC's traversal): the first bottom-up level
+   * after a top-down one reads every in-entry once through the propagation
+   * blocks (the layout FP32 Jacobi PageRank uses, see pr_blocks_log2) when
+   * more than this percentage of the in-edges belong to vertices some root
+   * has not reached; 0 picks the default (50), -1 never, 1..100. */
+  int32_t bfs_blocks_pct;
   /* Jacobi PageRank propagation blocks (both precisions): log2 of the
    * tile / bin width in vertices.  0 picks the default: on graphs of 2^20
    * to 2^27 vertices whose pr_segment_log2 for the precision is left at 0,

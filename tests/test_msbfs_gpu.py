@@ -214,24 +214,25 @@ def test_graph_replayed_levels_match_host_loop():
                 assert np.array_equal(a.depths[i], oracle.bfs_depth(g, int(roots[i]))), (tag, k, i)
 
 
-def _force_segments(dg, width_log2, pct=1):
-    """Source segments of 2^width_log2 sources for the 8-byte layout and a
-    segment pass on every bottom-up level (pct 1: whenever more than 1% of
-    the in-edges are unfinished)."""
-    gpu.set_tuning(dg, {"pr_segment_log2": [0, width_log2], "bfs_seg_pct": pct})
+def _force_blocks(dg, width_log2, pct=1):
+    """Propagation blocks of 2^width_log2 vertices, taken by the first
+    bottom-up level after a top-down one whenever more than pct percent of
+    the in-edges are unfinished (pct 1: practically always)."""
+    gpu.set_tuning(dg, {"pr_blocks_log2": width_log2, "bfs_blocks_pct": pct})
 
 
 @pytest.mark.parametrize("ordered", [True, False])
-@pytest.mark.parametrize("width_log2", [6, 9, 12])
-def test_batch_segment_pass_matches_oracle(ordered, width_log2):
-    # The bottom-up levels gather first through the source segments (pairs
-    # of all three kinds at these widths), then the light and heavy tiers
-    # take the rest of each list; depths must equal the oracle's for every
-    # root, through the captured level loop and the profiled host loop.
-    print(f"[msbfs] segment pass ordered={ordered} width=2^{width_log2}", flush=True)
+@pytest.mark.parametrize("width_log2", [4, 7, 10, 14])
+def test_batch_blocks_pass_matches_oracle(ordered, width_log2):
+    # The first bottom-up level ORs every in-entry's frontier mask through
+    # the propagation blocks (at 2^4 vertices: 4096 tiles, bins summed in
+    # several chunks and merged; at 2^14: four tiles), then the light and
+    # heavy tiers finish it from acc; depths must equal the oracle's for
+    # every root, through the captured level loop and the profiled host loop.
+    print(f"[msbfs] blocks pass ordered={ordered} width=2^{width_log2}", flush=True)
     g = cached_graph("g16", lambda: Graph.kronecker(16))
     dg = gpu.DeviceGraph(g, order_by_degree=ordered)
-    _force_segments(dg, width_log2)
+    _force_blocks(dg, width_log2)
     roots = g.pick_sources(64)
     for alpha in (15, 1000000):
         res = check_batch(g, dg, roots, alpha)
@@ -240,41 +241,51 @@ def test_batch_segment_pass_matches_oracle(ordered, width_log2):
         again = gpu.bfs_batch(dg, roots, alpha)
         rep = dg.profile_report()
         dg.profile(False)
-        assert "msbfs_segments" in rep, rep.keys()
+        assert "msbfs_blocks_or" in rep and rep["msbfs_blocks_or"]["alg_bytes"] > 0, rep.keys()
         assert np.array_equal(again.depths, res.depths)
+        assert again.stats["edges_examined"] >= int(g.in_offsets[-1])  # the pass reads every in-entry once
     dg.free()
 
 
-def test_batch_segment_pass_hubs_and_components():
-    # Star of stars (heavy vertices whose chunks start past the segments),
-    # isolated vertices, a second component, repeated roots.
-    print("[msbfs] segment pass hubs", flush=True)
+def test_batch_blocks_pass_hubs_and_components():
+    # Star of stars (heavy vertices, finished from acc by the heavy tier's
+    # second pass), isolated vertices, a second component, repeated roots.
+    print("[msbfs] blocks pass hubs", flush=True)
     a = np.zeros(9000, np.int32)
     b = np.full(5000, 1, np.int32)
     u = np.concatenate([a, b, [0, 20000, 20001]])
     v = np.concatenate([np.arange(2, 9002), np.arange(9002, 14002), [1, 20001, 20002]])
     g = Graph.from_edges(u, v)
     dg = gpu.DeviceGraph(g)
-    _force_segments(dg, 8)
+    _force_blocks(dg, 8)
     check_batch(g, dg, [0, 5, 9500, 1, 13000, 20000, 14500, 5, 0], 1000000)
     dg.free()
 
 
-def test_batch_segment_pass_in_bc_and_default_off():
-    # The batched BC's traversal (keep mode) takes the same path; default
-    # settings take no segment pass (and give the same depths).
-    print("[msbfs] segment pass in bc", flush=True)
+def test_batch_blocks_pass_in_bc_and_default_off():
+    # The batched BC's traversal (keep mode) takes the same path; on a graph
+    # below 2^20 vertices the default takes no blocks pass (and gives the
+    # same depths); -1 turns it off on any graph.
+    print("[msbfs] blocks pass in bc", flush=True)
     g = cached_graph("g16", lambda: Graph.kronecker(16))
     plain = gpu.DeviceGraph(g, order_by_degree=True)
     roots = g.pick_sources(16)
     want = gpu.bc_batch(plain, roots).scores
     plain.profile(True)
     gpu.bfs_batch(plain, roots)
-    assert "msbfs_segments" not in plain.profile_report()
+    assert "msbfs_blocks_or" not in plain.profile_report()
     plain.profile(False)
-    seg = gpu.DeviceGraph(g, order_by_degree=True)
-    _force_segments(seg, 10)
-    assert np.array_equal(gpu.bc_batch(seg, roots).scores, want)
-    assert np.array_equal(gpu.bfs_batch(seg, roots).depths, gpu.bfs_batch(plain, roots).depths)
+    blk = gpu.DeviceGraph(g, order_by_degree=True)
+    _force_blocks(blk, 10)
+    assert np.array_equal(gpu.bc_batch(blk, roots).scores, want)
+    assert np.array_equal(gpu.bfs_batch(blk, roots).depths, gpu.bfs_batch(plain, roots).depths)
+    gpu.set_tuning(blk, {"bfs_blocks_pct": -1})
+    blk.profile(True)
+    gpu.bfs_batch(blk, roots)
+    assert "msbfs_blocks_or" not in blk.profile_report()
+    blk.profile(False)
+    with pytest.raises(gpu.GpuError) as e:
+        gpu.set_tuning(blk, {"bfs_blocks_pct": 101})
+    assert e.value.code == -22
     plain.free()
-    seg.free()
+    blk.free()

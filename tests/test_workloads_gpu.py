@@ -103,7 +103,11 @@ class TestKronecker22:
         say("[k22] cc (Afforest and Shiloach-Vishkin)")
         want = oracle.cc_min_labels(k22["g"])
         assert np.array_equal(gpu.cc(k22["dg"]).comp, want)
-        assert np.array_equal(gpu.cc_sv(k22["dg"]).comp, want)
+        # Repeated: with labels written by plain stores (not coherent across
+        # the XCDs' L2 caches), SV on this degree-ordered graph ended with a
+        # wrong label in 5 of 20 runs.
+        for rep in range(8):
+            assert np.array_equal(gpu.cc_sv(k22["dg"]).comp, want), rep
 
     def test_bc(self, k22):
         g, dg, roots = k22["g"], k22["dg"], k22["roots"]
