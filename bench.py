@@ -687,7 +687,8 @@ def main():
             pdevs = devs + ([packed[0]] if k == "packed" else [])
             prof, psec = profiled(pdevs, lambda: run_kernel(k), 1 if k in STEP_CAP else min(steps, 2))
             r["profiled_seconds"] = round(psec, 6)
-            r["roofline"] = roofline(prof, traffic, copy_gbs)
+            # (the packed graphs are another workload: no counter traffic for them)
+            r["roofline"] = roofline(prof, {} if k == "packed" else traffic, copy_gbs)
             r["kernels"] = kernel_table(prof)
         if not args.no_e2e and k in ("bfs_batch", "bfs", "sssp", "sssp_batch", "pr", "pr_fp64", "cc", "bc",
                                      "bc_batch"):
