@@ -1021,7 +1021,11 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
   // on first use) for the first bottom-up level after a top-down one while
   // more than bfs_blocks_pct percent (default 50) of the in-edges are
   // unfinished.  The values share FP64 PageRank's arrays (8 bytes each).
-  const gg_graph::PrBlocks* PB = g->tune.bfs_blocks_pct >= 0 ? PropagationBlocks(g) : nullptr;
+  // By default only for batches of 32 roots or more: with fewer roots the
+  // gathers' early exit stops lists sooner (Kronecker 22, 16 roots: 1.54 ms
+  // without the pass, 2.09 ms with it).
+  const bool blocks_wanted = g->tune.bfs_blocks_pct > 0 || (g->tune.bfs_blocks_pct == 0 && __builtin_popcountll(full) >= 32);
+  const gg_graph::PrBlocks* PB = blocks_wanted ? PropagationBlocks(g) : nullptr;
   PbArrays pba{};
   uint64_t* pb_val = nullptr;
   uint64_t* pb_part = nullptr;

@@ -67,6 +67,22 @@ void TuneBatch(gg_graph* g, const int32_t* roots, int k) {
       g->tune.batch_alpha = a;
     }
   }
+  // On a graph with propagation blocks and the default block policy: the
+  // first bottom-up level with and without the blocks pass, at that alpha.
+  int64_t info[6];
+  Check(gg_pagerank_blocks(g, GG_PR_FP32, info));
+  if (info[0] > 0 && g->tune.bfs_blocks_pct == 0) {
+    int choice = 0;
+    for (int pct : {50, -1}) {
+      g->tune.bfs_blocks_pct = pct;
+      const double t = Time([&](gg_stats* st) { return gg_bfs_batch(g, roots, kk, g->tune.batch_alpha, nullptr, st); });
+      if (t < best) {
+        best = t;
+        choice = pct;
+      }
+    }
+    g->tune.bfs_blocks_pct = choice;
+  }
   g->tune.best_ms[1] = best;
   g->tune.tuned |= GG_TUNE_BATCH;
 }

@@ -303,7 +303,9 @@ int gg_graph_vertex_order(gg_graph* g, int32_t* new_id);
  * graph from the given roots (k >= 1; SourcePicker roots are the natural
  * choice) and keeps the fastest in the handle:
  *   GG_TUNE_BFS     gg_bfs alpha x beta over {2..240} x {2..162}
- *   GG_TUNE_BATCH   gg_bfs_batch / gg_bfs_packed alpha over {1..1000}
+ *   GG_TUNE_BATCH   gg_bfs_batch / gg_bfs_packed alpha over {1..1000}, then
+ *                   on a graph with propagation blocks the first bottom-up
+ *                   level with and without them (bfs_blocks_pct 50 or -1)
  *   GG_TUNE_SSSP    gg_sssp delta over powers of two 1..4096 (weighted
  *                   uploads only; slow candidates are cut off early)
  *   GG_TUNE_PR      on a graph whose Jacobi PageRank runs through
@@ -341,7 +343,9 @@ typedef struct gg_tuning {
    * after a top-down one reads every in-entry once through the propagation
    * blocks (the layout FP32 Jacobi PageRank uses, see pr_blocks_log2) when
    * more than this percentage of the in-edges belong to vertices some root
-   * has not reached; 0 picks the default (50), -1 never, 1..100. */
+   * has not reached; 0 picks the default (50, for batches of 32 roots or
+   * more; GG_TUNE_BATCH sets 50 or -1 by timing both), -1 never, 1..100
+   * (whatever the number of roots). */
   int32_t bfs_blocks_pct;
   /* Jacobi PageRank propagation blocks (both precisions): log2 of the
    * tile / bin width in vertices.  0 picks the default: on graphs of 2^20
