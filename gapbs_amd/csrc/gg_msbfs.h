@@ -77,6 +77,10 @@ struct MsCounters {
   unsigned long long blk_ex[kStripes];    // in-entries read by the propagation-block pass
   unsigned long long finished[kStripes];  // in-edges of vertices that became reached by all roots
   unsigned long long live[kStripes];      // OR of the new bits
+  // Depth-row bytes written (and read first) by the three kernels that
+  // write depths: [0] the bottom-up light tier, [1] the top-down update,
+  // [2] the heavy tier's finish (for the profile's byte credits).
+  unsigned long long depth_bytes[3][kStripes];
 };
 
 // Level state, kept on the device and advanced by MsStep after every level,
@@ -100,7 +104,10 @@ struct MsLevel {
 };
 
 // Mail posted by MsStep for the host-driven loop.
-enum { kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailBlocksEx, kMailValues };
+enum {
+  kMailNa, kMailBottomUp, kMailDone, kMailExamined, kMailHeavyEx, kMailLevel, kMailBlocksEx,
+  kMailDepthBottomUp, kMailDepthUpdate, kMailDepthHeavy, kMailValues
+};
 
 __device__ __forceinline__ uint64_t WaveOr(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
@@ -151,8 +158,12 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
                        unsigned long long seq) {
   if (threadIdx.x == 0) {
     if (!L->done) {
-      unsigned long long a = 0, s = 0, ex = 0, hx = 0, sx = 0, fin = 0, lv = 0;
+      unsigned long long a = 0, s = 0, ex = 0, hx = 0, sx = 0, fin = 0, lv = 0, db[3] = {0, 0, 0};
       for (int i = 0; i < kStripes; i++) {
+        for (int k = 0; k < 3; k++) {
+          db[k] += c->depth_bytes[k][i];
+          c->depth_bytes[k][i] = 0;
+        }
         a += c->awake[i];
         s += c->scout[i];
         ex += c->examined[i];
@@ -185,11 +196,11 @@ __global__ void MsStep(MsCounters* c, MsLevel* L, int alpha, int keep, unsigned 
         const unsigned long long vals[kMailValues] = {static_cast<unsigned long long>(na_next),
                                                       static_cast<unsigned long long>(next_bu),
                                                       static_cast<unsigned long long>(L->done), ex, hx,
-                                                      static_cast<unsigned long long>(L->level), sx};
+                                                      static_cast<unsigned long long>(L->level), sx, db[0], db[1], db[2]};
         MailPost(mb, vals, kMailValues, seq);
       }
     } else if (mb) {
-      const unsigned long long vals[kMailValues] = {0, 0, 1, 0, 0, static_cast<unsigned long long>(L->level), 0};
+      const unsigned long long vals[kMailValues] = {0, 0, 1, 0, 0, static_cast<unsigned long long>(L->level), 0, 0, 0, 0};
       MailPost(mb, vals, kMailValues, seq);
     }
   }
@@ -206,7 +217,8 @@ __global__ __launch_bounds__(kBS) void MsInit(uint64_t* __restrict__ seen, uint6
   }
   if (blockIdx.x == 0 && threadIdx.x < kStripes) {
     const int t = threadIdx.x;
-    c->awake[t] = c->scout[t] = c->examined[t] = c->heavy_ex[t] = c->finished[t] = c->live[t] = 0;
+    c->awake[t] = c->scout[t] = c->examined[t] = c->heavy_ex[t] = c->blk_ex[t] = c->finished[t] = c->live[t] = 0;
+    c->depth_bytes[0][t] = c->depth_bytes[1][t] = c->depth_bytes[2][t] = 0;
     if (t == 0) c->active_tail = c->touched_tail = c->compact_tail = c->nchunks = 0;
   }
 }
@@ -275,6 +287,17 @@ __device__ __forceinline__ void WaveWriteDepths(uint16_t* depth, int32_t w, uint
       *q = x;
     }
   }
+}
+
+// Bytes WaveWriteDepths moves for one vertex: 16 per 8-root chunk with new
+// bits, and 16 more for each such chunk it reads first (one that already
+// held depths).
+__device__ __forceinline__ unsigned long long DepthBytes(uint64_t nb, uint64_t prior) {
+  unsigned long long b = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    if ((nb >> (8 * c)) & 0xFFu) b += ((prior >> (8 * c)) & 0xFFu) ? 32 : 16;
+  return b;
 }
 
 // ---- top-down ---------------------------------------------------------------
@@ -364,7 +387,7 @@ __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touc
   __shared__ unsigned long long s_red[kBS / kWave];
   LdsQueueInit(lq);
   const int32_t nt = cc->touched_tail;
-  unsigned long long scout = 0, fin = 0, lv = 0;
+  unsigned long long scout = 0, fin = 0, lv = 0, db = 0;
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < nt; base += int64_t(gridDim.x) * kBS) {
     const int64_t i = base + threadIdx.x;
     bool take = false;
@@ -384,15 +407,20 @@ __global__ __launch_bounds__(kBS) void MsUpdate(const int32_t* __restrict__ touc
         take = true;
       }
     }
-    if (depth) WaveWriteDepths(depth, w, nb, s, level);
+    if (depth) {
+      WaveWriteDepths(depth, w, nb, s, level);
+      if (nb) db += DepthBytes(nb, s);
+    }
     LdsQueuePush(lq, take, w, active, &c->active_tail);
   }
   LdsQueueFlush(lq, active, &c->active_tail);
   scout = BlockSum<kBS>(scout, s_red);
   fin = BlockSum<kBS>(fin, s_red);
+  db = BlockSum<kBS>(db, s_red);
   lv = BlockOr<kBS>(lv, s_red);
   if (threadIdx.x == 0) {
     const int slot = blockIdx.x & (kStripes - 1);
+    if (db) atomicAdd(&c->depth_bytes[1][slot], db);
     if (scout) atomicAdd(&c->scout[slot], scout);
     if (fin) atomicAdd(&c->finished[slot], fin);
     if (lv) atomicOr(&c->live[slot], lv);
@@ -430,7 +458,7 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
   __shared__ int s_scan[kBS / kWave];
   __shared__ unsigned long long s_red[kBS / kWave];
   const int tid = threadIdx.x;
-  unsigned long long awake = 0, scout = 0, ex = 0, fin = 0, lv = 0;
+  unsigned long long awake = 0, scout = 0, ex = 0, fin = 0, lv = 0, db = 0;
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < n; base += int64_t(gridDim.x) * kBS) {
     const int64_t i = base + tid;
     uint64_t s = 0;
@@ -518,16 +546,21 @@ __global__ __launch_bounds__(kBS) void MsBottomUp(const int64_t* __restrict__ in
         awake++;
       }
     }
-    if (depth) WaveWriteDepths(depth, static_cast<int32_t>(i), nb, s, level);
+    if (depth) {
+      WaveWriteDepths(depth, static_cast<int32_t>(i), nb, s, level);
+      if (nb) db += DepthBytes(nb, s);
+    }
     __syncthreads();
   }
   awake = BlockSum<kBS>(awake, s_red);
   scout = BlockSum<kBS>(scout, s_red);
   ex = BlockSum<kBS>(ex, s_red);
   fin = BlockSum<kBS>(fin, s_red);
+  db = BlockSum<kBS>(db, s_red);
   lv = BlockOr<kBS>(lv, s_red);
   if (threadIdx.x == 0) {
     const int slot = blockIdx.x & (kStripes - 1);
+    if (db) atomicAdd(&c->depth_bytes[0][slot], db);
     if (awake) atomicAdd(&c->awake[slot], awake);
     if (scout) atomicAdd(&c->scout[slot], scout);
     if (ex) atomicAdd(&c->examined[slot], ex);
@@ -627,7 +660,7 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
   if (L->done || !L->bottom_up) return;
   const uint16_t level = static_cast<uint16_t>(L->level);
   __shared__ unsigned long long s_red[kBS / kWave];
-  unsigned long long awake = 0, scout = 0, fin = 0, lv = 0;
+  unsigned long long awake = 0, scout = 0, fin = 0, lv = 0, db = 0;
   // Block-uniform trip count: the depth writer needs whole wavefronts.
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < nheavy; base += int64_t(gridDim.x) * kBS) {
     const int64_t i = base + threadIdx.x;
@@ -648,14 +681,19 @@ __global__ __launch_bounds__(kBS) void MsHeavyFinish(const int32_t* __restrict__
         awake++;
       }
     }
-    if (depth) WaveWriteDepths(depth, v, nb, s, level);
+    if (depth) {
+      WaveWriteDepths(depth, v, nb, s, level);
+      if (nb) db += DepthBytes(nb, s);
+    }
   }
   awake = BlockSum<kBS>(awake, s_red);
   scout = BlockSum<kBS>(scout, s_red);
   fin = BlockSum<kBS>(fin, s_red);
+  db = BlockSum<kBS>(db, s_red);
   lv = BlockOr<kBS>(lv, s_red);
   if (threadIdx.x == 0) {
     const int slot = blockIdx.x & (kStripes - 1);
+    if (db) atomicAdd(&c->depth_bytes[2][slot], db);
     if (awake) atomicAdd(&c->awake[slot], awake);
     if (scout) atomicAdd(&c->scout[slot], scout);
     if (fin) atomicAdd(&c->finished[slot], fin);
@@ -857,7 +895,7 @@ struct MsBuffers {
 };
 
 struct MsLaunched {
-  int td = -1, bu = -1, heavy = -1, blk_write = -1, blk_or = -1;  // profile records to credit once the counts are known
+  int td = -1, update = -1, bu = -1, heavy = -1, heavy_finish = -1, blk_write = -1, blk_or = -1;  // profile records to credit once the counts are known
   int kernels = 0;
 };
 
@@ -890,7 +928,7 @@ inline MsLaunched MsLaunchLevel(gg_graph* g, const MsBuffers& b, const uint64_t*
           static_cast<const MsLevel*>(b.L), b.dc, static_cast<const EdgeChunk*>(b.chunks));
       r.kernels++;
     }
-    Run(g, "msbfs_update", 0, MsUpdate, dim3(g->cu_count * 4), blk, static_cast<const int32_t*>(b.touched),
+    r.update = Run(g, "msbfs_update", 0, MsUpdate, dim3(g->cu_count * 4), blk, static_cast<const int32_t*>(b.touched),
         static_cast<const MsCounters*>(b.dc), b.acc, b.seen, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
         static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, act_next, b.dc);
     r.kernels++;
@@ -925,7 +963,7 @@ inline MsLaunched MsLaunchLevel(gg_graph* g, const MsBuffers& b, const uint64_t*
                   dim3(GridFor(int64_t(b.nhchunks) * kWave, kBS, g->cu_count * kLevelBlocksPerCu)), blk,
                   static_cast<const int32_t*>(g->in_dst), fcur, static_cast<const uint64_t*>(b.seen), b.acc,
                   static_cast<const MsLevel*>(b.L), b.hchunks, b.nhchunks, b.dc);
-    Run(g, "msbfs_heavy_finish", 32.0 * b.nheavy, MsHeavyFinish, dim3(GridFor(b.nheavy, kBS, g->cu_count * 4)), blk,
+    r.heavy_finish = Run(g, "msbfs_heavy_finish", 32.0 * b.nheavy, MsHeavyFinish, dim3(GridFor(b.nheavy, kBS, g->cu_count * 4)), blk,
         b.heavy, b.nheavy, b.seen, b.acc, fnext, b.depth16, static_cast<const MsLevel*>(b.L),
         static_cast<const int64_t*>(g->out_off), static_cast<const int64_t*>(g->in_off), b.full, b.dc);
     r.kernels += 3;
@@ -1144,7 +1182,9 @@ inline MsTraversal MsTraverse(gg_graph* g, const int32_t* droots, const int32_t*
       launches++;
       MailWait(g, seq, mail, kMailValues);
       if (r.td >= 0) CreditBytes(g, r.td, 20.0 * mail[kMailExamined]);
-      if (r.bu >= 0) CreditBytes(g, r.bu, 12.0 * mail[kMailExamined]);
+      if (r.bu >= 0) CreditBytes(g, r.bu, 12.0 * mail[kMailExamined] + mail[kMailDepthBottomUp]);
+      if (r.update >= 0) CreditBytes(g, r.update, static_cast<double>(mail[kMailDepthUpdate]));
+      if (r.heavy_finish >= 0) CreditBytes(g, r.heavy_finish, static_cast<double>(mail[kMailDepthHeavy]));
       if (r.heavy >= 0) CreditBytes(g, r.heavy, 12.0 * mail[kMailHeavyEx]);
       if (mail[kMailBlocksEx] && b.pb) {  // the blocks pass ran: its streams per padded entry, and 8 B per vertex
         const double ent = 8.0 * b.pb->octets;
