@@ -306,3 +306,48 @@ def test_pr_blocks_width_is_validated():
     finally:
         gpu.set_tuning(big, {"pr_blocks_log2": 0})
     assert gpu.pagerank(big, 20, 1e-4).stats["iterations"] > 0
+
+
+def test_pr_and_batch_bfs_blocks_build_splits_hubs_and_items():
+    # The layout build's work items: vertices 0 and 1 take about 75000
+    # in-edges each (lists of 2^15 entries or more are cut into chunks of
+    # 2^16, each its own build item: two each), and the other vertices of
+    # bin 0 (2^14 vertices at
+    # width 2^14) take ~100 in-edges each, 1.6 M entries in one bin: more
+    # than one vertex-range item of ~2^20 entries.  PageRank (both
+    # precisions) and the batched BFS through the blocks must meet the
+    # oracle.
+    print("[pr] blocks build: hub chunks and split bins", flush=True)
+    rng = np.random.default_rng(23)
+    n = 100000  # 140000 draws of a source keep ~75000 distinct ones per hub
+    hubs_u = rng.integers(0, n, 280000)
+    hubs_v = np.repeat(np.array([0, 1]), 140000)
+    mid_v = np.repeat(np.arange(2, 16384), 100)
+    mid_u = rng.integers(0, n, mid_v.size)
+    tail_u = rng.integers(0, n, 200000)
+    tail_v = rng.integers(16384, n, 200000)
+    u = np.concatenate([hubs_u, mid_u, tail_u])
+    v = np.concatenate([hubs_v, mid_v, tail_v])
+    g = Graph.from_edges(u, v, symmetrize=False)
+    assert np.diff(g.in_offsets)[:2].min() > 2 ** 16
+    dg = gpu.DeviceGraph(g)
+    want, it, _ = oracle.pagerank(g, 20, 0.0, fp64=True)
+    b32, b64, info = _blocked(dg, 14, 0.0)
+    assert info["width_log2"] == 14 and info["tiles"] == 7
+    assert b32.stats["iterations"] == it == 20
+    assert (np.abs(b32.scores.astype(np.float64) - want) / want).max() < 1e-5
+    np.testing.assert_allclose(b64.scores, want.astype(np.float32), rtol=1e-6, atol=0)
+    # The batched BFS's blocks level over the same layout (forced for any
+    # number of roots and any unfinished share).
+    gpu.set_tuning(dg, {"pr_blocks_log2": 14, "bfs_blocks_pct": 1})
+    roots = [0, 1, 5, 100, 20000, 99999, 12345, 7]
+    res = gpu.bfs_batch(dg, roots, 1000000)
+    dg.profile(True)
+    again = gpu.bfs_batch(dg, roots, 1000000)
+    rep = dg.profile_report()
+    dg.profile(False)
+    assert "msbfs_blocks_or" in rep and rep["msbfs_blocks_or"]["alg_bytes"] > 0
+    for i, r in enumerate(roots):
+        assert np.array_equal(res.depths[i], oracle.bfs_depth(g, r)), r
+        assert np.array_equal(again.depths[i], res.depths[i]), r
+    dg.free()
