@@ -448,18 +448,47 @@ __device__ __forceinline__ float AddT(float a, float b) { return __fadd_rn(a, b)
 __device__ __forceinline__ double AddT(double a, double b) { return __dadd_rn(a, b); }
 
 // score[u] = the deltas of the roots that reached u, added in root order.
+// delta holds one 64-entry row per vertex, so a thread summing its own row
+// would touch a different cache line in every lane of a load. Instead each
+// wave takes 64 vertices: it reads their rows one row per load (lane r reads
+// root r, only where root r reached that vertex) into an LDS tile whose rows
+// are padded to 65 entries, then lane j adds row j from LDS in root order,
+// the order of the one-thread loop, so the sums are unchanged bit for bit.
+constexpr int kMbScoreRow = 65;
+constexpr int kMbScoreBatch = 32;  // rows loaded before they are written to LDS
 template <class T>
 __global__ __launch_bounds__(kMbBS) void MbScores(const uint64_t* __restrict__ seen, const T* __restrict__ delta,
                                                   int64_t n, T* __restrict__ score) {
-  for (int64_t u = blockIdx.x * int64_t(kMbBS) + threadIdx.x; u < n; u += int64_t(gridDim.x) * kMbBS) {
-    uint64_t s = seen[u];
+  __shared__ T tile[kMbBS / kWave][kWave * kMbScoreRow];
+  const int lane = threadIdx.x & (kWave - 1);
+  T* row = tile[threadIdx.x / kWave];
+  for (int64_t base = blockIdx.x * int64_t(kMbBS); base < n; base += int64_t(gridDim.x) * kMbBS) {
+    const int64_t v0 = base + (threadIdx.x - lane);
+    const int64_t u = v0 + lane;
+    const uint64_t mine = u < n ? seen[u] : 0;
+    const uint32_t lo = static_cast<uint32_t>(mine), hi = static_cast<uint32_t>(mine >> 32);
+    for (int i0 = 0; i0 < kWave; i0 += kMbScoreBatch) {
+      T v[kMbScoreBatch];
+#pragma unroll
+      for (int i = 0; i < kMbScoreBatch; ++i) {
+        // Bit `lane` of seen[v0 + i0 + i]; rows past n have seen 0.
+        const uint32_t wlo = __builtin_amdgcn_readlane(lo, i0 + i), whi = __builtin_amdgcn_readlane(hi, i0 + i);
+        const bool on = ((lane < 32 ? wlo >> lane : whi >> (lane - 32)) & 1u) != 0;
+        v[i] = on ? delta[(v0 + i0 + i) * kWave + lane] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < kMbScoreBatch; ++i) row[(i0 + i) * kMbScoreRow + lane] = v[i];
+    }
+    __syncthreads();
     T acc = T(0);
+    uint64_t s = mine;
     while (s) {
       const int r = __builtin_ctzll(s);
       s &= s - 1;
-      acc = AddT(acc, delta[u * 64 + r]);
+      acc = AddT(acc, row[lane * kMbScoreRow + r]);
     }
-    score[u] = acc;
+    if (u < n) score[u] = acc;
+    __syncthreads();  // the next pass overwrites the tile
   }
 }
 
