@@ -46,6 +46,8 @@ constexpr int64_t kHubEdges = 256;   // sigma pull: vertices with this many in-e
 constexpr int64_t kWaveEdges = 64;   // delta: vertices with this many edges use a wavefront
 constexpr int64_t kBcHubEdges = 16384;  // ... and with this many, a block (BcDeltaHub)
 constexpr int kBcHubProducers = 15;
+constexpr int kBcHubUnroll = 2;  // 64-edge runs per producer wavefront and round
+constexpr int kBcHubGrid = 2;    // BcDeltaHub blocks per CU
 constexpr int kBcHubBS = (kBcHubProducers + 1) * kWave;
 
 struct BcCounters {
@@ -53,6 +55,7 @@ struct BcCounters {
   int32_t nchunks;  // discovery chunks
   int32_t nred;     // sigma reduction chunks
   int32_t nwave;    // delta: vertices handed to the wavefront kernel
+  int32_t nhub;     // delta: vertices handed to the hub kernel
   unsigned long long examined[kStripes];
 };
 
@@ -74,6 +77,7 @@ __global__ __launch_bounds__(kBS) void BcResetSource(int32_t* __restrict__ depth
       c->nchunks = 0;
       c->nred = 0;
       c->nwave = 0;
+      c->nhub = 0;
     }
   }
 }
@@ -266,25 +270,27 @@ __device__ __forceinline__ void AddScore(double* score, double*, int32_t u, doub
 }
 
 // Thread per vertex below kWaveEdges edges; larger vertices are listed for
-// BcDeltaWave.
+// BcDeltaWave, and those with kBcHubEdges or more for BcDeltaHub.
 template <class T, bool kKahan>
 __global__ __launch_bounds__(kBS) void BcDelta(const int64_t* __restrict__ off, const int32_t* __restrict__ dst,
                                                const int32_t* __restrict__ depth,
                                                const double* __restrict__ sigma, T* __restrict__ delta,
                                                T* __restrict__ score, T* __restrict__ comp,
                                                const int32_t* __restrict__ queue, int32_t qb, int32_t qn,
-                                               int32_t level, int32_t* __restrict__ wave_list, BcCounters* c,
+                                               int32_t level, int32_t* __restrict__ wave_list,
+                                               int32_t* __restrict__ hub_list, BcCounters* c,
                                                unsigned long long* work) {
   __shared__ unsigned long long s_red[kBS / kWave];
   unsigned long long ent = 0, mat = 0;
   for (int64_t base = int64_t(blockIdx.x) * kBS; base < qn; base += int64_t(gridDim.x) * kBS) {
     const int64_t i = base + threadIdx.x;
-    bool big = false;
+    bool big = false, hub = false;
     int32_t u = 0;
     if (i < qn) {
       u = queue[qb + i];
       const int64_t b = off[u], e = off[u + 1];
       big = e - b >= kWaveEdges;
+      hub = e - b >= kBcHubEdges;
       if (!big) {
         const double su = sigma[u];
         typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
@@ -301,7 +307,8 @@ __global__ __launch_bounds__(kBS) void BcDelta(const int64_t* __restrict__ off, 
         AddScore<kKahan>(score, comp, u, d);
       }
     }
-    WaveAppend(big, u, wave_list, &c->nwave);
+    WaveAppend(big && !hub, u, wave_list, &c->nwave);
+    WaveAppend(hub, u, hub_list, &c->nhub);
   }
   WorkAdd(work, BlockSum<kBS>(ent, s_red), BlockSum<kBS>(mat, s_red));
 }
@@ -324,7 +331,6 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
   for (int64_t i = wave; i < nw; i += nwaves) {
     const int32_t u = wave_list[i];
     const int64_t b = off[u], e = off[u + 1];
-    if (e - b >= kBcHubEdges) continue;  // BcDeltaHub's (uniform per wavefront)
     const double su = sigma[u];
     typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
     for (int64_t base = b; base < e; base += kWave) {
@@ -356,55 +362,63 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
   WorkAdd(work, BlockSum<kBS>(ent, s_red), BlockSum<kBS>(mat, s_red));
 }
 
-// Listed vertices with at least kBcHubEdges successors, one block each:
-// per round, producer wavefront p computes the terms of 64 consecutive
-// edges (b + round * 64 kBcHubProducers + 64 p + lane) and packs the
-// matched ones, in edge order, into one half of an LDS double buffer; the
-// last wavefront folds the previous round's terms in edge order.  Same
-// terms, same order, same arithmetic as BcDeltaWave; only the loads and
-// divisions run in parallel.
+// The hub list's vertices (at least kBcHubEdges successors), one block each:
+// per round, producer wavefront p computes the terms of 64 kBcHubUnroll
+// consecutive edges (from b + 64 kBcHubUnroll (round kBcHubProducers + p))
+// and packs the matched ones, in edge order, into one half of an LDS double
+// buffer; the last wavefront folds the previous round's terms in edge
+// order.  Same terms, same order, same arithmetic as BcDeltaWave; only the
+// loads and divisions run in parallel.  The fold (widen, add in double,
+// round to float per term, as the reference's float `+=` of a double term)
+// is a chain of three dependent double-precision operations per matched
+// edge, so the largest hubs set the kernel's time.
 template <class T, bool kKahan>
 __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict__ off,
                                                        const int32_t* __restrict__ dst,
                                                        const int32_t* __restrict__ depth,
                                                        const double* __restrict__ sigma, T* __restrict__ delta,
                                                        T* __restrict__ score, T* __restrict__ comp,
-                                                       const int32_t* __restrict__ wave_list, int32_t level,
+                                                       const int32_t* __restrict__ hub_list, int32_t level,
                                                        const BcCounters* c, unsigned long long* work) {
-  __shared__ double terms[2][kBcHubProducers][kWave];
+  __shared__ double terms[2][kBcHubProducers][kWave * kBcHubUnroll];
   __shared__ int counts[2][kBcHubProducers];
   __shared__ unsigned long long s_red[kBcHubBS / kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
   unsigned long long ent = 0, mat = 0;  // lane 0 of each producer's
-  const int32_t nw = c->nwave;
-  constexpr int64_t kRound = int64_t(kBcHubProducers) * kWave;
-  for (int32_t i = blockIdx.x; i < nw; i += gridDim.x) {
-    const int32_t u = wave_list[i];
+  const int32_t nh = c->nhub;
+  constexpr int64_t kRound = int64_t(kBcHubProducers) * kWave * kBcHubUnroll;
+  for (int32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+    const int32_t u = hub_list[i];
     const int64_t b = off[u], e = off[u + 1];
-    if (e - b < kBcHubEdges) continue;  // uniform per block
     const double su = sigma[u];
     typename AccFor<T, kKahan>::Type acc = AccFor<T, kKahan>::Type::Zero();
     const int64_t rounds = (e - b + kRound - 1) / kRound;
     for (int64_t k = 0; k <= rounds; k++) {
       if (wid < kBcHubProducers) {
         if (k < rounds) {
-          const int64_t kk = b + k * kRound + int64_t(wid) * kWave + lane;
-          double term = 0.0;
-          bool has = false;
-          if (kk < e) {
-            const int32_t v = dst[kk];
-            if (depth[v] == level + 1) {
-              term = DeltaTerm(su, sigma[v], delta[v]);
-              has = true;
-            }
+          // kBcHubUnroll consecutive runs of 64 edges, their loads issued together.
+          const int64_t k0 = b + k * kRound + int64_t(wid) * (kWave * kBcHubUnroll) + lane;
+          int32_t v[kBcHubUnroll];
+          bool has[kBcHubUnroll];
+#pragma unroll
+          for (int r = 0; r < kBcHubUnroll; r++) v[r] = k0 + r * kWave < e ? dst[k0 + r * kWave] : -1;
+#pragma unroll
+          for (int r = 0; r < kBcHubUnroll; r++) has[r] = v[r] >= 0 && depth[v[r]] == level + 1;
+          double term[kBcHubUnroll];
+#pragma unroll
+          for (int r = 0; r < kBcHubUnroll; r++) term[r] = has[r] ? DeltaTerm(su, sigma[v[r]], delta[v[r]]) : 0.0;
+          int pos = 0;
+#pragma unroll
+          for (int r = 0; r < kBcHubUnroll; r++) {
+            const uint64_t m = __ballot(has[r]);
+            if (has[r]) terms[k & 1][wid][pos + __popcll(LanesBelow(m))] = term[r];
+            pos += __popcll(m);
+            const uint64_t in = __ballot(k0 + r * kWave < e);
+            if (lane == 0) ent += static_cast<unsigned long long>(__popcll(in));
           }
-          const uint64_t m = __ballot(has);
-          const uint64_t in = __ballot(kk < e);
-          if (has) terms[k & 1][wid][__popcll(LanesBelow(m))] = term;
           if (lane == 0) {
-            counts[k & 1][wid] = __popcll(m);
-            mat += static_cast<unsigned long long>(__popcll(m));
-            ent += static_cast<unsigned long long>(__popcll(in));
+            counts[k & 1][wid] = pos;
+            mat += static_cast<unsigned long long>(pos);
           }
         }
       } else if (k >= 1) {
@@ -487,6 +501,7 @@ __global__ __launch_bounds__(kBS) void BcClearReduce(BcCounters* c) {
   if (threadIdx.x == 0) {
     c->nred = 0;
     c->nwave = 0;
+    c->nhub = 0;
   }
 }
 
@@ -573,6 +588,8 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
   T* score = g->Scratch<T>((std::string("bc.score") + tag).c_str(), n);
   T* comp = sizeof(T) == 4 ? g->Scratch<T>("bc.comp32", n) : nullptr;  // zero unless kKahan
   int32_t* wave_list = g->Scratch<int32_t>("bc.wave_list", n + 64);
+  // A vertex with kBcHubEdges successors owns that many list entries.
+  int32_t* hub_list = g->Scratch<int32_t>("bc.hub_list", g->out_entries / kBcHubEdges + 64);
   float* out32 = g->Scratch<float>("bc.out32", n);
   unsigned long long* max_bits = g->Scratch<unsigned long long>("bc.max", 1);
   BcCounters* dc = g->Scratch<BcCounters>("bc.ctr", 1);
@@ -705,7 +722,7 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
       note(kWDelta, Run(g, "bc_delta", (16.0 + 2 * sizeof(T) + 8.0) * qn, BcDelta<T, kKahan>,
                         dim3(GridFor(qn, kBS, g->cu_count * 16)), blk, static_cast<const int64_t*>(g->out_off),
                         fold_dst, static_cast<const int32_t*>(depth), static_cast<const double*>(sigma), delta,
-                        score, comp, static_cast<const int32_t*>(queue), qb, qn, d, wave_list, dc,
+                        score, comp, static_cast<const int32_t*>(queue), qb, qn, d, wave_list, hub_list, dc,
                         wk(kWDelta)));
       launches++;
       if (g->max_degree >= kWaveEdges) {
@@ -718,9 +735,9 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
           GG_HIP(hipEventRecord(g->ev_fork, st));
           GG_HIP(hipStreamWaitEvent(g->side, g->ev_fork, 0));
           note(kWDeltaHub,
-               RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count), dim3(kBcHubBS),
+               RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count * kBcHubGrid), dim3(kBcHubBS),
                      static_cast<const int64_t*>(g->out_off), fold_dst, static_cast<const int32_t*>(depth),
-                     static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(wave_list),
+                     static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(hub_list),
                      d, static_cast<const BcCounters*>(dc), wk(kWDeltaHub)));
           GG_HIP(hipEventRecord(g->ev_join, g->side));
           launches++;
