@@ -56,6 +56,7 @@ struct BcCounters {
   int32_t nred;     // sigma reduction chunks
   int32_t nwave;    // delta: vertices handed to the wavefront kernel
   int32_t nhub;     // delta: vertices handed to the hub kernel
+  int32_t hub_next; // BcDeltaHub: next hub list entry to take
   unsigned long long examined[kStripes];
 };
 
@@ -78,6 +79,7 @@ __global__ __launch_bounds__(kBS) void BcResetSource(int32_t* __restrict__ depth
       c->nred = 0;
       c->nwave = 0;
       c->nhub = 0;
+      c->hub_next = 0;
     }
   }
 }
@@ -371,7 +373,10 @@ __global__ __launch_bounds__(kBS) void BcDeltaWave(const int64_t* __restrict__ o
 // loads and divisions run in parallel.  The fold (widen, add in double,
 // round to float per term, as the reference's float `+=` of a double term)
 // is a chain of three dependent double-precision operations per matched
-// edge, so the largest hubs set the kernel's time.
+// edge, so the largest hubs set the kernel's time.  Blocks take hubs from
+// a counter, so a block held by a large hub takes no others meanwhile (the
+// list follows the level's queue, which on a degree-ordered layout runs
+// roughly from the largest hub down).
 template <class T, bool kKahan>
 __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict__ off,
                                                        const int32_t* __restrict__ dst,
@@ -379,15 +384,21 @@ __global__ __launch_bounds__(kBcHubBS) void BcDeltaHub(const int64_t* __restrict
                                                        const double* __restrict__ sigma, T* __restrict__ delta,
                                                        T* __restrict__ score, T* __restrict__ comp,
                                                        const int32_t* __restrict__ hub_list, int32_t level,
-                                                       const BcCounters* c, unsigned long long* work) {
+                                                       BcCounters* c, unsigned long long* work) {
   __shared__ double terms[2][kBcHubProducers][kWave * kBcHubUnroll];
   __shared__ int counts[2][kBcHubProducers];
+  __shared__ int32_t s_take;
   __shared__ unsigned long long s_red[kBcHubBS / kWave];
   const int lane = LaneId(), wid = threadIdx.x / kWave;
   unsigned long long ent = 0, mat = 0;  // lane 0 of each producer's
   const int32_t nh = c->nhub;
   constexpr int64_t kRound = int64_t(kBcHubProducers) * kWave * kBcHubUnroll;
-  for (int32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) s_take = atomicAdd(&c->hub_next, 1);
+    __syncthreads();
+    const int32_t i = s_take;
+    __syncthreads();  // read by all before the next take overwrites it
+    if (i >= nh) break;
     const int32_t u = hub_list[i];
     const int64_t b = off[u], e = off[u + 1];
     const double su = sigma[u];
@@ -502,6 +513,7 @@ __global__ __launch_bounds__(kBS) void BcClearReduce(BcCounters* c) {
     c->nred = 0;
     c->nwave = 0;
     c->nhub = 0;
+    c->hub_next = 0;
   }
 }
 
@@ -738,7 +750,7 @@ void RunBrandes(gg_graph* g, const int32_t* roots, int k, float* scores_out, gg_
                RunOn(g, g->side, "bc_delta_hub", 0, BcDeltaHub<T, kKahan>, dim3(g->cu_count * kBcHubGrid), dim3(kBcHubBS),
                      static_cast<const int64_t*>(g->out_off), fold_dst, static_cast<const int32_t*>(depth),
                      static_cast<const double*>(sigma), delta, score, comp, static_cast<const int32_t*>(hub_list),
-                     d, static_cast<const BcCounters*>(dc), wk(kWDeltaHub)));
+                     d, dc, wk(kWDeltaHub)));
           GG_HIP(hipEventRecord(g->ev_join, g->side));
           launches++;
         }
