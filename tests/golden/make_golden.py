@@ -4,8 +4,8 @@ sources by `make -C oracle ref`) and the reference's own test fixtures.
 Run in a container that has /root/reference:  python tests/golden/make_golden.py
 The JSON holds data only: expected stats lines, hashes of the reference's
 serialized graphs, SourcePicker root sequences and per-root result digests
-of the reference kernels, so the GPU box (which has no reference) can pin the
-oracle and the GPU path against them.
+of the reference kernels, so a machine without the reference sources can pin
+the oracle and the GPU path against them.
 """
 
 import hashlib
